@@ -1,0 +1,429 @@
+"""Python face of the CPU oracle.  TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this module, and only as the checker: the product
+library never routes through it.
+
+Two layers:
+  * ctypes wrappers over ``oracle/_build/libzkx_oracle.so`` (``zkx_oracle.c``),
+    the faithful restatement of the reference CPU functions, pinned bit-for-bit
+    against the reference's recorded probe digests;
+  * numpy FP64 FFT forms of the same maths (``*_fft64``), used as the fast FP64
+    ground truth at full sizes, pinned against the C direct sums at small sizes.
+
+Reference functions followed are cited at each wrapper (paths relative to the
+reference repository).
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libzkx_oracle.so")
+
+WINDOWS = {"none": 0, "hann": 1, "hamming": 2, "explicit_fp32": 3, "explicit_fp64": 4}
+
+_lib = None
+
+
+def build():
+    """Compile the C oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i, d, sz, u64 = ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_uint64
+        sig = {
+            "orc_probe_complex_input": (None, [sz, i, i, P]),
+            "orc_probe_real_input": (None, [sz, i, i, P]),
+            "orc_upfirdn_probe_input": (None, [sz, u64, i, P]),
+            "orc_resample_probe_input": (None, [sz, u64, i, P]),
+            "orc_firfilter_probe_signal": (None, [i, i, i, P]),
+            "orc_window": (None, [i, i, i, P]),
+            "orc_pulse_compression": (i, [P, i, i, P, i, i, i, P, i, i, P]),
+            "orc_pulse_doppler": (i, [P, i, i, i, i, P, i, P]),
+            "orc_cfar_alpha": (d, [d, i]),
+            "orc_ca_cfar_count": (i, [i, i, i, i, i]),
+            "orc_ca_cfar_alpha_f": (ctypes.c_float, [i, i, i, i, i, d]),
+            "orc_ca_cfar": (i, [P, P, d, d, i, i, i, i, i, i, i, d, P, P]),
+            "orc_ca_cfar_exact": (i, [P, i, i, i, i, i, i, d, P]),
+            "orc_upfirdn_out_len": (ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong, i, i]),
+            "orc_upfirdn": (i, [P, i, P, i, i, i, i, i, i, P]),
+            "orc_resample_plan_fill": (i, [i, i, i, i, i, P]),
+            "orc_resample_bessel_i0": (d, [d]),
+            "orc_resample_design": (i, [i, i, i, d, P, P]),
+            "orc_resample_poly": (i, [P, i, P, i, i, i, i, i, i, P]),
+            "orc_firfilter": (i, [P, i, P, i, i, i, P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def sha256(arr) -> str:
+    return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
+
+
+# --------------------------------------------------------------------------
+# probe generators (reference probe programs, see zkx_oracle.c header)
+# --------------------------------------------------------------------------
+INTEGRAL = {"FP32": False, "FP16": False, "INT32": True, "INT16": True, "INT8": True}
+NP_DTYPE = {"FP32": np.float32, "FP16": np.float16, "INT32": np.int32,
+            "INT16": np.int16, "INT8": np.int8}
+
+
+def probe_complex_input(n, seed, dtype):
+    out = np.empty(2 * n, np.float32)
+    lib().orc_probe_complex_input(n, seed, INTEGRAL[dtype], _p(out))
+    return out.astype(NP_DTYPE[dtype])  # interleaved (re, im) of the typed dtype
+
+
+def probe_real_input(n, seed, dtype):
+    out = np.empty(n, np.float32)
+    lib().orc_probe_real_input(n, seed, INTEGRAL[dtype], _p(out))
+    return out.astype(NP_DTYPE[dtype])
+
+
+def upfirdn_probe_input(n, seed, dtype):
+    out = np.empty(n, np.float32)
+    lib().orc_upfirdn_probe_input(n, seed, INTEGRAL[dtype], _p(out))
+    return out.astype(NP_DTYPE[dtype])
+
+
+def resample_probe_input(n, seed, dtype):
+    out = np.empty(n, np.float32)
+    lib().orc_resample_probe_input(n, seed, INTEGRAL[dtype], _p(out))
+    return out.astype(NP_DTYPE[dtype])
+
+
+def firfilter_probe_signal(n, seed, dtype):
+    out = np.empty(n, np.float64)
+    lib().orc_firfilter_probe_signal(n, seed, INTEGRAL[dtype], _p(out))
+    if dtype == "FP16":  # task2_filter_probe.cu cv<__half>: double -> float -> half
+        return out.astype(np.float32).astype(np.float16)
+    return out.astype(NP_DTYPE[dtype])
+
+
+# --------------------------------------------------------------------------
+# radar ops
+# --------------------------------------------------------------------------
+def window(kind, length, exact=False):
+    out = np.empty(length, np.float64)
+    lib().orc_window(WINDOWS[kind], length, int(exact), _p(out))
+    return out
+
+
+def _as_c64_f32(x):
+    """complex array -> interleaved float32 (values the reference loads)."""
+    x = np.asarray(x)
+    if np.iscomplexobj(x):
+        v = np.empty(x.shape + (2,), np.float32)
+        v[..., 0] = x.real
+        v[..., 1] = x.imag
+        return np.ascontiguousarray(v.reshape(-1))
+    return np.ascontiguousarray(x.astype(np.float32).reshape(-1))
+
+
+def pulse_compression(x, tmpl, nfft=-1, window="none", normalize=False, exact=False,
+                      explicit_window=None):
+    """radartools_typed.cpp:618-675 pulse_compression_typed_cpu (direct circular
+    correlation in FP64).  x: complex [P, S]; tmpl: complex [T]. -> complex128 [P, nfft]."""
+    x = np.asarray(x)
+    P, S = x.shape
+    T = np.asarray(tmpl).shape[0]
+    n = S if nfft == -1 else nfft
+    out = np.empty(2 * P * n, np.float64)
+    ew = None if explicit_window is None else np.ascontiguousarray(explicit_window, np.float32)
+    rc = lib().orc_pulse_compression(_p(_as_c64_f32(x)), P, S, _p(_as_c64_f32(tmpl)), T, nfft,
+                                     WINDOWS[window], _p(ew), int(normalize), int(exact), _p(out))
+    if rc:
+        raise ValueError(f"pulse_compression rc={rc}")
+    return out.view(np.complex128).reshape(P, n)
+
+
+def pulse_doppler(x, nfft=-1, window="none", exact=False, explicit_window=None):
+    """radartools_typed.cpp:714-759 pulse_doppler_typed_cpu (direct DFT along
+    pulses; faithful mode keeps the float pi).  x: complex [P, R] -> complex128 [nfft, R]."""
+    x = np.asarray(x)
+    P, R = x.shape
+    n = P if nfft == -1 else nfft
+    out = np.empty(2 * n * R, np.float64)
+    ew = None if explicit_window is None else np.ascontiguousarray(explicit_window, np.float32)
+    rc = lib().orc_pulse_doppler(_p(_as_c64_f32(x)), P, R, nfft, WINDOWS[window], _p(ew),
+                                 int(exact), _p(out))
+    if rc:
+        raise ValueError(f"pulse_doppler rc={rc}")
+    return out.view(np.complex128).reshape(n, R)
+
+
+def cfar_alpha(pfa, reference):
+    """radartools_typed.cpp:592-603 cfar_alpha_typed_cpu (FP64)."""
+    return lib().orc_cfar_alpha(float(pfa), int(reference))
+
+
+def ca_cfar_count(guard, ref):
+    if len(guard) == 1:
+        return 2 * ref[0]
+    return lib().orc_ca_cfar_count(2, guard[0], guard[1], ref[0], ref[1])
+
+
+def ca_cfar_alpha_f(guard, ref, pfa):
+    rank = len(guard)
+    g1 = guard[1] if rank == 2 else 0
+    r1 = ref[1] if rank == 2 else 0
+    return np.float32(lib().orc_ca_cfar_alpha_f(rank, guard[0], g1, ref[0], r1, float(pfa)))
+
+
+def ca_cfar(x, guard, ref, pfa):
+    """radartools_typed.cpp:465-555 ca_cfar_typed_cpu: FP32 integral image
+    (axis 0 then axis 1), border threshold 0, strict '>' detection.
+    x: real array, rank 1 or 2 (integer dtypes use the integer detection rule)."""
+    x = np.asarray(x)
+    rank = x.ndim
+    rows = x.shape[0]
+    cols = x.shape[1] if rank == 2 else 1
+    ints = None
+    lo = hi = 0.0
+    if np.issubdtype(x.dtype, np.integer):
+        ints = np.ascontiguousarray(x.astype(np.int64).reshape(-1))
+        info = np.iinfo(x.dtype)
+        lo, hi = float(info.min), float(info.max)
+    xf = np.ascontiguousarray(x.astype(np.float32).reshape(-1))
+    thr = np.empty(xf.size, np.float32)
+    det = np.empty(xf.size, np.uint8)
+    g1 = guard[1] if rank == 2 else 0
+    r1 = ref[1] if rank == 2 else 0
+    rc = lib().orc_ca_cfar(_p(xf), _p(ints), lo, hi, rank, rows, cols, guard[0], g1, ref[0],
+                           r1, float(pfa), _p(thr), _p(det))
+    if rc:
+        raise ValueError(f"ca_cfar rc={rc}")
+    return thr.reshape(x.shape), det.reshape(x.shape)
+
+
+def ca_cfar_exact(x, guard, ref, pfa):
+    """FP64 direct box sums with an FP64 alpha (the exact statistic)."""
+    x = np.ascontiguousarray(np.asarray(x, np.float32))
+    rows, cols = x.shape
+    thr = np.empty(x.size, np.float64)
+    lib().orc_ca_cfar_exact(_p(x), rows, cols, guard[0], guard[1], ref[0], ref[1],
+                            float(pfa), _p(thr))
+    return thr.reshape(rows, cols)
+
+
+# --------------------------------------------------------------------------
+# filtering ops
+# --------------------------------------------------------------------------
+def _lines(shape, axis):
+    shape = list(shape)
+    axis = axis + len(shape) if axis < 0 else axis
+    outer = int(np.prod(shape[:axis])) if axis > 0 else 1
+    inner = int(np.prod(shape[axis + 1:])) if axis + 1 < len(shape) else 1
+    return outer, shape[axis], inner, axis
+
+
+def upfirdn(h, x, up=1, down=1, axis=-1, fp64_acc=False):
+    """filtering_typed.cpp:2548-2616 upfirdn_typed_cpu.  Float path sums float
+    products in float; fp64_acc is the INT32 path.  Returns float64 values."""
+    x = np.asarray(x)
+    h = np.ascontiguousarray(np.asarray(h, np.float64).reshape(-1))
+    outer, n, inner, ax = _lines(x.shape, axis)
+    out_n = lib().orc_upfirdn_out_len(n, h.size, up, down)
+    shape = list(x.shape)
+    shape[ax] = out_n
+    out = np.empty(int(np.prod(shape)), np.float64)
+    rc = lib().orc_upfirdn(_p(h), h.size, _p(np.ascontiguousarray(x.astype(np.float64).reshape(-1))),
+                           outer, n, inner, up, down, int(fp64_acc), _p(out))
+    if rc:
+        raise ValueError(f"upfirdn rc={rc}")
+    return out.reshape(shape)
+
+
+def resample_plan(n, up, down, custom_count=None):
+    f = np.empty(8, np.int32)
+    rc = lib().orc_resample_plan_fill(n, up, down, int(custom_count is not None),
+                                      custom_count or 0, _p(f))
+    if rc:
+        raise ValueError(f"resample_poly rc={rc}")
+    keys = ["up", "down", "identity", "half_length", "coefficient_count", "prefix_zeros",
+            "output_start", "output_count"]
+    return dict(zip(keys, map(int, f)))
+
+
+def resample_design(up, down, window_mode="kaiser", beta=5.0, design_window=None):
+    """filtering_typed.cpp:176-218 design_resample_poly_coefficients (FP64)."""
+    plan = resample_plan(1, up, down)
+    out = np.empty(plan["coefficient_count"], np.float64)
+    dw = None if design_window is None else np.ascontiguousarray(design_window, np.float64)
+    mode = {"kaiser": 0, "hamming": 1, "explicit": 2}[window_mode]
+    lib().orc_resample_design(up, down, mode, float(beta), _p(dw), _p(out))
+    return out
+
+
+def resample_poly_core(taps, x, up, down, axis, fp64_acc):
+    """Polyphase loop of filtering_typed.cpp:2650-2712 given scaled taps."""
+    x = np.asarray(x)
+    taps = np.ascontiguousarray(np.asarray(taps, np.float64))
+    outer, n, inner, ax = _lines(x.shape, axis)
+    plan = resample_plan(n, up, down, taps.size)
+    shape = list(x.shape)
+    shape[ax] = plan["output_count"]
+    out = np.empty(int(np.prod(shape)), np.float64)
+    rc = lib().orc_resample_poly(_p(taps), taps.size,
+                                 _p(np.ascontiguousarray(x.astype(np.float64).reshape(-1))),
+                                 outer, n, inner, up, down, int(fp64_acc), _p(out))
+    if rc:
+        raise ValueError(f"resample_poly rc={rc}")
+    return out.reshape(shape)
+
+
+def resample_poly_f32(x, up, down, axis=0, taps=None, window_mode="kaiser", beta=5.0):
+    """resample_poly_typed_cpu for float32 input.  Custom taps: taps*up in
+    float, FP32 polyphase sum, FP32 result.  Designed taps: FP64 design and
+    FP64 sum (FP64 result).  Returns (values, dtype_name)."""
+    x = np.asarray(x, np.float32)
+    outer, n, inner, ax = _lines(x.shape, axis)
+    plan = resample_plan(n, up, down, None if taps is None else len(taps))
+    if plan["identity"]:
+        return x.astype(np.float64), "FP32"
+    if taps is not None:
+        t = np.asarray(taps, np.float32)
+        if plan["up"] < 65536:
+            scaled = (t * np.float32(plan["up"])).astype(np.float64)
+        else:
+            scaled = t.astype(np.float64) * plan["up"]
+        return resample_poly_core(scaled, x, up, down, axis, False), "FP32"
+    coeffs = resample_design(up, down, window_mode, beta)
+    return resample_poly_core(coeffs, x, up, down, axis, True), "FP64"
+
+
+def _scaled_custom(value, up, dtype):
+    """filtering_typed.cpp:129-157 resample_poly_scaled_custom_value."""
+    def wrap(v, bits):
+        v = int(v) & ((1 << bits) - 1)
+        return v - (1 << bits) if v >= 1 << (bits - 1) else v
+    if dtype == "FP32":
+        return float(np.float32(value) * np.float32(up)) if up < 65536 else float(value) * up
+    if dtype == "FP16":
+        loaded = np.float32(np.float16(value))
+        if up <= 255:
+            return float(np.float32(np.float16(loaded * np.float32(up))))
+        return float(loaded * np.float32(up)) if up < 65536 else float(loaded) * up
+    v = int(value)
+    if dtype == "INT32":
+        return float(wrap(v * up, 32))
+    if dtype == "INT16":
+        return float(wrap(v * up, 16 if up < 32768 else 32))
+    if up <= 127:
+        return float(wrap(v * up, 8))
+    return float(wrap(v * up, 16 if up < 32768 else 32))
+
+
+def resample_poly_typed(x, dtype, up, down, axis=0, taps=None, window_mode="kaiser", beta=5.0):
+    """resample_poly_typed_cpu (filtering_typed.cpp:2617-2715) for any of the
+    five input dtypes.  Returns (values as float64, output dtype name)."""
+    x = np.asarray(x)
+    outer, n, inner, ax = _lines(x.shape, axis)
+    plan = resample_plan(n, up, down, None if taps is None else len(taps))
+    if plan["identity"]:
+        return x.astype(np.float64), dtype
+    if taps is None:
+        return resample_poly_core(resample_design(up, down, window_mode, beta), x, up, down,
+                                  axis, True), "FP64"
+    scaled = np.array([_scaled_custom(v, plan["up"], dtype) for v in np.asarray(taps).tolist()])
+    if dtype == "INT32":
+        fp32 = False
+    elif dtype in ("INT16", "INT8"):
+        fp32 = plan["up"] < 32768
+    else:
+        fp32 = plan["up"] < 65536
+    if fp32:
+        scaled = scaled.astype(np.float32).astype(np.float64)
+    return resample_poly_core(scaled, x, up, down, axis, not fp32), "FP32" if fp32 else "FP64"
+
+
+def firfilter(b, x, axis=-1, zi=None):
+    """filtering_typed.cpp:1536-1592 firfilter_typed_cpu (FP64 accumulate -> f32).
+    zi, if given, must already have the state shape (axis length K-1).
+    Returns (y float32, zf float32 or None)."""
+    x = np.asarray(x)
+    b = np.ascontiguousarray(np.asarray(b, np.float64).reshape(-1))
+    outer, n, inner, ax = _lines(x.shape, axis)
+    y = np.empty(x.size, np.float32)
+    zf = None
+    zi_arr = None
+    if zi is not None:
+        zshape = list(x.shape)
+        zshape[ax] = b.size - 1
+        zi_arr = np.ascontiguousarray(np.broadcast_to(np.asarray(zi, np.float64), zshape).reshape(-1))
+        zf = np.empty(int(np.prod(zshape)), np.float32)
+    rc = lib().orc_firfilter(_p(b), b.size, _p(np.ascontiguousarray(x.astype(np.float64).reshape(-1))),
+                             outer, n, inner, _p(zi_arr), _p(y), _p(zf))
+    if rc:
+        raise ValueError(f"firfilter rc={rc}")
+    if zf is not None:
+        zshape = list(x.shape)
+        zshape[ax] = b.size - 1
+        zf = zf.reshape(zshape)
+    return y.reshape(x.shape), zf
+
+
+# --------------------------------------------------------------------------
+# FP64 FFT forms (numpy), the fast exact ground truth at full sizes
+# --------------------------------------------------------------------------
+def pulse_compression_fft64(x, tmpl, nfft=-1, window="none", normalize=False):
+    """Exact-mode pulse compression via FP64 FFTs: circular correlation of the
+    zero-padded/truncated pulse with the windowed, normalised template."""
+    x = np.asarray(x, np.complex128)
+    P, S = x.shape
+    n = S if nfft == -1 else nfft
+    t = np.asarray(tmpl, np.complex128) * window_exact(window, len(tmpl))
+    if normalize:
+        t = t / math.sqrt(float(np.sum(np.abs(t) ** 2)))
+    t = t[:n]
+    xp = np.zeros((P, n), np.complex128)
+    m = min(S, n)
+    xp[:, :m] = x[:, :m]
+    H = np.fft.fft(t, n)
+    return np.fft.ifft(np.fft.fft(xp, axis=1) * np.conj(H)[None, :], axis=1)
+
+
+def pulse_doppler_fft64(x, nfft=-1, window="none"):
+    """Exact-mode Doppler processing via an FP64 FFT along pulses."""
+    x = np.asarray(x, np.complex128)
+    P, R = x.shape
+    n = P if nfft == -1 else nfft
+    w = window_exact(window, P)
+    xw = x * w[:, None]
+    used = min(P, n)
+    return np.fft.fft(xw[:used], n, axis=0)
+
+
+def window_exact(kind, length):
+    return window(kind, length, exact=True)
+
+
+def rel_l2(a, ref, floor=1e-12):
+    """ZKX/test_all/support/metrics/accuracy.cpp:7-23: ||a-r||_2 / max(||r||_2, floor)."""
+    a = np.asarray(a).astype(np.complex128).reshape(-1)
+    r = np.asarray(ref).astype(np.complex128).reshape(-1)
+    den = max(float(np.linalg.norm(r)), floor)
+    return float(np.linalg.norm(a - r)) / den
