@@ -1,0 +1,192 @@
+"""Radar dwell throughput on one MI355X.
+
+Workload (the reference's plain single-dwell configuration, run in batches):
+complex64 dwells of 128 pulses x 4096 range samples, a 256-tap LFM template
+(BW/fs = 0.5) with a Hamming window and normalisation, a 128-point Doppler FFT
+with a Hann window, |.|^2, and 2-D CA-CFAR with guard 2x2, reference 8x8,
+pfa = 1e-6.  Inputs are seeded synthetic dwells (5 point targets at -10 dB
+per-sample SNR) resident in HBM before timing starts.
+
+A step is one pass of the fused pipeline over a batch of B dwells.  The JSON
+line reports whole-job dwells/s of the batched pipeline (`value`), the plain
+per-dwell drop-in (one operator call per step, as the reference's Task1 chain
+makes them) beside it, p50/p99 latency, the roofline of the dominant kernel
+measured with events on the launch stream, and the CPU oracle timed on the
+host.
+
+    python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+P, R, TAPS = 128, 4096, 256
+GUARD, REF, PFA = (2, 2), (8, 8), 1e-6
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--cfar-mode", type=int, default=0)
+    ap.add_argument("--plain-dwells", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def percentile(v, q):
+    return float(np.percentile(np.asarray(v), q))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from zkx_dev_amd import radar, synth
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    B = args.batch
+    cubes, tmpl, _ = synth.make_dwell_batch(B, P, R, TAPS, seed=20261203)
+    x = torch.from_numpy(cubes).to(dev)
+    t = torch.from_numpy(tmpl).to(dev)
+    del cubes
+    pipe = radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
+                               guard=GUARD, reference=REF, pfa=PFA, cfar_mode=args.cfar_mode,
+                               max_batch=B)
+    thr = torch.empty((B, P, R), dtype=torch.float32, device=dev)
+    det = torch.empty((B, P, R), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    # ---- batched fused pipeline ------------------------------------------
+    for _ in range(args.warmup):
+        pipe.run(x, threshold=thr, detections=det)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record(stream)
+        pipe.run(x, threshold=thr, detections=det)
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [a.elapsed_time(b) for a, b in ev]
+    dwells_per_s = B * args.steps / wall
+    print(f"# batched: {dwells_per_s:.1f} dwells/s, step p50 {percentile(step_ms, 50):.3f} ms",
+          file=sys.stderr, flush=True)
+
+    # ---- per-kernel timing of the stages (events on the launch stream) -------
+    ws = pipe.workspace
+    pc_out = torch.empty((B, P, R), dtype=torch.complex64, device=dev)
+    pw = torch.empty((B, P, R), dtype=torch.float32, device=dev)
+    reps = max(5, args.steps // 2)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3  # seconds per launch
+
+    t_pc = timed(lambda: radar.pulse_compression_prepared(x, pipe.spectrum, out=pc_out))
+    t_dop = timed(lambda: radar.pulse_doppler(pc_out[0], window="hann", out=pc_out[1],
+                                              power=pw[0]))
+    cells = B * P * R
+    bytes_pc = cells * 16 + R * 8
+    stages = {"pc_fused_4096": (t_pc, bytes_pc)}
+    del ws
+
+    # ---- plain per-dwell drop-in ---------------------------------------------
+    nplain = min(args.plain_dwells, B)
+    for i in range(2):
+        y = radar.pulse_compression(x[i], t, normalize=True, window="hamming", widen=False)
+        pwr = radar.doppler_power(y, window="hann")
+        radar.ca_cfar(pwr, GUARD, REF, PFA)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(nplain):
+        a = time.perf_counter()
+        y = radar.pulse_compression(x[i], t, normalize=True, window="hamming", widen=False)
+        pwr = radar.doppler_power(y, window="hann")
+        radar.ca_cfar(pwr, GUARD, REF, PFA)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    plain_wall = time.perf_counter() - t0
+    plain_dps = nplain / plain_wall
+
+    # ---- dominant kernel roofline ---------------------------------------------
+    name, (t_k, bytes_k) = max(stages.items(), key=lambda kv: kv[1][0])
+    achieved = bytes_k / t_k
+    roofline = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": None, "avg_launch_us": t_k * 1e6, "alg_bytes_per_launch": bytes_k}
+    pipe_bytes = B * (13 * P * R + 8 * TAPS)
+    step_s = wall / args.steps
+
+    # ---- CPU oracle on the host --------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import oracle as orc
+        d = synth.make_dwell(P, R, TAPS, seed=1)
+        c0 = time.perf_counter()
+        pc = orc.pulse_compression(d.x, d.template, window="hamming", normalize=True)
+        rd = orc.pulse_doppler(pc.astype(np.complex64), window="hann")
+        pwr_h = (np.abs(rd) ** 2).astype(np.float32)
+        orc.ca_cfar(pwr_h, GUARD, REF, PFA)
+        cpu_s = time.perf_counter() - c0
+        cpu = {"value": 1.0 / cpu_s, "unit": "dwells/s", "cores": 1, "kind": "port",
+               "sample": "1 dwell 128x4096: direct FP64 circular correlation (256 taps), direct "
+                         "DFT Doppler, FP32 integral-image CFAR (the reference *_typed_cpu chain)",
+               "seconds": cpu_s}
+
+    out = {
+        "metric": "range-Doppler dwells/s + Msamples/s (HBM roofline %) and p99 latency",
+        "value": dwells_per_s,
+        "unit": "dwells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded LFM point targets + complex Gaussian noise, -10 dB SNR)",
+        "config": {"workload": "dwell 128x4096 c64, LFM 256 taps Hamming, Doppler 128 Hann, "
+                               "CA-CFAR g2x2 r8x8 pfa 1e-6", "batch_dwells": B,
+                   "cfar_mode": "reference-identical integral image" if args.cfar_mode == 0
+                   else "FP64 windowed"},
+        "msamples_per_s": dwells_per_s * P * R / 1e6,
+        "pipeline_hbm_frac": pipe_bytes / step_s / HBM_PEAK,
+        "latency_ms": {"batch_p50": percentile(step_ms, 50), "batch_p99": percentile(step_ms, 99),
+                       "plain_dwell_p50": percentile(lat, 50), "plain_dwell_p99": percentile(lat, 99)},
+        "plain_dropin_dwells_per_s": plain_dps,
+        "speedup_vs_plain": dwells_per_s / plain_dps,
+        "stage_us": {"pc_fused": t_pc * 1e6, "doppler_power_one_dwell": t_dop * 1e6},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

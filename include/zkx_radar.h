@@ -1,0 +1,189 @@
+/*
+ * zkx_radar.h -- C ABI of the gfx950 radar library (libzkx_radar.so).
+ *
+ * Every entry point takes plain device pointers, sizes and POD options, runs
+ * on the given HIP stream (NULL = the default stream, as the reference used)
+ * and returns 0 or one of the reference's public ErrorCode numbers
+ * (ZKX/cusignal_cpp/include/cusignal/runtime/errors.h:18-41).  No torch or
+ * HIP C++ types appear in the signatures; a stream is passed as void*.
+ *
+ * Complex data is interleaved {float re, im}, the layout of the reference's
+ * ComplexFloat / TypedComplex<float> (fft_interface.h:104-117,
+ * filtering_typed.h:13).  Arrays are row-major.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository).  Validation follows the reference's own
+ * checks and returns ZKX_INVALID_SHAPE / ZKX_INVALID_ARGUMENT where the
+ * reference throws std::invalid_argument; the reason is readable through
+ * zkx_last_error_message().
+ */
+#ifndef ZKX_RADAR_H
+#define ZKX_RADAR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes: the reference's ErrorCode values. */
+#define ZKX_SUCCESS 0
+#define ZKX_INVALID_ARGUMENT 1001
+#define ZKX_INVALID_SHAPE 1002
+#define ZKX_INVALID_DTYPE 1003
+#define ZKX_UNSUPPORTED_OPERATION 1004
+#define ZKX_RUNTIME_ERROR 2002
+#define ZKX_OUT_OF_MEMORY 2003
+#define ZKX_BACKEND_ERROR 2099
+#define ZKX_INTERNAL_ERROR 9001
+
+/* RadarWindowKind (radartools_typed.h:59-65), same numbering. */
+#define ZKX_WINDOW_NONE 0
+#define ZKX_WINDOW_HANN 1
+#define ZKX_WINDOW_HAMMING 2
+#define ZKX_WINDOW_EXPLICIT_FP32 3
+#define ZKX_WINDOW_EXPLICIT_FP64 4
+
+typedef struct {
+    float re;
+    float im;
+} zkx_complex64;
+
+typedef struct {
+    double re;
+    double im;
+} zkx_complex128;
+
+/* ---------------------------------------------------------------------- */
+/* library                                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* zkx_last_error_message(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int zkx_version(void);
+/* Largest power-of-two transform length one LDS-resident pass supports. */
+int zkx_max_lds_fft_length(void);
+
+/* ---------------------------------------------------------------------- */
+/* radar operators                                                          */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Pulse compression, complex64 in and out.
+ * Replaces pulse_compression_device<float> (radartools_typed.h:237-262,
+ * radartools_typed.cpp:677-712, radartools_typed.cu:184-223).
+ *   x        [num_pulses, samples_per_pulse]
+ *   tmpl     [template_len]
+ *   out      [num_pulses, nfft]   (nfft = -1 means samples_per_pulse)
+ *   y[p,k] = sum_{t<min(T,nfft)} x[p,(k+t) mod nfft] * conj(h~[t]),
+ *   h~ = w*h (symmetric window over template_len), divided by ||w*h||_2 when
+ *   normalize != 0.  explicit_window: device float[template_len] for the
+ *   explicit kinds, else NULL.  nfft must be a power of two no larger than
+ *   zkx_max_lds_fft_length().
+ * The spectrum of the template is rebuilt on every call; use
+ * zkx_pc_template_prepare + zkx_pulse_compression_prepared_c64 to keep it
+ * resident instead.
+ */
+int zkx_pulse_compression_c64(const zkx_complex64* x, int num_pulses, int samples_per_pulse,
+                              const zkx_complex64* tmpl, int template_len, int nfft,
+                              int normalize, int window, const float* explicit_window,
+                              zkx_complex64* out, void* stream);
+
+/* Template spectrum H = FFT_nfft(h~) into spectrum[nfft] (device). */
+int zkx_pc_template_prepare(const zkx_complex64* tmpl, int template_len, int nfft,
+                            int normalize, int window, const float* explicit_window,
+                            zkx_complex64* spectrum, void* stream);
+
+/* Pulse compression with a prepared spectrum: y = IFFT(FFT(x_pad) * conj(H)).
+ * batch independent [num_pulses, samples_per_pulse] cubes are laid out back
+ * to back (dwell stride num_pulses*samples_per_pulse in, num_pulses*nfft out). */
+int zkx_pulse_compression_prepared_c64(const zkx_complex64* x, int num_pulses,
+                                       int samples_per_pulse, const zkx_complex64* spectrum,
+                                       int nfft, zkx_complex64* out, void* stream);
+
+/*
+ * Pulse-Doppler processing, complex64 in and out (FP32 compute).
+ * Replaces the preallocated pulse_doppler_device<float> overload
+ * (radartools_typed.h:304-311, radartools_typed.cpp:761-789,
+ * radartools_typed.cu:225-244).
+ *   x   [num_pulses, samples_per_pulse];  out [nfft, samples_per_pulse]
+ *   Y[f,r] = sum_{p<min(P,nfft)} w[p] x[p,r] exp(-2 pi i f p / nfft),
+ *   w symmetric over num_pulses; no fftshift.  nfft = -1 means num_pulses.
+ *   power (may be NULL): float [nfft, samples_per_pulse] receives |Y|^2, the
+ *   Task1 step-4 power map (Task1/pipeline/src/steps/step4/step4.cu:11-19).
+ *   out may be NULL when only the power map is wanted.
+ */
+int zkx_pulse_doppler_c64(const zkx_complex64* x, int num_pulses, int samples_per_pulse,
+                          int nfft, int window, const float* explicit_window,
+                          zkx_complex64* out, float* power, void* stream);
+
+/*
+ * CFAR threshold multiplier alpha = N * (pfa^(-1/N) - 1) in FP64 on the host.
+ * Replaces cfar_alpha_typed_cpu / cfar_alpha_device (radartools_typed.h:207-222,
+ * radartools_typed.cpp:592-617).
+ */
+int zkx_cfar_alpha(double pfa, int reference, double* alpha);
+
+/*
+ * Cell-averaging CFAR on a float32 map of rank 1 or 2.
+ * Replaces ca_cfar_device<float> (radartools_typed.h:154-195,
+ * radartools_typed.cpp:557-591, radartools_typed.cu:128-170).
+ *   shape[rank], guard[rank], reference[rank]; threshold float[count];
+ *   detections uint8[count] (the reference's CfarDetection {0,1}).
+ * The threshold is bit-identical to the reference: an FP32 integral image
+ * summed along axis 0 then axis 1 in the reference's order, T=(alpha/N)*(outer-
+ * inner) with alpha rounded to float, T=0 on border cells, det = x > T.
+ * workspace: device float[count] scratch, or NULL to let the library keep a
+ * per-thread buffer that grows on demand.
+ */
+int zkx_ca_cfar_f32(const float* x, int rank, const int* shape, const int* guard,
+                    const int* reference, double pfa, float* threshold, uint8_t* detections,
+                    float* workspace, void* stream);
+
+/*
+ * Windowed CA-CFAR for rank 2 with FP64 box sums built as 2-D prefix sums in
+ * LDS: threshold T = (alpha/N)*(outer-inner) with alpha and the sums in FP64,
+ * det = x > T compared in FP64, T stored as float.  Borders as above.  This is
+ * the CFAR of the fused dwell pipeline; it is not bit-identical to the
+ * reference's FP32 integral image (which loses precision far from the origin)
+ * and is checked against the FP64 oracle instead.
+ */
+int zkx_ca_cfar_window_f32(const float* x, int rows, int cols, int guard_rows, int guard_cols,
+                           int ref_rows, int ref_cols, double pfa, float* threshold,
+                           uint8_t* detections, void* stream);
+
+/* ComplexFP64 storage from a ComplexFP32 result (the reference widens on the
+ * host, host_output_finalize.h:40-57; here it is a device kernel). */
+int zkx_widen_c64_to_c128(const zkx_complex64* in, zkx_complex128* out, size_t count, void* stream);
+int zkx_widen_f32_to_f64(const float* in, double* out, size_t count, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* fused dwell pipeline (Task1 steps 2-4 in one pass per dwell)             */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * A batch of same-shape dwells: pulse compression with a prepared template
+ * spectrum -> Doppler FFT (window over pulses) -> |.|^2 -> CA-CFAR.
+ * Follows the reference's Task1 chain (Task1/pipeline/src/steps/step2/step2.cu:20-36,
+ * step3/step3.cu:17-35, step4/step4.cu:66-84) with the Doppler nfft = num_pulses.
+ *   x          [batch, P, R] complex64
+ *   spectrum   [R] prepared by zkx_pc_template_prepare with nfft = R
+ *   threshold  [batch, P, R] float; detections [batch, P, R] uint8
+ *   rd_map     optional [batch, P, R] complex64 range-Doppler map (NULL to skip)
+ *   cfar_mode  0: bit-identical reference integral image; 1: FP64 windowed sums
+ *   workspace  device buffer of zkx_dwell_workspace_bytes(...) bytes
+ */
+size_t zkx_dwell_workspace_bytes(int batch, int num_pulses, int range_bins);
+int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int range_bins,
+                        const zkx_complex64* spectrum, int doppler_window,
+                        int guard_doppler, int guard_range, int ref_doppler, int ref_range,
+                        double pfa, int cfar_mode, float* threshold, uint8_t* detections,
+                        zkx_complex64* rd_map, void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKX_RADAR_H */

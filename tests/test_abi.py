@@ -1,0 +1,40 @@
+"""CPU checks of the C ABI: the library loads and exports every function the
+header declares (no compute calls: this container has no GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(zkx_\w+|fft_\w+)\s*\(", text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_header_declares_functions():
+    assert "zkx_pulse_compression_c64" in declared_functions()
+
+
+def test_library_exports_every_declared_symbol():
+    from zkx_dev_amd._lib import LIB_PATH
+    if not os.path.exists(LIB_PATH):
+        pytest.fail(f"{LIB_PATH} not built; run python -m zkx_dev_amd.build")
+    lib = ctypes.CDLL(LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    from zkx_dev_amd import _lib
+    missing = [n for n in declared_functions() if n.startswith("zkx_") and n not in _lib.SIGNATURES]
+    assert not missing, missing

@@ -1,0 +1,75 @@
+"""ctypes binding of libzkx_radar.so (the C ABI declared in include/zkx_radar.h).
+
+The library is loaded from zkx_dev_amd/lib/.  There is no fallback: if the HIP
+library is missing, every operator raises.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libzkx_radar.so")
+
+ERRORS = {
+    0: "SUCCESS", 1001: "INVALID_ARGUMENT", 1002: "INVALID_SHAPE", 1003: "INVALID_DTYPE",
+    1004: "UNSUPPORTED_OPERATION", 2002: "CUDA_RUNTIME_ERROR", 2003: "OUT_OF_MEMORY",
+    2099: "BACKEND_ERROR", 9001: "INTERNAL_ERROR",
+}
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+SZ = ctypes.c_size_t
+LL = ctypes.c_longlong
+
+SIGNATURES = {
+    "zkx_last_error_message": (ctypes.c_char_p, []),
+    "zkx_version": (I, []),
+    "zkx_max_lds_fft_length": (I, []),
+    "zkx_pulse_compression_c64": (I, [P, I, I, P, I, I, I, I, P, P, P]),
+    "zkx_pc_template_prepare": (I, [P, I, I, I, I, P, P, P]),
+    "zkx_pulse_compression_prepared_c64": (I, [P, I, I, P, I, P, P]),
+    "zkx_pulse_doppler_c64": (I, [P, I, I, I, I, P, P, P, P]),
+    "zkx_cfar_alpha": (I, [D, I, ctypes.POINTER(D)]),
+    "zkx_ca_cfar_f32": (I, [P, I, P, P, P, D, P, P, P, P]),
+    "zkx_ca_cfar_window_f32": (I, [P, I, I, I, I, I, I, D, P, P, P]),
+    "zkx_widen_c64_to_c128": (I, [P, P, SZ, P]),
+    "zkx_widen_f32_to_f64": (I, [P, P, SZ, P]),
+    "zkx_dwell_workspace_bytes": (SZ, [I, I, I]),
+    "zkx_dwell_batch_c64": (I, [P, I, I, I, P, I, I, I, I, I, D, I, P, P, P, P, P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class ZkxError(RuntimeError):
+    """A non-zero return from the C ABI; .code is the reference ErrorCode."""
+
+    def __init__(self, code, message):
+        self.code = code
+        self.name = ERRORS.get(code, str(code))
+        super().__init__(f"{self.name} ({code}): {message}")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(
+                        f"{LIB_PATH} is missing: build it with `python -m zkx_dev_amd.build`")
+                L = ctypes.CDLL(LIB_PATH)
+                for name, (res, args) in SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        msg = lib().zkx_last_error_message().decode(errors="replace")
+        raise ZkxError(code, msg)

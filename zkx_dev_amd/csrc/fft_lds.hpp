@@ -1,0 +1,187 @@
+// LDS-resident Stockham FFT building blocks for gfx950 (wave64).
+//
+// A transform of N = 2^n points is owned by T = N/E threads; each thread keeps
+// E complex values in registers.  The register convention on entry and on exit
+// is the same natural, strided one:
+//
+//     thread t holds element  t + k*T  in v[k],   k = 0..E-1
+//
+// so callers fuse their own load (window, zero-pad, corner turn) and store
+// (spectral multiply, |.|^2, scaling) around the transform.  Passes follow the
+// self-sorting Stockham decimation-in-time form: pass s with radix R and stride
+// Ns reads x[j + r*N/R], twiddles by W_{Ns*R}^{(j mod Ns)*r}, does a radix-R DFT
+// in registers and writes x'[(j/Ns)*Ns*R + j mod Ns + r*Ns].  The first pass
+// reads the caller's registers and the last pass leaves its results in
+// registers, so only the (log_E N - 1) middle exchanges touch LDS.
+//
+// The LDS layout is a policy object, so a row transform (one signal per slot,
+// padded to break bank conflicts) and a column transform (TC signals
+// interleaved so that a wavefront's lanes hit consecutive banks) share the same
+// pass code.
+#pragma once
+
+#include "zkx_hip_common.hpp"
+
+namespace zkx {
+namespace fft {
+
+// cos(2*pi*k/16), k = 0..3 (the rest follow by symmetry)
+__host__ __device__ constexpr float c16(int k)
+{
+    return k == 0 ? 1.0f : k == 1 ? 0.923879532511286756f : k == 2 ? 0.707106781186547524f
+                                                                  : 0.382683432365089772f;
+}
+
+// Multiply a by W_16^idx = exp(-2 pi i idx / 16) (forward) or its conjugate
+// (inverse).  idx is a compile-time constant after unrolling, so the branches
+// fold and the trivial roots cost no multiplies.
+template <bool INV>
+__device__ __forceinline__ float2 mul_w16(float2 a, int idx)
+{
+    idx &= 15;
+    if (idx == 0) return a;
+    if (idx == 8) return make_float2(-a.x, -a.y);
+    if (idx == 4) return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+    if (idx == 12) return INV ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+    // general root: cos and sin of 2*pi*idx/16 from the first-quadrant table
+    const int q = idx >> 2, m = idx & 3;
+    float c, s;  // cos, sin of 2*pi*idx/16
+    if (q == 0) { c = c16(m); s = c16(4 - m); }
+    else if (q == 1) { c = -c16(4 - m); s = c16(m); }
+    else if (q == 2) { c = -c16(m); s = -c16(4 - m); }
+    else { c = c16(4 - m); s = -c16(m); }
+    const float2 w = INV ? make_float2(c, s) : make_float2(c, -s);
+    return cmul(a, w);
+}
+
+// In-register DFT of R <= 16 points, natural order in and out (radix-2 DIT
+// recursion; every twiddle is a 16th root of unity, folded at compile time).
+template <int R, bool INV>
+struct Dft {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        constexpr int M = R / 2;
+        float2 e[M], o[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        Dft<M, INV>::run(e);
+        Dft<M, INV>::run(o);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const float2 t = mul_w16<INV>(o[k], k * (16 / R));
+            v[k] = cadd(e[k], t);
+            v[k + M] = csub(e[k], t);
+        }
+    }
+};
+template <bool INV>
+struct Dft<1, INV> {
+    static __device__ __forceinline__ void run(float2*) {}
+};
+
+// One signal per slot, re/im split, one padding word per 16 to spread banks.
+struct RowLds {
+    float* re;
+    float* im;
+    __device__ __forceinline__ int idx(int i) const { return i + (i >> 4); }
+    __device__ __forceinline__ float2 load(int i) const
+    {
+        const int a = idx(i);
+        return make_float2(re[a], im[a]);
+    }
+    __device__ __forceinline__ void store(int i, float2 v) const
+    {
+        const int a = idx(i);
+        re[a] = v.x;
+        im[a] = v.y;
+    }
+    static constexpr int floats_for(int n) { return n + n / 16; }
+};
+
+// TC signals interleaved: element i of slot s lives at i*TC + s.
+template <int TC>
+struct ColLds {
+    float* re;
+    float* im;
+    int slot;
+    __device__ __forceinline__ float2 load(int i) const
+    {
+        const int a = i * TC + slot;
+        return make_float2(re[a], im[a]);
+    }
+    __device__ __forceinline__ void store(int i, float2 v) const
+    {
+        const int a = i * TC + slot;
+        re[a] = v.x;
+        im[a] = v.y;
+    }
+};
+
+// Stockham passes from stride Ns on.  tw[k] = exp(-2 pi i k / N), k < N.
+// Every thread of the block must call this (it contains barriers).
+template <int N, int E, int Ns, bool INV, class Lds>
+__device__ __forceinline__ void passes(float2 (&v)[E], const Lds& lds, const float2* __restrict__ tw, int t)
+{
+    constexpr int REM = N / Ns;
+    constexpr int R = REM >= E ? E : REM;
+    constexpr int NB = E / R;
+    constexpr int T = N / E;
+    constexpr bool FIRST = Ns == 1;
+    constexpr bool LAST = Ns * R == N;
+    static_assert(R >= 2 && E % R == 0, "radix schedule");
+    if constexpr (!FIRST) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = t + b * T;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b + r * NB] = lds.load(j + r * (N / R));
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int jm = (t + b * T) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const float2 w = tw[jm * r * (N / (Ns * R))];
+                v[b + r * NB] = INV ? cmulc(v[b + r * NB], w) : cmul(v[b + r * NB], w);
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        float2 u[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = v[b + r * NB];
+        Dft<R, INV>::run(u);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b + r * NB] = u[r];
+    }
+    if constexpr (!LAST) {
+        __syncthreads();  // earlier readers of this LDS region are done
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = t + b * T;
+            const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds.store(base + r * Ns, v[b + r * NB]);
+        }
+        __syncthreads();
+        passes<N, E, Ns * R, INV>(v, lds, tw, t);
+    }
+}
+
+// Full transform: v holds x[t + k*T] on entry and X[t + k*T] on exit.
+// The inverse is unscaled here; callers fold 1/N into their store.
+template <int N, int E, bool INV, class Lds>
+__device__ __forceinline__ void transform(float2 (&v)[E], const Lds& lds, const float2* __restrict__ tw, int t)
+{
+    passes<N, E, 1, INV>(v, lds, tw, t);
+}
+
+// Elements per thread used for each size: E=16 from N=16 up, N itself below.
+__host__ __device__ constexpr int elems_for(int n) { return n >= 16 ? 16 : n; }
+
+}  // namespace fft
+}  // namespace zkx

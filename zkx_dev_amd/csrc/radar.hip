@@ -1,0 +1,948 @@
+// Radar operators on gfx950: LDS Stockham FFT, fused pulse compression,
+// Doppler FFT with a fused |.|^2 epilogue, CA-CFAR (bit-identical integral
+// image and FP64 windowed variants), and the batched dwell pipeline.
+//
+// Reference behaviour followed (paths relative to the reference repository):
+//   pulse_compression  ZKX/cusignal_cpp/src/operators/radartools/radartools_typed.cpp:618-712
+//   pulse_doppler      radartools_typed.cpp:714-808
+//   cfar_alpha         radartools_typed.cpp:592-617
+//   ca_cfar            radartools_typed.cpp:148-236,465-591; radartools_kernels.cuh:23-134
+//   windows            radartools_typed.cpp:238-251 (symmetric Hann/Hamming)
+//   Task1 chain        ZKX/Task1/pipeline/src/steps/step{2,3,4}/step{2,3,4}.cu
+#include "fft_lds.hpp"
+#include "runtime.hpp"
+
+#include <cmath>
+#include <limits>
+
+namespace zkx {
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// Symmetric window value in FP64 (kind 1 Hann, 2 Hamming), rounded by callers.
+__host__ __device__ inline double window_value(int kind, int i, int len)
+{
+    if (kind == ZKX_WINDOW_NONE || len <= 1) return 1.0;
+    const double c = cos(2.0 * kPi * static_cast<double>(i) / static_cast<double>(len - 1));
+    return kind == ZKX_WINDOW_HANN ? 0.5 - 0.5 * c : 0.54 - 0.46 * c;
+}
+
+// ---------------------------------------------------------------------------
+// row FFT kernels (one signal per slot, FPB slots per block)
+// ---------------------------------------------------------------------------
+template <int N>
+struct RowCfg {
+    static constexpr int E = fft::elems_for(N);
+    static constexpr int T = N / E;
+    static constexpr int FPB = T >= 256 ? 1 : 256 / T;  // signals per block
+    static constexpr int THREADS = T * FPB;
+    static constexpr int SLOT_FLOATS = fft::RowLds::floats_for(N);
+    static constexpr size_t LDS_BYTES = sizeof(float) * 2 * SLOT_FLOATS * FPB;
+};
+
+// Template spectrum H = FFT_N(h~), h~ = window*h / ||window*h|| (normalize)
+// truncated/zero-padded to N.  One block, FPB slots but only slot 0 works.
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+pc_template_kernel(const float2* __restrict__ h, int taps, int normalize, int window,
+                   const float* __restrict__ explicit_window, const float2* __restrict__ tw,
+                   float2* __restrict__ spectrum)
+{
+    using C = RowCfg<N>;
+    extern __shared__ float lds[];
+    __shared__ double red[1024];
+    const int tid = threadIdx.x;
+    const bool expl = window == ZKX_WINDOW_EXPLICIT_FP32 || window == ZKX_WINDOW_EXPLICIT_FP64;
+    // ||w*h||^2 over all taps in FP64
+    double acc = 0.0;
+    for (int i = tid; i < taps; i += blockDim.x) {
+        const double w = expl ? static_cast<double>(explicit_window[i]) : window_value(window, i, taps);
+        const double re = h[i].x * w, im = h[i].y * w;
+        acc += re * re + im * im;
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double scale = normalize ? 1.0 / sqrt(red[0]) : 1.0;
+    const int slot = tid / C::T, t = tid % C::T;
+    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int i = t + k * C::T;
+        float2 val = make_float2(0.f, 0.f);
+        if (slot == 0 && i < taps) {
+            const double w = expl ? static_cast<double>(explicit_window[i]) : window_value(window, i, taps);
+            val = make_float2(static_cast<float>(h[i].x * w * scale), static_cast<float>(h[i].y * w * scale));
+        }
+        v[k] = val;
+    }
+    fft::transform<N, C::E, false>(v, L, tw, t);
+    if (slot == 0) {
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) spectrum[t + k * C::T] = v[k];
+    }
+}
+
+// Fused pulse compression of rows: y = IFFT_N(FFT_N(x_pad) * conj(H)) / N.
+// rows = total pulses (batch * P); each row reads min(S, N) samples.
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+pc_fused_kernel(const float2* __restrict__ x, long long rows, int S, const float2* __restrict__ H,
+                const float2* __restrict__ tw, float2* __restrict__ y)
+{
+    using C = RowCfg<N>;
+    extern __shared__ float lds[];
+    const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
+    const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
+    const bool active = row < rows;
+    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    const float2* src = x + (active ? row : 0) * static_cast<long long>(S);
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int i = t + k * C::T;
+        v[k] = (active && i < S) ? src[i] : make_float2(0.f, 0.f);
+    }
+    fft::transform<N, C::E, false>(v, L, tw, t);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], H[t + k * C::T]);
+    fft::transform<N, C::E, true>(v, L, tw, t);
+    if (active) {
+        const float s = 1.0f / static_cast<float>(N);
+        float2* dst = y + row * static_cast<long long>(N);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) dst[t + k * C::T] = cscale(v[k], s);
+    }
+}
+
+// Plain batched C2C row FFT (forward or inverse scaled by `scale`).
+template <int N, bool INV>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+fft_rows_kernel(const float2* __restrict__ x, long long rows, const float2* __restrict__ tw,
+                float scale, float2* __restrict__ y)
+{
+    using C = RowCfg<N>;
+    extern __shared__ float lds[];
+    const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
+    const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
+    const bool active = row < rows;
+    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    const float2* src = x + (active ? row : 0) * static_cast<long long>(N);
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) v[k] = active ? src[t + k * C::T] : make_float2(0.f, 0.f);
+    fft::transform<N, C::E, INV>(v, L, tw, t);
+    if (active) {
+        float2* dst = y + row * static_cast<long long>(N);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) dst[t + k * C::T] = cscale(v[k], scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Doppler: column FFT along pulses, TC range bins per block
+// ---------------------------------------------------------------------------
+template <int N>
+struct ColCfg {
+    static constexpr int E = fft::elems_for(N);
+    static constexpr int T = N / E;
+    // range bins per block: enough for coalesced rows, at most 1024 threads
+    static constexpr int TC = T <= 16 ? (256 / T > 64 ? 64 : 256 / T)
+                                      : (T >= 512 ? 1 : (1024 / T < 16 ? 1024 / T : 16));
+    static constexpr int THREADS = T * TC;
+    static constexpr size_t LDS_BYTES = sizeof(float) * (2 * N * TC + N);
+};
+
+// x [batch, P, R] -> Y [batch, N, R] (optional), |Y|^2 [batch, N, R] (optional),
+// and the axis-0 running FP32 sum of |Y|^2 (optional): the first half of the
+// reference CFAR's integral image (radartools_typed.cpp:501-510), fused here so
+// the power map never makes an extra HBM round trip for it.
+// Window over P pulses (symmetric), computed per block in FP64 into LDS.
+template <int N>
+__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+doppler_kernel(const float2* __restrict__ x, int P, int R, int window,
+               const float* __restrict__ explicit_window, const float2* __restrict__ tw,
+               float2* __restrict__ y, float* __restrict__ power, float* __restrict__ colsum)
+{
+    using C = ColCfg<N>;
+    extern __shared__ float lds[];
+    float* wtab = lds + 2 * N * C::TC;
+    const int used = P < N ? P : N;
+    const bool expl = window == ZKX_WINDOW_EXPLICIT_FP32 || window == ZKX_WINDOW_EXPLICIT_FP64;
+    for (int p = threadIdx.x; p < used; p += blockDim.x)
+        wtab[p] = expl ? explicit_window[p] : static_cast<float>(window_value(window, p, P));
+    __syncthreads();
+    const int s = threadIdx.x % C::TC, t = threadIdx.x / C::TC;
+    const int col = blockIdx.x * C::TC + s;
+    const long long dwell = blockIdx.y;
+    const bool active = col < R;
+    const float2* src = x + dwell * static_cast<long long>(P) * R;
+    fft::ColLds<C::TC> L{lds, lds + N * C::TC, s};
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int p = t + k * C::T;
+        v[k] = (active && p < used) ? cscale(src[static_cast<long long>(p) * R + col], wtab[p])
+                                    : make_float2(0.f, 0.f);
+    }
+    fft::transform<N, C::E, false>(v, L, tw, t);
+    const long long obase = dwell * static_cast<long long>(N) * R;
+    float pw[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        pw[k] = v[k].x * v[k].x + v[k].y * v[k].y;
+        if (active) {
+            const long long o = obase + static_cast<long long>(t + k * C::T) * R + col;
+            if (y) y[o] = v[k];
+            if (power) power[o] = pw[k];
+        }
+    }
+    if (colsum) {
+        // stage |Y|^2 as [f][slot] and let one thread per column run the
+        // sequential FP32 sum down the N Doppler rows
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) lds[(t + k * C::T) * C::TC + s] = pw[k];
+        __syncthreads();
+        if (t == 0 && active) {
+            float run = 0.0f;
+            for (int f = 0; f < N; ++f) {
+                run = __fadd_rn(run, lds[f * C::TC + s]);
+                colsum[obase + static_cast<long long>(f) * R + col] = run;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CA-CFAR, reference-identical FP32 integral image
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+// Running FP32 sum down each column (axis 0), one thread per column.
+__global__ void cfar_colscan_kernel(const float* __restrict__ x, int rows, int cols,
+                                    float* __restrict__ I)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    const long long base = static_cast<long long>(blockIdx.y) * rows * cols;
+    float run = 0.0f;
+    for (int r = 0; r < rows; ++r) {
+        const long long i = base + static_cast<long long>(r) * cols + c;
+        run += x[i];
+        I[i] = run;
+    }
+}
+
+// Running FP32 sum along each row (axis 1), in place.  64 rows per block, one
+// thread per row; 64-column tiles staged through LDS so HBM traffic stays
+// coalesced while every row keeps the reference's left-to-right order.
+constexpr int kScanRows = 64;
+constexpr int kScanCols = 64;
+__global__ void __launch_bounds__(kScanRows) cfar_rowscan_kernel(float* __restrict__ I, int rows, int cols)
+{
+    __shared__ float tile[kScanRows][kScanCols + 1];
+    const int lane = threadIdx.x;
+    const int r0 = blockIdx.x * kScanRows;
+    const long long base = static_cast<long long>(blockIdx.y) * rows * cols;
+    float run = 0.0f;
+    for (int c0 = 0; c0 < cols; c0 += kScanCols) {
+        const int c = c0 + lane;
+        for (int i = 0; i < kScanRows; ++i) {
+            const int r = r0 + i;
+            tile[i][lane] = (r < rows && c < cols) ? I[base + static_cast<long long>(r) * cols + c] : 0.0f;
+        }
+        __syncthreads();
+        const int width = cols - c0 < kScanCols ? cols - c0 : kScanCols;
+        if (width == kScanCols) {
+#pragma unroll
+            for (int j = 0; j < kScanCols; ++j) {
+                run += tile[lane][j];
+                tile[lane][j] = run;
+            }
+        } else {
+            for (int j = 0; j < width; ++j) {
+                run += tile[lane][j];
+                tile[lane][j] = run;
+            }
+        }
+        __syncthreads();
+        for (int i = 0; i < kScanRows; ++i) {
+            const int r = r0 + i;
+            if (r < rows && c < cols) I[base + static_cast<long long>(r) * cols + c] = tile[i][lane];
+        }
+        __syncthreads();
+    }
+}
+
+// Threshold and detection per cell from the integral image, in the reference's
+// association: T = (alpha/N) * (outer - inner); borders T = 0; det = x > T.
+__global__ void cfar_threshold_kernel(const float* __restrict__ x, const float* __restrict__ I,
+                                      int rows, int cols, int g0, int g1, int r0, int r1,
+                                      float alpha, int count, float* __restrict__ thr,
+                                      unsigned char* __restrict__ det)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= cols) return;
+    const long long base = static_cast<long long>(blockIdx.z) * rows * cols;
+    const float* A = I + base;
+    const int rm = g0 + r0, cm = g1 + r1;
+    float T = 0.0f;
+    if (r >= rm && r < rows - rm && c >= cm && c < cols - cm) {
+        auto at = [&](int rr, int cc) { return A[static_cast<long long>(rr) * cols + cc]; };
+        const int ot = r + rm, ol = r - rm - 1, orc = c + cm, olc = c - cm - 1;
+        float outer = at(ot, orc);
+        if (ol >= 0) outer -= at(ol, orc);
+        if (olc >= 0) outer -= at(ot, olc);
+        if (ol >= 0 && olc >= 0) outer += at(ol, olc);
+        const float inner = at(r + g0, c + g1) - at(r - g0 - 1, c + g1) - at(r + g0, c - g1 - 1) +
+                            at(r - g0 - 1, c - g1 - 1);
+        T = (alpha / static_cast<float>(count)) * (outer - inner);
+    }
+    const long long i = base + static_cast<long long>(r) * cols + c;
+    thr[i] = T;
+    det[i] = (!isnan(T) && x[i] > T) ? 1 : 0;
+}
+
+// Row scan + threshold + detection, fused.  A block owns an output band of
+// `band` rows of one dwell and scans the rows its windows need (the band plus
+// g0+r0+1 rows above and g0+r0 below) left to right, one thread per row, so
+// every integral-image value is the reference's sequential FP32 sum.  Columns
+// stream through LDS in chunks of kBandCW: the next chunk's column prefixes
+// are prefetched into registers while the current one is scanned; scanned
+// values go to a ring of the last `ring` columns (a power of two covering
+// kBandCW + 2*(g1+r1) + 1), from which all threads then evaluate the
+// thresholds of the chunk's cells, reading x and writing T and the mask with
+// coalesced accesses.
+constexpr int kBandCW = 32;
+
+constexpr int kBandMaxRows = 128;  // rows one block scans (band + halo)
+
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS)
+cfar_band_kernel(const float* __restrict__ x, const float* __restrict__ C, int rows, int cols, int g0, int g1,
+                 int r0, int r1, float alpha, int count, int band, int ring, float* __restrict__ thr,
+                 unsigned char* __restrict__ det)
+{
+    extern __shared__ float sm[];
+    constexpr int MAXR = kBandMaxRows;
+    constexpr int SW = kBandCW + 1;  // stage row stride (padded)
+    const int RWS = ring + 1;        // ring row stride (padded)
+    float* stage = sm;                        // [MAXR][SW]  column prefixes of the chunk
+    float* irng = sm + MAXR * SW;             // [MAXR][RWS] integral-image ring
+    float* xrng = irng + MAXR * RWS;          // [band][RWS] x ring of the output rows
+    const int tid = threadIdx.x;
+    const int rm = g0 + r0, cm = g1 + r1;
+    const int ro0 = blockIdx.x * band;
+    const int ro1 = min(rows, ro0 + band);
+    const int BR = ro1 - ro0;
+    const int sr0 = max(0, ro0 - rm - 1);
+    const int sr1 = min(rows, ro1 + rm);
+    const int SR = sr1 - sr0;
+    const long long base = static_cast<long long>(blockIdx.y) * rows * cols;
+    const float* Cb = C + base;
+    const float* Xb = x + base;
+    constexpr int LOADS = MAXR * kBandCW / THREADS;
+    float pc[LOADS], px[LOADS];
+    auto prefetch = [&](int c0) {
+#pragma unroll
+        for (int i = 0; i < LOADS; ++i) {
+            const int e = tid + i * THREADS;
+            const int rr = e / kBandCW, cc = e % kBandCW;
+            const bool inb = c0 + cc < cols;
+            pc[i] = (rr < SR && inb) ? Cb[static_cast<long long>(sr0 + rr) * cols + c0 + cc] : 0.0f;
+            px[i] = (rr < BR && inb) ? Xb[static_cast<long long>(ro0 + rr) * cols + c0 + cc] : 0.0f;
+        }
+    };
+    auto I = [&](int r, int c) { return irng[(r - sr0) * RWS + (c & (ring - 1))]; };
+    auto X = [&](int r, int c) { return xrng[(r - ro0) * RWS + (c & (ring - 1))]; };
+    const float scale = __fdiv_rn(alpha, static_cast<float>(count));
+    auto cell = [&](int r, int c) {
+        float T = 0.0f;
+        if (r >= rm && r < rows - rm && c >= cm && c < cols - cm) {
+            const int ot = r + rm, ol = r - rm - 1, orc = c + cm, olc = c - cm - 1;
+            float outer = I(ot, orc);
+            if (ol >= 0) outer = __fsub_rn(outer, I(ol, orc));
+            if (olc >= 0) outer = __fsub_rn(outer, I(ot, olc));
+            if (ol >= 0 && olc >= 0) outer = __fadd_rn(outer, I(ol, olc));
+            float inner = __fsub_rn(I(r + g0, c + g1), I(r - g0 - 1, c + g1));
+            inner = __fsub_rn(inner, I(r + g0, c - g1 - 1));
+            inner = __fadd_rn(inner, I(r - g0 - 1, c - g1 - 1));
+            T = __fmul_rn(scale, __fsub_rn(outer, inner));
+        }
+        const long long o = base + static_cast<long long>(r) * cols + c;
+        thr[o] = T;
+        det[o] = (!isnan(T) && X(r, c) > T) ? 1 : 0;
+    };
+    float run = 0.0f;
+    prefetch(0);
+    for (int c0 = 0; c0 < cols; c0 += kBandCW) {
+#pragma unroll
+        for (int i = 0; i < LOADS; ++i) {
+            const int e = tid + i * THREADS;
+            const int rr = e / kBandCW, cc = e % kBandCW;
+            if (rr < SR) stage[rr * SW + cc] = pc[i];
+            if (rr < BR) xrng[rr * RWS + ((c0 + cc) & (ring - 1))] = px[i];
+        }
+        __syncthreads();
+        if (c0 + kBandCW < cols) prefetch(c0 + kBandCW);
+        const int width = min(kBandCW, cols - c0);
+        if (tid < SR) {
+            const float* st = stage + tid * SW;
+            float* rw = irng + tid * RWS;
+            if (width == kBandCW) {
+#pragma unroll
+                for (int j = 0; j < kBandCW; ++j) {
+                    run = __fadd_rn(run, st[j]);
+                    rw[(c0 + j) & (ring - 1)] = run;
+                }
+            } else {
+                for (int j = 0; j < width; ++j) {
+                    run = __fadd_rn(run, st[j]);
+                    rw[(c0 + j) & (ring - 1)] = run;
+                }
+            }
+        }
+        __syncthreads();
+        // cells of columns [c0 - cm, c0 + width - cm): every window they need is scanned
+        const int chi = c0 + width - cm;
+        for (int q = tid; q < BR * kBandCW; q += THREADS) {
+            const int rr = q / kBandCW, cc = q % kBandCW;
+            const int c = c0 - cm + cc;
+            if (c >= 0 && c < chi) cell(ro0 + rr, c);
+        }
+        if (c0 + width >= cols) {
+            // right border columns [max(0, chi), cols): threshold 0
+            const int t0 = max(0, chi);
+            const int tw = cols - t0;
+            for (int q = tid; q < BR * tw; q += THREADS) cell(ro0 + q / tw, t0 + q % tw);
+        }
+    }
+}
+
+// Rank 1: sequential FP32 prefix sum (one thread), then per-cell threshold.
+__global__ void cfar1d_scan_kernel(const float* __restrict__ x, int n, float* __restrict__ I)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float run = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        run += x[i];
+        I[i] = run;
+    }
+}
+
+__global__ void cfar1d_threshold_kernel(const float* __restrict__ x, const float* __restrict__ I, int n,
+                                        int g, int ref, float alpha, int count, float* __restrict__ thr,
+                                        unsigned char* __restrict__ det)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = g + ref;
+    float T = 0.0f;
+    if (i >= m && i < n - m) {
+        const int orr = i + m, ol = i - m - 1, ir = i + g, il = i - g - 1;
+        const float outer = ol >= 0 ? I[orr] - I[ol] : I[orr];
+        const float inner = I[ir] - I[il];
+        T = (alpha / static_cast<float>(count)) * (outer - inner);
+    }
+    thr[i] = T;
+    det[i] = (!isnan(T) && x[i] > T) ? 1 : 0;
+}
+
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------
+// CA-CFAR with FP64 windowed sums (2-D prefix sums in LDS)
+// ---------------------------------------------------------------------------
+// Block tile: TR x TCW output cells; the LDS holds the tile plus a halo of
+// (g+r) cells on every side as an FP64 2-D prefix-sum table.
+constexpr int kWinTileR = 32;
+constexpr int kWinTileC = 64;
+
+__global__ void __launch_bounds__(256)
+cfar_window_kernel(const float* __restrict__ x, int rows, int cols, int g0, int g1, int r0, int r1,
+                   double alpha_over_n, float* __restrict__ thr, unsigned char* __restrict__ det)
+{
+    extern __shared__ double S[];  // (TR + 2*rm + 1) x (TC + 2*cm + 1) prefix table
+    const int rm = g0 + r0, cm = g1 + r1;
+    const int HR = kWinTileR + 2 * rm, HC = kWinTileC + 2 * cm;
+    const int W = HC + 1;
+    const int tr0 = blockIdx.y * kWinTileR, tc0 = blockIdx.x * kWinTileC;
+    const long long base = static_cast<long long>(blockIdx.z) * rows * cols;
+    const float* X = x + base;
+    // load the haloed tile (zero outside the map) into rows 1.., cols 1..
+    for (int i = threadIdx.x; i < HR * HC; i += blockDim.x) {
+        const int hr = i / HC, hc = i % HC;
+        const int r = tr0 - rm + hr, c = tc0 - cm + hc;
+        S[(hr + 1) * W + hc + 1] = (r >= 0 && r < rows && c >= 0 && c < cols)
+                                       ? static_cast<double>(X[static_cast<long long>(r) * cols + c]) : 0.0;
+    }
+    for (int i = threadIdx.x; i < W; i += blockDim.x) S[i] = 0.0;
+    for (int i = threadIdx.x; i < HR + 1; i += blockDim.x) S[i * W] = 0.0;
+    __syncthreads();
+    // prefix along rows (each thread one table row), then along columns
+    for (int hr = threadIdx.x; hr < HR; hr += blockDim.x) {
+        double run = 0.0;
+        double* rowp = S + (hr + 1) * W;
+        for (int hc = 1; hc <= HC; ++hc) {
+            run += rowp[hc];
+            rowp[hc] = run;
+        }
+    }
+    __syncthreads();
+    for (int hc = threadIdx.x; hc < HC; hc += blockDim.x) {
+        double run = 0.0;
+        for (int hr = 1; hr <= HR; ++hr) {
+            run += S[hr * W + hc + 1];
+            S[hr * W + hc + 1] = run;
+        }
+    }
+    __syncthreads();
+    auto box = [&](int ra, int rb, int ca, int cb) {  // inclusive table coords (0-based in halo tile)
+        return S[(rb + 1) * W + cb + 1] - S[ra * W + cb + 1] - S[(rb + 1) * W + ca] + S[ra * W + ca];
+    };
+    for (int i = threadIdx.x; i < kWinTileR * kWinTileC; i += blockDim.x) {
+        const int lr = i / kWinTileC, lc = i % kWinTileC;
+        const int r = tr0 + lr, c = tc0 + lc;
+        if (r >= rows || c >= cols) continue;
+        const long long o = base + static_cast<long long>(r) * cols + c;
+        double T = 0.0;
+        if (r >= rm && r < rows - rm && c >= cm && c < cols - cm) {
+            const int hr = lr + rm, hc = lc + cm;  // cell position inside the halo tile
+            const double outer = box(hr - rm, hr + rm, hc - cm, hc + cm);
+            const double inner = box(hr - g0, hr + g0, hc - g1, hc + g1);
+            T = alpha_over_n * (outer - inner);
+        }
+        const double xv = static_cast<double>(x[o]);
+        thr[o] = static_cast<float>(T);
+        det[o] = xv > T ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch helpers
+// ---------------------------------------------------------------------------
+constexpr int kMaxLdsFft = 8192;
+
+// Kernels needing more than 64 KiB of dynamic LDS must opt in once.
+template <class K>
+int allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    return ZKX_SUCCESS;
+}
+
+template <int N>
+int launch_fft_rows(const float2* x, long long rows, bool inverse, float scale, float2* y, hipStream_t st)
+{
+    using C = RowCfg<N>;
+    const float2* tw = nullptr;
+    int rc = twiddles(N, &tw);
+    if (rc) return rc;
+    const long long blocks = (rows + C::FPB - 1) / C::FPB;
+    if ((rc = allow_lds(fft_rows_kernel<N, true>, C::LDS_BYTES)) || (rc = allow_lds(fft_rows_kernel<N, false>, C::LDS_BYTES)))
+        return rc;
+    if (inverse)
+        hipLaunchKernelGGL((fft_rows_kernel<N, true>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
+                           C::LDS_BYTES, st, x, rows, tw, scale, y);
+    else
+        hipLaunchKernelGGL((fft_rows_kernel<N, false>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
+                           C::LDS_BYTES, st, x, rows, tw, scale, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+template <int N>
+int launch_pc_template(const float2* h, int taps, int normalize, int window, const float* ew, float2* H,
+                       hipStream_t st)
+{
+    using C = RowCfg<N>;
+    const float2* tw = nullptr;
+    int rc = twiddles(N, &tw);
+    if (rc) return rc;
+    if ((rc = allow_lds(pc_template_kernel<N>, C::LDS_BYTES))) return rc;
+    hipLaunchKernelGGL((pc_template_kernel<N>), dim3(1), dim3(C::THREADS), C::LDS_BYTES, st, h, taps,
+                       normalize, window, ew, tw, H);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+template <int N>
+int launch_pc_fused(const float2* x, long long rows, int S, const float2* H, float2* y, hipStream_t st)
+{
+    using C = RowCfg<N>;
+    const float2* tw = nullptr;
+    int rc = twiddles(N, &tw);
+    if (rc) return rc;
+    const long long blocks = (rows + C::FPB - 1) / C::FPB;
+    if ((rc = allow_lds(pc_fused_kernel<N>, C::LDS_BYTES))) return rc;
+    hipLaunchKernelGGL((pc_fused_kernel<N>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
+                       C::LDS_BYTES, st, x, rows, S, H, tw, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+template <int N>
+int launch_doppler(const float2* x, int batch, int P, int R, int window, const float* ew, float2* y,
+                   float* power, float* colsum, hipStream_t st)
+{
+    using C = ColCfg<N>;
+    const float2* tw = nullptr;
+    int rc = twiddles(N, &tw);
+    if (rc) return rc;
+    dim3 grid((R + C::TC - 1) / C::TC, batch);
+    if ((rc = allow_lds(doppler_kernel<N>, C::LDS_BYTES))) return rc;
+    hipLaunchKernelGGL((doppler_kernel<N>), grid, dim3(C::THREADS), C::LDS_BYTES, st, x, P, R, window, ew,
+                       tw, y, power, colsum);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+#define ZKX_POW2_CASES(F, ...)                     \
+    switch (n) {                                   \
+    case 2: return F<2>(__VA_ARGS__);              \
+    case 4: return F<4>(__VA_ARGS__);              \
+    case 8: return F<8>(__VA_ARGS__);              \
+    case 16: return F<16>(__VA_ARGS__);            \
+    case 32: return F<32>(__VA_ARGS__);            \
+    case 64: return F<64>(__VA_ARGS__);            \
+    case 128: return F<128>(__VA_ARGS__);          \
+    case 256: return F<256>(__VA_ARGS__);          \
+    case 512: return F<512>(__VA_ARGS__);          \
+    case 1024: return F<1024>(__VA_ARGS__);        \
+    case 2048: return F<2048>(__VA_ARGS__);        \
+    case 4096: return F<4096>(__VA_ARGS__);        \
+    case 8192: return F<8192>(__VA_ARGS__);        \
+    default: break;                                \
+    }
+
+int fft_rows(int n, const float2* x, long long rows, bool inverse, float scale, float2* y, hipStream_t st)
+{
+    ZKX_POW2_CASES(launch_fft_rows, x, rows, inverse, scale, y, st)
+    ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "FFT length must be a power of two in [2, 8192]");
+}
+
+int pc_template(int n, const float2* h, int taps, int normalize, int window, const float* ew, float2* H,
+                hipStream_t st)
+{
+    ZKX_POW2_CASES(launch_pc_template, h, taps, normalize, window, ew, H, st)
+    ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression nfft must be a power of two in [2, 8192]");
+}
+
+int pc_fused(int n, const float2* x, long long rows, int S, const float2* H, float2* y, hipStream_t st)
+{
+    ZKX_POW2_CASES(launch_pc_fused, x, rows, S, H, y, st)
+    ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression nfft must be a power of two in [2, 8192]");
+}
+
+int doppler(int n, const float2* x, int batch, int P, int R, int window, const float* ew, float2* y,
+            float* power, float* colsum, hipStream_t st)
+{
+    ZKX_POW2_CASES(launch_doppler, x, batch, P, R, window, ew, y, power, colsum, st)
+    ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_doppler nfft must be a power of two in [2, 8192]");
+}
+
+bool valid_window(int w) { return w >= ZKX_WINDOW_NONE && w <= ZKX_WINDOW_EXPLICIT_FP64; }
+bool explicit_kind(int w) { return w == ZKX_WINDOW_EXPLICIT_FP32 || w == ZKX_WINDOW_EXPLICIT_FP64; }
+
+float cfar_alpha_float(int count, double pfa)
+{
+    const double n = static_cast<double>(count);
+    return static_cast<float>(n * (std::pow(pfa, -1.0 / n) - 1.0));
+}
+
+thread_local ThreadWorkspace t_cfar_ws;
+
+// Reference-identical 2-D CFAR.  colsum: the axis-0 FP32 running sums of x
+// when the producer already made them (the fused Doppler epilogue), else NULL
+// and they are built here into I.
+int cfar_faithful_2d(const float* x, const float* colsum, int batch, int rows, int cols, int g0, int g1, int r0,
+                     int r1, double pfa, float* thr, unsigned char* det, float* I, hipStream_t st)
+{
+    const int count = 2 * r0 * (2 * r1 + 2 * g1 + 1) + 2 * (2 * g0 + 1) * r1;
+    const float alpha = cfar_alpha_float(count, pfa);
+    if (!colsum) {
+        hipLaunchKernelGGL(cfar_colscan_kernel, dim3((cols + 255) / 256, batch), dim3(256), 0, st, x, rows, cols, I);
+        ZKX_LAUNCH_CHECK();
+        colsum = I;
+    }
+    const int rm = g0 + r0, cm = g1 + r1;
+    int ring = 32;
+    while (ring < kBandCW + 2 * cm + 1) ring *= 2;
+    const int halo = 2 * rm + 1;
+    // band rows: whole dwells when there are enough dwells to fill the chip,
+    // else narrower bands (each re-scans its halo rows) for more blocks
+    auto pick = [&](int threads, int& band) {
+        if (rows <= threads) {
+            band = rows;
+            const int want_blocks = 256;
+            while (band > 32 && static_cast<long long>(batch) * ((rows + band - 1) / band) < want_blocks &&
+                   (band / 2) + halo <= threads)
+                band /= 2;
+            return true;
+        }
+        band = threads - halo;
+        return band >= 16;
+    };
+    int band = 0;
+    if (ring <= 128 && pick(kBandMaxRows, band)) {
+        constexpr int TH = 256;
+        const size_t lds = sizeof(float) * (kBandMaxRows * (kBandCW + 1 + ring + 1) +
+                                            static_cast<size_t>(band) * (ring + 1));
+        int rc = allow_lds(cfar_band_kernel<TH>, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(cfar_band_kernel<TH>, dim3((rows + band - 1) / band, batch), dim3(TH), lds, st, x,
+                           colsum, rows, cols, g0, g1, r0, r1, alpha, count, band, ring, thr, det);
+        ZKX_LAUNCH_CHECK();
+        return ZKX_SUCCESS;
+    }
+    // very wide windows: separate row scan and threshold passes
+    hipLaunchKernelGGL(cfar_rowscan_kernel, dim3((rows + kScanRows - 1) / kScanRows, batch), dim3(kScanRows), 0,
+                       st, I, rows, cols);
+    ZKX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(cfar_threshold_kernel, dim3((cols + 255) / 256, rows, batch), dim3(256), 0, st, x, I, rows,
+                       cols, g0, g1, r0, r1, alpha, count, thr, det);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+int cfar_window_2d(const float* x, int batch, int rows, int cols, int g0, int g1, int r0, int r1, double pfa,
+                   float* thr, unsigned char* det, hipStream_t st)
+{
+    const int count = 2 * r0 * (2 * r1 + 2 * g1 + 1) + 2 * (2 * g0 + 1) * r1;
+    const double n = static_cast<double>(count);
+    const double alpha = n * (std::pow(pfa, -1.0 / n) - 1.0);
+    const int rm = g0 + r0, cm = g1 + r1;
+    const size_t lds = sizeof(double) * static_cast<size_t>(kWinTileR + 2 * rm + 1) * (kWinTileC + 2 * cm + 1);
+    if (lds > 160 * 1024) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "ca_cfar window too large for the LDS tile");
+    int rc = allow_lds(cfar_window_kernel, lds);
+    if (rc) return rc;
+    dim3 grid((cols + kWinTileC - 1) / kWinTileC, (rows + kWinTileR - 1) / kWinTileR, batch);
+    hipLaunchKernelGGL(cfar_window_kernel, grid, dim3(256), lds, st, x, rows, cols, g0, g1, r0, r1, alpha / n, thr,
+                       det);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+int validate_cfar2d(int rows, int cols, int g0, int g1, int r0, int r1, double pfa)
+{
+    if (g0 < 0 || g1 < 0 || r0 < 1 || r1 < 1)
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar guard/reference cells are invalid");
+    if (rows < 0 || cols < 0) ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar shape dimensions must be nonnegative");
+    if (static_cast<long long>(rows) <= 2LL * (g0 + r0) || static_cast<long long>(cols) <= 2LL * (g1 + r1))
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar input dimension is too small");
+    if (!std::isfinite(pfa) || pfa <= 0.0 || pfa > 1.0) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar pfa must be in (0,1]");
+    return ZKX_SUCCESS;
+}
+
+}  // namespace
+
+// exported for other translation units of the library
+int fft_rows_dispatch(int n, const float2* x, long long rows, bool inverse, float scale, float2* y,
+                      hipStream_t st)
+{
+    return fft_rows(n, x, rows, inverse, scale, y, st);
+}
+
+}  // namespace zkx
+
+using namespace zkx;
+
+extern "C" {
+
+int zkx_max_lds_fft_length(void) { return kMaxLdsFft; }
+
+int zkx_pc_template_prepare(const zkx_complex64* tmpl, int template_len, int nfft, int normalize, int window,
+                            const float* explicit_window, zkx_complex64* spectrum, void* stream)
+{
+    clear_last_error();
+    if (nfft < 1 || template_len < 1 || !tmpl || !spectrum)
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "pulse_compression shape is invalid");
+    if (!valid_window(window)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unsupported pulse_compression window");
+    if (explicit_kind(window) != (explicit_window != nullptr))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_compression window is invalid");
+    if (nfft == 1) {
+        // a 1-point transform is the (windowed, normalised) first tap itself
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression nfft must be a power of two in [2, 8192]");
+    }
+    return pc_template(nfft, reinterpret_cast<const float2*>(tmpl), template_len, normalize, window,
+                       explicit_window, reinterpret_cast<float2*>(spectrum), (hipStream_t)stream);
+}
+
+int zkx_pulse_compression_prepared_c64(const zkx_complex64* x, int num_pulses, int samples_per_pulse,
+                                       const zkx_complex64* spectrum, int nfft, zkx_complex64* out,
+                                       void* stream)
+{
+    clear_last_error();
+    if (num_pulses < 1 || samples_per_pulse < 1 || nfft < 1 || !x || !spectrum || !out)
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "pulse_compression shape is invalid");
+    return pc_fused(nfft, reinterpret_cast<const float2*>(x), num_pulses, samples_per_pulse,
+                    reinterpret_cast<const float2*>(spectrum), reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
+int zkx_pulse_compression_c64(const zkx_complex64* x, int num_pulses, int samples_per_pulse,
+                              const zkx_complex64* tmpl, int template_len, int nfft, int normalize, int window,
+                              const float* explicit_window, zkx_complex64* out, void* stream)
+{
+    clear_last_error();
+    if (nfft == -1) nfft = samples_per_pulse;
+    const long long in_count = static_cast<long long>(num_pulses) * samples_per_pulse;
+    const long long out_count = static_cast<long long>(num_pulses) * nfft;
+    if (num_pulses < 1 || samples_per_pulse < 1 || nfft < 1 || template_len < 1 ||
+        in_count > std::numeric_limits<int>::max() || out_count > std::numeric_limits<int>::max() || !x ||
+        !tmpl || !out)
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "pulse_compression shape is invalid");
+    if (!valid_window(window)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unsupported pulse_compression window");
+    if (explicit_kind(window) != (explicit_window != nullptr))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_compression window is invalid");
+    if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression nfft must be a power of two in [2, 8192]");
+    // per-call spectrum buffer, kept per host thread like the reference's plan pool
+    static thread_local ThreadWorkspace ws;
+    void* H = nullptr;
+    int rc = ws.reserve(sizeof(float2) * static_cast<size_t>(nfft), &H);
+    if (rc) return rc;
+    rc = zkx_pc_template_prepare(tmpl, template_len, nfft, normalize, window, explicit_window,
+                                 static_cast<zkx_complex64*>(H), stream);
+    if (rc) return rc;
+    return zkx_pulse_compression_prepared_c64(x, num_pulses, samples_per_pulse, static_cast<zkx_complex64*>(H),
+                                              nfft, out, stream);
+}
+
+int zkx_pulse_doppler_c64(const zkx_complex64* x, int num_pulses, int samples_per_pulse, int nfft, int window,
+                          const float* explicit_window, zkx_complex64* out, float* power, void* stream)
+{
+    clear_last_error();
+    if (nfft == -1) nfft = num_pulses;
+    const long long in_count = static_cast<long long>(num_pulses) * samples_per_pulse;
+    const long long out_count = static_cast<long long>(nfft) * samples_per_pulse;
+    if (num_pulses < 1 || samples_per_pulse < 1 || nfft < 1 || in_count > std::numeric_limits<int>::max() ||
+        out_count > std::numeric_limits<int>::max() || !x || (!out && !power))
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "pulse_doppler shape is invalid");
+    if (!valid_window(window)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unsupported pulse_doppler window");
+    if (explicit_kind(window) != (explicit_window != nullptr))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_doppler window is invalid");
+    if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_doppler nfft must be a power of two in [2, 8192]");
+    return doppler(nfft, reinterpret_cast<const float2*>(x), 1, num_pulses, samples_per_pulse, window,
+                   explicit_window, reinterpret_cast<float2*>(out), power, nullptr, (hipStream_t)stream);
+}
+
+int zkx_cfar_alpha(double pfa, int reference, double* alpha)
+{
+    clear_last_error();
+    if (!alpha) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "cfar_alpha: null output");
+    if (reference < 1 || !std::isfinite(pfa) || pfa <= 0.0 || pfa > 1.0)
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "cfar_alpha requires reference >= 1 and pfa in (0,1]");
+    *alpha = static_cast<double>(reference) * (std::pow(pfa, -1.0 / static_cast<double>(reference)) - 1.0);
+    return ZKX_SUCCESS;
+}
+
+int zkx_ca_cfar_f32(const float* x, int rank, const int* shape, const int* guard, const int* reference, double pfa,
+                    float* threshold, uint8_t* detections, float* workspace, void* stream)
+{
+    clear_last_error();
+    hipStream_t st = (hipStream_t)stream;
+    if (rank > 2 || rank < 0) ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar supports only scalar, 1D, or 2D input");
+    if (rank == 0) {
+        // scalar: threshold 0, detection x > 0 (no reference window exists)
+        if (!x || !threshold || !detections) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar: null pointer");
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "ca_cfar scalar input is handled by the host wrapper");
+    }
+    if (!shape || !guard || !reference || !x || !threshold || !detections)
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar: null pointer");
+    for (int d = 0; d < rank; ++d) {
+        if (shape[d] < 0) ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar shape dimensions must be nonnegative");
+        if (guard[d] < 0 || reference[d] < 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar guard/reference cells are invalid");
+        if (static_cast<long long>(shape[d]) <= 2LL * (guard[d] + reference[d]))
+            ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar input dimension is too small");
+    }
+    if (!std::isfinite(pfa) || pfa <= 0.0 || pfa > 1.0) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar pfa must be in (0,1]");
+    const long long count = rank == 2 ? static_cast<long long>(shape[0]) * shape[1] : shape[0];
+    if (count > std::numeric_limits<int>::max()) ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar input size or capacity is invalid");
+    float* I = workspace;
+    if (!I) {
+        void* p = nullptr;
+        int rc = t_cfar_ws.reserve(sizeof(float) * static_cast<size_t>(count), &p);
+        if (rc) return rc;
+        I = static_cast<float*>(p);
+    }
+    if (rank == 1) {
+        const int n = shape[0], g = guard[0], r = reference[0];
+        const int cnt = 2 * r;
+        const float alpha = cfar_alpha_float(cnt, pfa);
+        hipLaunchKernelGGL(cfar1d_scan_kernel, dim3(1), dim3(64), 0, st, x, n, I);
+        ZKX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(cfar1d_threshold_kernel, dim3((n + 255) / 256), dim3(256), 0, st, x, I, n, g, r, alpha,
+                           cnt, threshold, detections);
+        ZKX_LAUNCH_CHECK();
+        return ZKX_SUCCESS;
+    }
+    return cfar_faithful_2d(x, nullptr, 1, shape[0], shape[1], guard[0], guard[1], reference[0], reference[1], pfa,
+                            threshold, detections, I, st);
+}
+
+int zkx_ca_cfar_window_f32(const float* x, int rows, int cols, int guard_rows, int guard_cols, int ref_rows,
+                           int ref_cols, double pfa, float* threshold, uint8_t* detections, void* stream)
+{
+    clear_last_error();
+    if (!x || !threshold || !detections) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar: null pointer");
+    int rc = validate_cfar2d(rows, cols, guard_rows, guard_cols, ref_rows, ref_cols, pfa);
+    if (rc) return rc;
+    return cfar_window_2d(x, 1, rows, cols, guard_rows, guard_cols, ref_rows, ref_cols, pfa, threshold, detections,
+                          (hipStream_t)stream);
+}
+
+size_t zkx_dwell_workspace_bytes(int batch, int num_pulses, int range_bins)
+{
+    const size_t cells = static_cast<size_t>(batch) * num_pulses * range_bins;
+    // range-compressed cube (c64) + power map (f32) + integral image (f32), 256-byte aligned
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return al(cells * sizeof(float2)) + al(cells * sizeof(float)) + al(cells * sizeof(float));
+}
+
+int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int range_bins,
+                        const zkx_complex64* spectrum, int doppler_window, int guard_doppler, int guard_range,
+                        int ref_doppler, int ref_range, double pfa, int cfar_mode, float* threshold,
+                        uint8_t* detections, zkx_complex64* rd_map, void* workspace, void* stream)
+{
+    clear_last_error();
+    hipStream_t st = (hipStream_t)stream;
+    const int P = num_pulses, R = range_bins;
+    if (batch < 1 || P < 2 || R < 2 || !x || !spectrum || !threshold || !detections || !workspace)
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "dwell batch shape is invalid");
+    if (!is_pow2(P) || !is_pow2(R) || P > kMaxLdsFft || R > kMaxLdsFft)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "dwell batch needs power-of-two pulses and range bins <= 8192");
+    if (!valid_window(doppler_window) || explicit_kind(doppler_window))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "dwell batch doppler window must be none, hann or hamming");
+    if (cfar_mode != 0 && cfar_mode != 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "cfar_mode must be 0 or 1");
+    int rc = validate_cfar2d(P, R, guard_doppler, guard_range, ref_doppler, ref_range, pfa);
+    if (rc) return rc;
+    const size_t cells = static_cast<size_t>(batch) * P * R;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    char* ws = static_cast<char*>(workspace);
+    float2* pc = reinterpret_cast<float2*>(ws);
+    float* pw = reinterpret_cast<float*>(ws + al(cells * sizeof(float2)));
+    float* I = reinterpret_cast<float*>(ws + al(cells * sizeof(float2)) + al(cells * sizeof(float)));
+    rc = pc_fused(R, reinterpret_cast<const float2*>(x), static_cast<long long>(batch) * P, R,
+                  reinterpret_cast<const float2*>(spectrum), pc, st);
+    if (rc) return rc;
+    rc = doppler(P, pc, batch, P, R, doppler_window, nullptr, reinterpret_cast<float2*>(rd_map), pw,
+                 cfar_mode == 0 ? I : nullptr, st);
+    if (rc) return rc;
+    if (cfar_mode == 0)
+        return cfar_faithful_2d(pw, I, batch, P, R, guard_doppler, guard_range, ref_doppler, ref_range, pfa, threshold,
+                                detections, I, st);
+    return cfar_window_2d(pw, batch, P, R, guard_doppler, guard_range, ref_doppler, ref_range, pfa, threshold,
+                          detections, st);
+}
+
+}  // extern "C"

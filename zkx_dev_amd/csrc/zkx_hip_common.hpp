@@ -1,0 +1,71 @@
+// Shared definitions for the gfx950 radar kernels: error codes, complex
+// helpers, launch checks.  Error numbers are the reference's public ErrorCode
+// values (ZKX/cusignal_cpp/include/cusignal/runtime/errors.h:18-41).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/zkx_radar.h"
+
+namespace zkx {
+
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__host__ __device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// Map a HIP runtime status onto the public error numbers.
+inline int hip_status_to_code(hipError_t e)
+{
+    if (e == hipSuccess) return ZKX_SUCCESS;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return ZKX_OUT_OF_MEMORY;
+    return ZKX_RUNTIME_ERROR;
+}
+
+// Thread-local record of the last failure, read by zkx_last_error_message().
+void set_last_error(int code, const char* msg);
+
+#define ZKX_LAUNCH_CHECK()                                                        \
+    do {                                                                          \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess) {                                                   \
+            ::zkx::set_last_error(::zkx::hip_status_to_code(e_), hipGetErrorString(e_)); \
+            return ::zkx::hip_status_to_code(e_);                                 \
+        }                                                                         \
+    } while (0)
+
+#define ZKX_HIP_CHECK(call)                                                       \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ::zkx::set_last_error(::zkx::hip_status_to_code(e_), hipGetErrorString(e_)); \
+            return ::zkx::hip_status_to_code(e_);                                 \
+        }                                                                         \
+    } while (0)
+
+#define ZKX_FAIL(code, msg)                                                       \
+    do {                                                                          \
+        ::zkx::set_last_error((code), (msg));                                     \
+        return (code);                                                            \
+    } while (0)
+
+inline bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
+inline int ilog2(long long n)
+{
+    int l = 0;
+    while ((1LL << l) < n) ++l;
+    return l;
+}
+
+}  // namespace zkx
