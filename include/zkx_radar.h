@@ -65,6 +65,11 @@ const char* zkx_last_error_message(void);
 int zkx_version(void);
 /* Largest power-of-two transform length one LDS-resident pass supports. */
 int zkx_max_lds_fft_length(void);
+/* Kernel variant used for an operation ("pulse_compression", ...), as chosen
+ * by the autotuner; variants compute identical results.  No counterpart in
+ * the reference (its launches have one fixed shape). */
+int zkx_set_kernel_variant(const char* op, int variant);
+int zkx_get_kernel_variant(const char* op);
 
 /* ---------------------------------------------------------------------- */
 /* radar operators                                                          */
@@ -181,6 +186,60 @@ int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int r
                         int guard_doppler, int guard_range, int ref_doppler, int ref_range,
                         double pfa, int cfar_mode, float* threshold, uint8_t* detections,
                         zkx_complex64* rd_map, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* polyphase filtering                                                      */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * upfirdn: y[n] = sum_k h[k] x_up[n*down - k], x_up the zero-stuffed input.
+ * Replaces upfirdn_device<float> (filtering_typed.h:744-748,
+ * filtering_typed.cpp:1046-1123, filtering_typed.cu:596-645) for float32
+ * lines, and adds complex64 lines (real taps, per component).
+ *   x laid out as [outer, n, inner] with the filtered axis in the middle;
+ *   out [outer, out_n, inner], out_n = floor(((n-1)*up + K-1)/down) + 1.
+ */
+long long zkx_upfirdn_out_len(long long n, long long taps, int up, int down);
+int zkx_upfirdn_f32(const float* h, int taps, const float* x, int outer, int n, int inner,
+                    int up, int down, float* out, void* stream);
+int zkx_upfirdn_c64(const float* h, int taps, const zkx_complex64* x, int outer, int n,
+                    int inner, int up, int down, zkx_complex64* out, void* stream);
+
+/*
+ * resample_poly with caller taps (already scaled: h*up as the reference's
+ * resample_poly_scaled_custom_value, filtering_typed.cpp:129-157, or the FP64
+ * design of :176-218 rounded to float).  Replaces resample_poly_device<float>
+ * (filtering_typed.h:765-770, filtering_typed.cpp:2717-2779,
+ * filtering_typed.cu:646-681).  up/down are reduced by their gcd here.
+ *   out_n = ceil(n*up/down); the polyphase phase schedule follows
+ *   prepare_resample_poly_layout (filtering_typed.cpp:933-1044).
+ */
+int zkx_resample_poly_f32(const float* taps, int ntaps, const float* x, int outer, int n,
+                          int inner, int up, int down, float* out, void* stream);
+int zkx_resample_poly_c64(const float* taps, int ntaps, const zkx_complex64* x, int outer,
+                          int n, int inner, int up, int down, zkx_complex64* out, void* stream);
+
+/*
+ * firfilter: y[n] = sum_{k<=K-1} b[k] x[n-k] (+ zi[n] for n < K-1), zf = the
+ * full-convolution tail.  Replaces firfilter_device<float> and
+ * firfilter_fp32_resident_device (filtering_typed.h:552-565,819-822,
+ * filtering_typed.cpp:1125-1224, filtering_typed.cu:117-184).
+ *   zi, zf: [outer, K-1, inner] or NULL (zi already broadcast by the caller).
+ */
+int zkx_firfilter_f32(const float* b, int taps, const float* x, int outer, int n, int inner,
+                      const float* zi, float* y, float* zf, void* stream);
+
+/*
+ * fftconvolve, full/same/valid (mode 0/1/2) linear convolution of two 1-D
+ * complex64 signals via the LDS FFT.  No C++ counterpart exists in the
+ * reference library (operator_contracts.cmake:10-63); semantics follow
+ * cuSignal's fftconvolve (cusignal/convolution/convolve.py:159-320).
+ * batch rows of in1 [batch, n1] are convolved with the one in2 [n2]
+ * (n2 <= 4096); long rows run as overlap-save segments of 8192 points.
+ */
+long long zkx_fftconvolve_out_len(long long n1, long long n2, int mode);
+int zkx_fftconvolve_c64(const zkx_complex64* in1, int batch, int n1, const zkx_complex64* in2,
+                        int n2, int mode, zkx_complex64* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,164 @@
+"""GPU parity of upfirdn / resample_poly / firfilter against the CPU oracle
+(faithful FP32 path and FP64), real and complex64 lines, any axis."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as orc  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def F():
+    from zkx_dev_amd import filtering
+    assert torch.cuda.is_available()
+    return filtering
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def per_component(fn, x):
+    """The complex extension applies the real operator to re and im."""
+    return fn(x.real) + 1j * fn(x.imag)
+
+
+@pytest.mark.parametrize("up,down,K,shape,axis", [
+    (1, 4, 255, (3, 5000), 1), (3, 2, 121, (2, 3000), -1), (2, 1, 5, (100,), 0),
+    (2, 3, 9, (100, 100), 0), (3, 2, 7, (32, 32), 1), (1, 1, 33, (4, 777), 1),
+    (4, 4, 13, (3, 513), 1), (5, 3, 40, (257,), 0), (7, 1, 3, (50,), 0), (1, 9, 64, (2, 999), 1)])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_upfirdn(F, up, down, K, shape, axis, cplx):
+    rng = np.random.default_rng(K * 31 + up * 7 + down)
+    h = rng.standard_normal(K).astype(np.float32)
+    x = rng.standard_normal(shape).astype(np.float32)
+    if cplx:
+        x = (x + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    y = F.upfirdn(dev(h), dev(x), up, down, axis).cpu().numpy()
+    f = lambda v: orc.upfirdn(h, v, up, down, axis)  # noqa: E731
+    ref = per_component(f, x) if cplx else f(x)
+    assert y.shape == ref.shape
+    assert orc.rel_l2(y, ref) < TOL
+    ref64 = per_component(lambda v: orc.upfirdn(h, v, up, down, axis, fp64_acc=True), x) if cplx \
+        else orc.upfirdn(h, x, up, down, axis, fp64_acc=True)
+    assert orc.rel_l2(y, ref64) < TOL
+
+
+def test_upfirdn_probe_cases(F):
+    """The reference probe's deterministic inputs (seeded integer patterns)."""
+    for rank, rows, cols, taps, axis, up, down, seed in [(1, 100, 1, 5, 0, 2, 1, 84410001),
+                                                         (2, 100, 100, 9, 0, 2, 3, 84410003),
+                                                         (2, 32, 32, 7, 1, 3, 2, 84410002)]:
+        h = orc.upfirdn_probe_input(taps, seed + 1, "FP32")
+        x = orc.upfirdn_probe_input(rows * (cols if rank == 2 else 1), seed, "FP32")
+        shape = (rows,) if rank == 1 else (rows, cols)
+        y = F.upfirdn(dev(h), dev(x.reshape(shape)), up, down, axis).cpu().numpy()
+        ref = orc.upfirdn(h, x.reshape(shape), up, down, axis).astype(np.float32)
+        assert np.array_equal(y, ref)  # small integer multiples of 1/4: exact
+
+
+@pytest.mark.parametrize("up,down,K,shape,axis", [
+    (3, 2, 121, (2, 4000), 1), (2, 3, 5, (100, 100), 0), (2, 1, 21, (10, 10), 1),
+    (160, 147, 301, (3000,), 0), (1, 4, 41, (5, 2048), 1), (5, 2, 51, (2, 1111), -1)])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_resample_poly_custom(F, up, down, K, shape, axis, cplx):
+    rng = np.random.default_rng(up * 100 + down + K)
+    taps = rng.standard_normal(K).astype(np.float32)
+    x = rng.standard_normal(shape).astype(np.float32)
+    if cplx:
+        x = (x + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    y = F.resample_poly(dev(x), up, down, axis, coefficients=taps).cpu().numpy()
+    f = lambda v: orc.resample_poly_f32(v, up, down, axis, taps=taps)[0]  # noqa: E731
+    ref = per_component(f, x) if cplx else f(x)
+    assert y.shape == ref.shape
+    assert orc.rel_l2(y, ref) < TOL
+
+
+@pytest.mark.parametrize("up,down", [(2, 1), (3, 2), (1, 3), (4, 6)])
+def test_resample_poly_default_design(F, up, down):
+    rng = np.random.default_rng(up + 10 * down)
+    x = rng.standard_normal((3, 500)).astype(np.float32)
+    y = F.resample_poly(dev(x), up, down, axis=1)
+    assert y.dtype == torch.float64  # FP64 storage, as the reference's default design
+    ref, dt = orc.resample_poly_f32(x, up, down, axis=1)
+    assert dt == "FP64"
+    assert orc.rel_l2(y.cpu().numpy(), ref) < TOL
+    np.testing.assert_allclose(F.design_resample_poly_coefficients(up, down),
+                               orc.resample_design(up, down), rtol=1e-14, atol=1e-15)
+
+
+def test_resample_poly_identity(F):
+    x = dev(np.arange(12, dtype=np.float32).reshape(3, 4))
+    assert torch.equal(F.resample_poly(x, 3, 3, axis=1), x)
+
+
+@pytest.mark.parametrize("K,shape,axis", [(9, (10000,), 0), (255, (3, 4096), 1), (31, (64, 17), 0),
+                                          (1, (100,), 0), (128, (2, 300), -1)])
+def test_firfilter(F, K, shape, axis):
+    rng = np.random.default_rng(K)
+    b = rng.standard_normal(K).astype(np.float32)
+    x = rng.uniform(-1, 1, shape).astype(np.float32)
+    y = F.firfilter(dev(b), dev(x), axis).cpu().numpy()
+    ref, _ = orc.firfilter(b, x, axis)
+    assert orc.rel_l2(y, ref) < TOL
+
+
+@pytest.mark.parametrize("K,shape,axis,zshape", [(9, (200,), 0, (8,)), (5, (4, 50), 1, (4, 4)),
+                                                 (5, (4, 50), 1, (1, 4)), (7, (30, 3), 0, (6, 1))])
+def test_firfilter_zi_zf(F, K, shape, axis, zshape):
+    rng = np.random.default_rng(K + len(shape))
+    b = rng.standard_normal(K).astype(np.float32)
+    x = rng.standard_normal(shape).astype(np.float32)
+    zi = rng.standard_normal(zshape).astype(np.float32)
+    y, zf = F.firfilter(dev(b), dev(x), axis, zi=dev(zi))
+    ref_y, ref_zf = orc.firfilter(b, x, axis, zi=zi)
+    assert orc.rel_l2(y.cpu().numpy(), ref_y) < TOL
+    assert orc.rel_l2(zf.cpu().numpy(), ref_zf) < TOL
+
+
+def test_firfilter_probe_case(F):
+    x = orc.firfilter_probe_signal(10000, 29, "FP32")
+    b = np.array([4 if i == 4 else (1 if i % 2 else 0) for i in range(9)], np.float32)
+    y = F.firfilter(dev(b), dev(x)).cpu().numpy()
+    ref, _ = orc.firfilter(b, x)
+    assert orc.rel_l2(y, ref) < 1e-6
+
+
+def test_filter_validation(F):
+    x = dev(np.ones(10, np.float32))
+    with pytest.raises(ValueError):
+        F.upfirdn(dev(np.zeros(0, np.float32)), x)
+    with pytest.raises(ValueError):
+        F.upfirdn(dev(np.ones(3, np.float32)), x, 0, 1)
+    with pytest.raises(ValueError):
+        F.upfirdn(dev(np.ones(3, np.float32)), x, 1, 1, axis=2)
+    with pytest.raises(ValueError):
+        F.resample_poly(x, 0, 2)
+    with pytest.raises(ValueError):
+        F.firfilter(dev(np.ones(3, np.float32)), x, axis=3)
+
+
+@pytest.mark.parametrize("n1,n2,batch", [(1000, 64, 3), (50, 50, 1), (64, 1000, 2), (20000, 255, 2),
+                                         (3, 7, 4), (100000, 4096, 1), (8193, 1, 1)])
+@pytest.mark.parametrize("mode", ["full", "same", "valid"])
+def test_fftconvolve(F, n1, n2, batch, mode):
+    """Against scipy's FP64 fftconvolve (the library cuSignal's tests pin to)
+    and a direct numpy convolution."""
+    from scipy import signal
+    rng = np.random.default_rng(n1 + n2)
+    a = (rng.standard_normal((batch, n1)) + 1j * rng.standard_normal((batch, n1))).astype(np.complex64)
+    b = (rng.standard_normal(n2) + 1j * rng.standard_normal(n2)).astype(np.complex64)
+    y = F.fftconvolve(dev(a), dev(b), mode).cpu().numpy()
+    ref = np.stack([signal.fftconvolve(a[i].astype(np.complex128), b.astype(np.complex128), mode)
+                    for i in range(batch)])
+    assert y.shape == ref.shape
+    assert orc.rel_l2(y, ref) < TOL
+    if n1 * n2 <= 1_000_000:
+        direct = np.stack([np.convolve(a[i].astype(np.complex128), b.astype(np.complex128), mode)
+                           for i in range(batch)])
+        if mode == "full" or n1 >= n2:  # numpy's same/valid follow the longer input
+            assert orc.rel_l2(y, direct) < TOL

@@ -26,6 +26,8 @@ SIGNATURES = {
     "zkx_last_error_message": (ctypes.c_char_p, []),
     "zkx_version": (I, []),
     "zkx_max_lds_fft_length": (I, []),
+    "zkx_set_kernel_variant": (I, [ctypes.c_char_p, I]),
+    "zkx_get_kernel_variant": (I, [ctypes.c_char_p]),
     "zkx_pulse_compression_c64": (I, [P, I, I, P, I, I, I, I, P, P, P]),
     "zkx_pc_template_prepare": (I, [P, I, I, I, I, P, P, P]),
     "zkx_pulse_compression_prepared_c64": (I, [P, I, I, P, I, P, P]),
@@ -37,6 +39,14 @@ SIGNATURES = {
     "zkx_widen_f32_to_f64": (I, [P, P, SZ, P]),
     "zkx_dwell_workspace_bytes": (SZ, [I, I, I]),
     "zkx_dwell_batch_c64": (I, [P, I, I, I, P, I, I, I, I, I, D, I, P, P, P, P, P]),
+    "zkx_upfirdn_out_len": (LL, [LL, LL, I, I]),
+    "zkx_upfirdn_f32": (I, [P, I, P, I, I, I, I, I, P, P]),
+    "zkx_upfirdn_c64": (I, [P, I, P, I, I, I, I, I, P, P]),
+    "zkx_resample_poly_f32": (I, [P, I, P, I, I, I, I, I, P, P]),
+    "zkx_resample_poly_c64": (I, [P, I, P, I, I, I, I, I, P, P]),
+    "zkx_firfilter_f32": (I, [P, I, P, I, I, I, P, P, P, P]),
+    "zkx_fftconvolve_out_len": (LL, [LL, LL, I]),
+    "zkx_fftconvolve_c64": (I, [P, I, I, P, I, I, P, P]),
 }
 
 _lib = None

@@ -142,11 +142,17 @@ __device__ __forceinline__ void passes(float2 (&v)[E], const Lds& lds, const flo
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int jm = (t + b * T) & (Ns - 1);
+            // one table read per butterfly; powers w^r by a product tree of
+            // depth <= 4 (w^r = w^p * w^(r-p), p the largest power of two <= r)
+            float2 wp[R];
+            wp[1] = tw[jm * (N / (Ns * R))];
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const float2 w = tw[jm * r * (N / (Ns * R))];
-                v[b + r * NB] = INV ? cmulc(v[b + r * NB], w) : cmul(v[b + r * NB], w);
+            for (int r = 2; r < R; ++r) {
+                const int p = (r & 8) ? 8 : (r & 4) ? 4 : (r & 2) ? 2 : 1;
+                wp[r] = (p == r) ? cmul(wp[r / 2], wp[r / 2]) : cmul(wp[p], wp[r - p]);
             }
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b + r * NB] = INV ? cmulc(v[b + r * NB], wp[r]) : cmul(v[b + r * NB], wp[r]);
         }
     }
 #pragma unroll
@@ -182,6 +188,18 @@ __device__ __forceinline__ void transform(float2 (&v)[E], const Lds& lds, const 
 
 // Elements per thread used for each size: E=16 from N=16 up, N itself below.
 __host__ __device__ constexpr int elems_for(int n) { return n >= 16 ? 16 : n; }
+
+// Launch shape of the row kernels: FPB signals of N points per block, each
+// owned by T = N/E threads, one padded LDS slot per signal.
+template <int N>
+struct RowCfg {
+    static constexpr int E = elems_for(N);
+    static constexpr int T = N / E;
+    static constexpr int FPB = T >= 256 ? 1 : 256 / T;
+    static constexpr int THREADS = T * FPB;
+    static constexpr int SLOT_FLOATS = RowLds::floats_for(N);
+    static constexpr size_t LDS_BYTES = sizeof(float) * 2 * SLOT_FLOATS * FPB;
+};
 
 }  // namespace fft
 }  // namespace zkx

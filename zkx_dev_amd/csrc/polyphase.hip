@@ -1,0 +1,382 @@
+// Polyphase FIR on gfx950: upfirdn, resample_poly and firfilter for float32
+// and complex64 lines with real taps.
+//
+//   y[n] = sum_k h[k] * x_up[m(n) - k],  m(n) = n*down + m_off,
+//   x_up[u] = x[u/up] when up | u, else 0.
+//
+// upfirdn has m_off = 0 (ZKX/cusignal_cpp/src/operators/filtering/filtering_typed.cpp:2548-2616);
+// resample_poly has m_off = start*down - prefix_zeros with gcd-reduced up/down
+// (filtering_typed.cpp:933-1044, 2617-2715); firfilter is up = down = 1
+// (filtering_typed.cpp:1536-1592).
+//
+// Output n has phase phi = m mod up and base q = m div up, and
+//   y[n] = sum_j h[phi + j*up] * x[q - j].
+// With Pn = up/gcd(up,down) and D = down/gcd(up,down), the outputs
+// n = T*Pn + r (fixed residue r) share one phase and their bases advance by D:
+// each residue is a plain decimate-by-D FIR.  A block covers TT consecutive T
+// for all residues (TT*Pn consecutive outputs).  A wavefront takes one
+// residue at a time, so its taps are uniform, and each lane register-blocks M
+// consecutive T: one LDS read of x feeds M FMAs, with the taps of the residue
+// laid out in LDS as a zero-padded [shift][M] table read as broadcast
+// vectors.  x for the tile and the outputs are staged through LDS so that HBM
+// reads and writes stay coalesced.
+#include "zkx_hip_common.hpp"
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <limits>
+#include <numeric>
+
+namespace zkx {
+namespace {
+
+template <class V>
+struct VOps;
+template <>
+struct VOps<float> {
+    static __device__ __forceinline__ float zero() { return 0.0f; }
+    static __device__ __forceinline__ float fma(float h, float x, float acc) { return __builtin_fmaf(h, x, acc); }
+};
+template <>
+struct VOps<float2> {
+    static __device__ __forceinline__ float2 zero() { return make_float2(0.f, 0.f); }
+    static __device__ __forceinline__ float2 fma(float h, float2 x, float2 acc)
+    {
+        return make_float2(__builtin_fmaf(h, x.x, acc.x), __builtin_fmaf(h, x.y, acc.y));
+    }
+};
+
+struct PolyPlan {
+    int up, down;       // as applied (upfirdn: as given; resample: gcd-reduced)
+    int Pn, D;          // residue period and per-residue input stride
+    long long m_off;    // m(n) = n*down + m_off, m_off >= 0
+    int K, Jmax;        // taps, taps per phase (max)
+    long long n_in, n_out;
+};
+
+constexpr int kWaves = 4;
+
+// Shared layout: taps table [Pn][SL][M] floats, x tile [XL], out tile [TT*Pn].
+template <class V, int M>
+__global__ void __launch_bounds__(64 * kWaves)
+polyphase_kernel(const float* __restrict__ h, const V* __restrict__ x, V* __restrict__ y, PolyPlan p, int TT, int XL)
+{
+    extern __shared__ __align__(16) unsigned char smraw[];
+    const int SL = p.Jmax + (M - 1) * p.D;
+    float* tab = reinterpret_cast<float*>(smraw);
+    V* xs = reinterpret_cast<V*>(smraw + ((sizeof(float) * p.Pn * SL * M + 15) / 16) * 16);
+    V* ys = xs + XL;
+    const int tid = threadIdx.x;
+    const long long line = blockIdx.y;
+    const V* xl = x + line * p.n_in;
+    V* yl = y + line * p.n_out;
+    const long long T0 = static_cast<long long>(blockIdx.x) * TT;
+    // m of residue r at T: T*up*D + m_r, m_r = r*down + m_off
+    const long long q0 = p.m_off / p.up;  // base of residue 0 at T = 0
+    const long long xstart = T0 * p.D + q0 - (p.Jmax - 1);
+    // taps table: tab[r][s][i] = h[phi_r + j*up], j = i*D - s + Jmax - 1, zero outside the phase
+    for (int e = tid; e < p.Pn * SL * M; e += blockDim.x) {
+        const int r = e / (SL * M), rem = e % (SL * M), s = rem / M, i = rem % M;
+        const long long mr = static_cast<long long>(r) * p.down + p.m_off;
+        const int phi = static_cast<int>(mr % p.up);
+        const int j = i * p.D - s + p.Jmax - 1;
+        const long long k = static_cast<long long>(phi) + static_cast<long long>(j) * p.up;
+        tab[e] = (j >= 0 && k < p.K) ? h[k] : 0.0f;
+    }
+    for (int e = tid; e < XL; e += blockDim.x) {
+        const long long u = xstart + e;
+        xs[e] = (u >= 0 && u < p.n_in) ? xl[u] : VOps<V>::zero();
+    }
+    __syncthreads();
+    const int wave = tid / 64, lane = tid % 64;
+    const int NC = TT / (64 * M);
+    for (int task = wave; task < p.Pn * NC; task += kWaves) {
+        const int r = task / NC, tc = task % NC;
+        const long long mr = static_cast<long long>(r) * p.down + p.m_off;
+        const int dq = static_cast<int>(mr / p.up - q0);
+        const int t0 = tc * 64 * M + lane * M;
+        const float* tr = tab + r * SL * M;
+        const V* xb = xs + t0 * p.D + dq;
+        V acc[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) acc[i] = VOps<V>::zero();
+        for (int s = SL - 1; s >= 0; --s) {
+            const V xv = xb[s];
+            const float* trs = tr + s * M;
+#pragma unroll
+            for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[i], xv, acc[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) ys[(t0 + i) * p.Pn + r] = acc[i];
+    }
+    __syncthreads();
+    const long long nbase = T0 * p.Pn;
+    for (int e = tid; e < TT * p.Pn; e += blockDim.x) {
+        const long long n = nbase + e;
+        if (n < p.n_out) yl[n] = ys[e];
+    }
+}
+
+// Any layout [outer, n, inner]: one thread per output, taps of its phase in order.
+template <class V>
+__global__ void polyphase_generic_kernel(const float* __restrict__ h, const V* __restrict__ x, V* __restrict__ y,
+                                         PolyPlan p, int outer, int inner)
+{
+    const long long total = static_cast<long long>(outer) * inner * p.n_out;
+    for (long long g = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long in = g % inner;
+        const long long n = (g / inner) % p.n_out;
+        const long long o = g / (static_cast<long long>(inner) * p.n_out);
+        const long long m = n * p.down + p.m_off;
+        const int phi = static_cast<int>(m % p.up);
+        const long long q = m / p.up;
+        const V* xl = x + o * p.n_in * inner + in;
+        V acc = VOps<V>::zero();
+        for (long long k = phi, j = 0; k < p.K; k += p.up, ++j) {
+            const long long u = q - j;
+            if (u < 0) break;
+            if (u < p.n_in) acc = VOps<V>::fma(h[k], xl[u * inner], acc);
+        }
+        y[o * p.n_out * inner + n * inner + in] = acc;
+    }
+}
+
+// firfilter state: y[n] += zi[n] (n < K-1) and zf[k] = full[N+k] (+ zi[N+k]).
+template <class V>
+__global__ void firfilter_state_kernel(const V* __restrict__ full, int outer, long long n, int inner, int state,
+                                       const V* __restrict__ zi, V* __restrict__ yout, V* __restrict__ zf)
+{
+    const long long total = static_cast<long long>(outer) * inner * (n + state);
+    for (long long g = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; g < total;
+         g += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long in = g % inner;
+        const long long f = (g / inner) % (n + state);
+        const long long o = g / (static_cast<long long>(inner) * (n + state));
+        V v = full[o * (n + state) * inner + f * inner + in];
+        if (zi && f < state) {
+            const V z = zi[o * state * inner + f * inner + in];
+            v = VOps<V>::fma(1.0f, z, v);
+        }
+        if (f < n) {
+            yout[o * n * inner + f * inner + in] = v;
+        } else if (zf) {
+            zf[o * state * inner + (f - n) * inner + in] = v;
+        }
+    }
+}
+
+unsigned grid_for(long long n)
+{
+    const long long b = (n + 255) / 256;
+    return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(b, 65535)));
+}
+
+template <class V, int M>
+int launch_tiled(const float* h, const V* x, V* y, const PolyPlan& p, long long lines, hipStream_t st, bool* done)
+{
+    *done = false;
+    const int NC = kWaves / std::gcd(p.Pn, kWaves);
+    const int TT = NC * 64 * M;
+    const int SL = p.Jmax + (M - 1) * p.D;
+    const long long q_last = ((static_cast<long long>(p.Pn) - 1) * p.down + p.m_off) / p.up - p.m_off / p.up;
+    const long long XL = static_cast<long long>(TT - 1) * p.D + q_last + p.Jmax + (M - 1) * p.D + 1;
+    const size_t tab_bytes = ((sizeof(float) * static_cast<size_t>(p.Pn) * SL * M + 15) / 16) * 16;
+    const size_t lds = tab_bytes + sizeof(V) * (static_cast<size_t>(XL) + static_cast<size_t>(TT) * p.Pn);
+    if (lds > 150 * 1024 || XL > std::numeric_limits<int>::max()) return ZKX_SUCCESS;  // caller falls back
+    if (lds > 64 * 1024)
+        ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(polyphase_kernel<V, M>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const long long Tcount = (p.n_out + p.Pn - 1) / p.Pn;
+    const long long blocks = (Tcount + TT - 1) / TT;
+    if (blocks > std::numeric_limits<int>::max() || lines > 65535) return ZKX_SUCCESS;
+    hipLaunchKernelGGL((polyphase_kernel<V, M>), dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(lines)),
+                       dim3(64 * kWaves), lds, st, h, x, y, p, TT, static_cast<int>(XL));
+    ZKX_LAUNCH_CHECK();
+    *done = true;
+    return ZKX_SUCCESS;
+}
+
+// Register blocking depth: enough reuse that the FMAs, not LDS, set the pace,
+// without padding waste ((M-1)*D zero taps per output group) outgrowing it.
+int pick_m(const PolyPlan& p)
+{
+    return (p.Jmax >= 16 * p.D) ? 8 : 4;
+}
+
+template <class V>
+int run_polyphase(const float* h, const V* x, V* y, const PolyPlan& p, int outer, int inner, hipStream_t st)
+{
+    if (p.n_out <= 0 || outer <= 0 || inner <= 0) return ZKX_SUCCESS;
+    if (inner == 1) {
+        bool done = false;
+        int rc = pick_m(p) == 8 ? launch_tiled<V, 8>(h, x, y, p, outer, st, &done)
+                                : launch_tiled<V, 4>(h, x, y, p, outer, st, &done);
+        if (rc) return rc;
+        if (done) return ZKX_SUCCESS;
+    }
+    const long long total = static_cast<long long>(outer) * inner * p.n_out;
+    hipLaunchKernelGGL((polyphase_generic_kernel<V>), dim3(grid_for(total)), dim3(256), 0, st, h, x, y, p, outer,
+                       inner);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+PolyPlan make_plan(int up, int down, long long m_off, int K, long long n_in, long long n_out)
+{
+    PolyPlan p;
+    p.up = up;
+    p.down = down;
+    const int g = std::gcd(up, down);
+    p.Pn = up / g;
+    p.D = down / g;
+    p.m_off = m_off;
+    p.K = K;
+    p.Jmax = (K + up - 1) / up;
+    p.n_in = n_in;
+    p.n_out = n_out;
+    return p;
+}
+
+int check_lines(int outer, int n, int inner)
+{
+    if (outer < 0 || n < 0 || inner < 0) ZKX_FAIL(ZKX_INVALID_SHAPE, "input shape");
+    const long long total = static_cast<long long>(outer) * n * inner;
+    if (total > std::numeric_limits<int>::max()) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "input indexing");
+    return ZKX_SUCCESS;
+}
+
+long long upfirdn_len(long long n, long long K, int up, int down)
+{
+    const long long num = (n - 1) * up + K - 1;
+    long long q = num / down;
+    if (num % down < 0) --q;
+    return q + 1;
+}
+
+template <class V>
+int upfirdn_impl(const float* h, int taps, const V* x, int outer, int n, int inner, int up, int down, V* out,
+                 hipStream_t st)
+{
+    if (taps < 1) ZKX_FAIL(ZKX_INVALID_SHAPE, "upfirdn coefficients");
+    if (up < 1 || down < 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "upfirdn factors");
+    int rc = check_lines(outer, n, inner);
+    if (rc) return rc;
+    if (!h || (!x && n > 0) || !out) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "upfirdn: null pointer");
+    const long long out_n = upfirdn_len(n, taps, up, down);
+    if (out_n < 0 || out_n > std::numeric_limits<int>::max() ||
+        static_cast<long long>(outer) * inner * out_n > std::numeric_limits<int>::max())
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "upfirdn output sample count");
+    return run_polyphase<V>(h, x, out, make_plan(up, down, 0, taps, n, out_n), outer, inner, st);
+}
+
+template <class V>
+int resample_impl(const float* taps, int ntaps, const V* x, int outer, int n, int inner, int up, int down, V* out,
+                  hipStream_t st)
+{
+    if (up < 1 || down < 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "resample_poly factors");
+    int rc = check_lines(outer, n, inner);
+    if (rc) return rc;
+    const int g = std::gcd(up, down);
+    const int u = up / g, d = down / g;
+    if (u == 1 && d == 1) {
+        const size_t bytes = sizeof(V) * static_cast<size_t>(outer) * n * inner;
+        if (bytes) ZKX_HIP_CHECK(hipMemcpyAsync(out, x, bytes, hipMemcpyDeviceToDevice, st));
+        return ZKX_SUCCESS;
+    }
+    if (ntaps < 1 || !taps) ZKX_FAIL(ZKX_INVALID_SHAPE, "resample_poly coefficients");
+    if (!out || (!x && n > 0)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "resample_poly: null pointer");
+    // prepare_resample_poly_layout (filtering_typed.cpp:1003-1040)
+    const long long out_n = (static_cast<long long>(n) * u + d - 1) / d;
+    if (out_n > std::numeric_limits<int>::max() ||
+        static_cast<long long>(outer) * inner * out_n > std::numeric_limits<int>::max())
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "resample_poly output sample count");
+    const int half = (ntaps - 1) / 2;
+    const int rem = ((half % d) + d) % d;
+    const int prefix = d - rem;
+    const long long start = (static_cast<long long>(half) + prefix) / d;
+    const long long m_off = start * d - prefix;  // = half, since d | half + prefix
+    return run_polyphase<V>(taps, x, out, make_plan(u, d, m_off, ntaps, n, out_n), outer, inner, st);
+}
+
+thread_local ThreadWorkspace t_fir_ws;
+
+int firfilter_impl(const float* b, int taps, const float* x, int outer, int n, int inner, const float* zi, float* y,
+                   float* zf, hipStream_t st)
+{
+    if (taps < 1 || !b) ZKX_FAIL(ZKX_INVALID_SHAPE, "firfilter coefficients");
+    if (outer < 1 || n < 1 || inner < 1) ZKX_FAIL(ZKX_INVALID_SHAPE, "firfilter input shape");
+    int rc = check_lines(outer, n, inner);
+    if (rc) return rc;
+    if (!x || !y) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "firfilter: null pointer");
+    const int state = taps - 1;
+    if (!zi && !zf) {
+        // y is the first n outputs of the full convolution
+        return run_polyphase<float>(b, x, y, make_plan(1, 1, 0, taps, n, n), outer, inner, st);
+    }
+    // full convolution into scratch, then zi / zf split
+    const long long full_n = static_cast<long long>(n) + state;
+    void* ws = nullptr;
+    rc = t_fir_ws.reserve(sizeof(float) * static_cast<size_t>(outer) * inner * full_n, &ws);
+    if (rc) return rc;
+    float* full = static_cast<float*>(ws);
+    rc = run_polyphase<float>(b, x, full, make_plan(1, 1, 0, taps, n, full_n), outer, inner, st);
+    if (rc) return rc;
+    const long long total = static_cast<long long>(outer) * inner * full_n;
+    hipLaunchKernelGGL((firfilter_state_kernel<float>), dim3(grid_for(total)), dim3(256), 0, st, full, outer,
+                       static_cast<long long>(n), inner, state, zi, y, zf);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+}  // namespace
+}  // namespace zkx
+
+using namespace zkx;
+
+extern "C" {
+
+long long zkx_upfirdn_out_len(long long n, long long taps, int up, int down)
+{
+    if (taps < 1 || up < 1 || down < 1 || n < 0) return -1;
+    return upfirdn_len(n, taps, up, down);
+}
+
+int zkx_upfirdn_f32(const float* h, int taps, const float* x, int outer, int n, int inner, int up, int down,
+                    float* out, void* stream)
+{
+    clear_last_error();
+    return upfirdn_impl<float>(h, taps, x, outer, n, inner, up, down, out, (hipStream_t)stream);
+}
+
+int zkx_upfirdn_c64(const float* h, int taps, const zkx_complex64* x, int outer, int n, int inner, int up, int down,
+                    zkx_complex64* out, void* stream)
+{
+    clear_last_error();
+    return upfirdn_impl<float2>(h, taps, reinterpret_cast<const float2*>(x), outer, n, inner, up, down,
+                                reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
+int zkx_resample_poly_f32(const float* taps, int ntaps, const float* x, int outer, int n, int inner, int up, int down,
+                          float* out, void* stream)
+{
+    clear_last_error();
+    return resample_impl<float>(taps, ntaps, x, outer, n, inner, up, down, out, (hipStream_t)stream);
+}
+
+int zkx_resample_poly_c64(const float* taps, int ntaps, const zkx_complex64* x, int outer, int n, int inner, int up,
+                          int down, zkx_complex64* out, void* stream)
+{
+    clear_last_error();
+    return resample_impl<float2>(taps, ntaps, reinterpret_cast<const float2*>(x), outer, n, inner, up, down,
+                                 reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
+int zkx_firfilter_f32(const float* b, int taps, const float* x, int outer, int n, int inner, const float* zi,
+                      float* y, float* zf, void* stream)
+{
+    clear_last_error();
+    return firfilter_impl(b, taps, x, outer, n, inner, zi, y, zf, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -12,8 +12,10 @@
 #include "fft_lds.hpp"
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
+#include <string>
 
 namespace zkx {
 namespace {
@@ -31,15 +33,7 @@ __host__ __device__ inline double window_value(int kind, int i, int len)
 // ---------------------------------------------------------------------------
 // row FFT kernels (one signal per slot, FPB slots per block)
 // ---------------------------------------------------------------------------
-template <int N>
-struct RowCfg {
-    static constexpr int E = fft::elems_for(N);
-    static constexpr int T = N / E;
-    static constexpr int FPB = T >= 256 ? 1 : 256 / T;  // signals per block
-    static constexpr int THREADS = T * FPB;
-    static constexpr int SLOT_FLOATS = fft::RowLds::floats_for(N);
-    static constexpr size_t LDS_BYTES = sizeof(float) * 2 * SLOT_FLOATS * FPB;
-};
+using fft::RowCfg;
 
 // Template spectrum H = FFT_N(h~), h~ = window*h / ||window*h|| (normalize)
 // truncated/zero-padded to N.  One block, FPB slots but only slot 0 works.
@@ -117,6 +111,58 @@ pc_fused_kernel(const float2* __restrict__ x, long long rows, int S, const float
         float2* dst = y + row * static_cast<long long>(N);
 #pragma unroll
         for (int k = 0; k < C::E; ++k) dst[t + k * C::T] = cscale(v[k], s);
+    }
+}
+
+// Persistent form: each block walks rows blockIdx.x*FPB+slot, +gridDim.x*FPB,
+// ... with the next row's samples loaded into registers while the current row
+// is transformed, and the template spectrum held in registers for the whole
+// walk, so HBM latency overlaps the two transforms.
+template <int N, int MINW>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS, MINW)
+pc_persistent_kernel(const float2* __restrict__ x, long long rows, int S, const float2* __restrict__ H,
+                     const float2* __restrict__ tw, float2* __restrict__ y)
+{
+    using C = RowCfg<N>;
+    extern __shared__ float lds[];
+    const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
+    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    const long long stride = static_cast<long long>(gridDim.x) * C::FPB;
+    long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
+    float2 h[C::E], nxt[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) h[k] = H[t + k * C::T];
+    auto fetch = [&](long long r) {
+        const bool act = r < rows;
+        const float2* src = x + (act ? r : 0) * static_cast<long long>(S);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const int i = t + k * C::T;
+            nxt[k] = (act && i < S) ? src[i] : make_float2(0.f, 0.f);
+        }
+    };
+    fetch(row);
+    // every slot of the block runs the same number of iterations (barriers)
+    const long long first = static_cast<long long>(blockIdx.x) * C::FPB;
+    for (long long base = first; base < rows; base += stride, row += stride) {
+        float2 v[C::E];
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) v[k] = nxt[k];
+        if (base + stride < rows) fetch(row + stride);
+        // opaque copy of t: keeps the compiler from hoisting every pass's
+        // twiddles out of the row loop into registers
+        int tt = t;
+        asm volatile("" : "+v"(tt));
+        fft::transform<N, C::E, false>(v, L, tw, tt);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], h[k]);
+        fft::transform<N, C::E, true>(v, L, tw, tt);
+        if (row < rows) {
+            const float s = 1.0f / static_cast<float>(N);
+            float2* dst = y + row * static_cast<long long>(N);
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) dst[t + k * C::T] = cscale(v[k], s);
+        }
     }
 }
 
@@ -210,10 +256,17 @@ doppler_kernel(const float2* __restrict__ x, int P, int R, int window,
         for (int k = 0; k < C::E; ++k) lds[(t + k * C::T) * C::TC + s] = pw[k];
         __syncthreads();
         if (t == 0 && active) {
+            constexpr int CH = N < 16 ? N : 16;
             float run = 0.0f;
-            for (int f = 0; f < N; ++f) {
-                run = __fadd_rn(run, lds[f * C::TC + s]);
-                colsum[obase + static_cast<long long>(f) * R + col] = run;
+            for (int f0 = 0; f0 < N; f0 += CH) {
+                float v[CH];
+#pragma unroll
+                for (int j = 0; j < CH; ++j) v[j] = lds[(f0 + j) * C::TC + s];
+#pragma unroll
+                for (int j = 0; j < CH; ++j) {
+                    run = __fadd_rn(run, v[j]);
+                    colsum[obase + static_cast<long long>(f0 + j) * R + col] = run;
+                }
             }
         }
     }
@@ -397,11 +450,17 @@ cfar_band_kernel(const float* __restrict__ x, const float* __restrict__ C, int r
             const float* st = stage + tid * SW;
             float* rw = irng + tid * RWS;
             if (width == kBandCW) {
+                // all reads first, then the dependent adds, then the writes
+                float v[kBandCW];
+#pragma unroll
+                for (int j = 0; j < kBandCW; ++j) v[j] = st[j];
 #pragma unroll
                 for (int j = 0; j < kBandCW; ++j) {
-                    run = __fadd_rn(run, st[j]);
-                    rw[(c0 + j) & (ring - 1)] = run;
+                    run = __fadd_rn(run, v[j]);
+                    v[j] = run;
                 }
+#pragma unroll
+                for (int j = 0; j < kBandCW; ++j) rw[(c0 + j) & (ring - 1)] = v[j];
             } else {
                 for (int j = 0; j < width; ++j) {
                     run = __fadd_rn(run, st[j]);
@@ -575,6 +634,27 @@ int launch_pc_template(const float2* h, int taps, int normalize, int window, con
     return ZKX_SUCCESS;
 }
 
+// Kernel variant per operation, chosen by the autotuner (zkx_set_kernel_variant).
+int g_pc_variant = 0;
+
+template <int N, int MINW>
+int launch_pc_persistent(const float2* x, long long rows, int S, const float2* H, const float2* tw, float2* y,
+                         int per_cu, hipStream_t st)
+{
+    using C = RowCfg<N>;
+    int rc = allow_lds(pc_persistent_kernel<N, MINW>, C::LDS_BYTES);
+    if (rc) return rc;
+    int dev = 0, cus = 256;
+    ZKX_HIP_CHECK(hipGetDevice(&dev));
+    ZKX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long need = (rows + C::FPB - 1) / C::FPB;
+    const long long blocks = std::min<long long>(need, static_cast<long long>(cus) * per_cu);
+    hipLaunchKernelGGL((pc_persistent_kernel<N, MINW>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
+                       C::LDS_BYTES, st, x, rows, S, H, tw, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
 template <int N>
 int launch_pc_fused(const float2* x, long long rows, int S, const float2* H, float2* y, hipStream_t st)
 {
@@ -582,6 +662,9 @@ int launch_pc_fused(const float2* x, long long rows, int S, const float2* H, flo
     const float2* tw = nullptr;
     int rc = twiddles(N, &tw);
     if (rc) return rc;
+    if (g_pc_variant == 1) return launch_pc_persistent<N, 2>(x, rows, S, H, tw, y, 2, st);
+    if (g_pc_variant == 2) return launch_pc_persistent<N, 2>(x, rows, S, H, tw, y, 3, st);
+    if (g_pc_variant == 3) return launch_pc_persistent<N, 1>(x, rows, S, H, tw, y, 1, st);
     const long long blocks = (rows + C::FPB - 1) / C::FPB;
     if ((rc = allow_lds(pc_fused_kernel<N>, C::LDS_BYTES))) return rc;
     hipLaunchKernelGGL((pc_fused_kernel<N>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
@@ -759,6 +842,27 @@ using namespace zkx;
 extern "C" {
 
 int zkx_max_lds_fft_length(void) { return kMaxLdsFft; }
+
+int zkx_set_kernel_variant(const char* op, int variant)
+{
+    clear_last_error();
+    if (!op) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "kernel variant: null op");
+    const std::string name(op);
+    if (name == "pulse_compression") {
+        if (variant < 0 || variant > 3) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_compression variant out of range");
+        g_pc_variant = variant;
+        return ZKX_SUCCESS;
+    }
+    ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unknown kernel variant op");
+}
+
+int zkx_get_kernel_variant(const char* op)
+{
+    if (!op) return -1;
+    const std::string name(op);
+    if (name == "pulse_compression") return g_pc_variant;
+    return -1;
+}
 
 int zkx_pc_template_prepare(const zkx_complex64* tmpl, int template_len, int nfft, int normalize, int window,
                             const float* explicit_window, zkx_complex64* spectrum, void* stream)

@@ -1,0 +1,201 @@
+"""Host-side mirror of the reference's polyphase filtering API on torch device tensors.
+
+Follows ZKX/cusignal_cpp/include/cusignal/operators/filtering/filtering_typed.h:
+upfirdn (UpfirdnOptions{shape, axis=-1}), resample_poly (ResamplePolyOptions{shape,
+axis=0, window_mode, kaiser_beta, design_window}, optional custom coefficients)
+and firfilter (FirfilterOptions{shape, axis=-1}, optional zi).  float32 lines
+are the reference's FP32 path; complex64 lines are this library's extension
+(real taps applied per component).  All compute runs in libzkx_radar.so.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .radar import _ptr, _stream
+
+
+def _lines(shape, axis):
+    rank = len(shape)
+    ax = axis + rank if axis < 0 else axis
+    if ax < 0 or ax >= rank:
+        raise ValueError("axis is outside rank")
+    outer = int(np.prod(shape[:ax])) if ax > 0 else 1
+    inner = int(np.prod(shape[ax + 1:])) if ax + 1 < rank else 1
+    return outer, int(shape[ax]), inner, ax
+
+
+def _signal(x, name="x"):
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if x.dtype not in (torch.float32, torch.complex64):
+        raise ValueError(f"{name} must be float32 or complex64")
+    return x.contiguous()
+
+
+def _taps(h):
+    if not isinstance(h, torch.Tensor):
+        h = torch.as_tensor(np.asarray(h, np.float32), device="cuda")
+    return h.to(torch.float32).contiguous()
+
+
+def upfirdn(h, x, up=1, down=1, axis=-1, *, stream=None):
+    """upfirdn_device (filtering_typed.h:744-748): upsample by `up`, filter
+    with h, downsample by `down`, along `axis` of a rank-1 or rank-2 signal."""
+    x = _signal(x)
+    h = _taps(h)
+    if h.numel() == 0:
+        raise ValueError("upfirdn coefficients")
+    if up < 1 or down < 1:
+        raise ValueError("upfirdn factors")
+    if x.dim() < 1 or x.dim() > 2:
+        raise ValueError("upfirdn rank")
+    outer, n, inner, ax = _lines(x.shape, axis)
+    out_n = lib().zkx_upfirdn_out_len(n, h.numel(), up, down)
+    shape = list(x.shape)
+    shape[ax] = out_n
+    y = torch.empty(shape, dtype=x.dtype, device=x.device)
+    fn = lib().zkx_upfirdn_c64 if x.is_complex() else lib().zkx_upfirdn_f32
+    check(fn(_ptr(h), h.numel(), _ptr(x), outer, n, inner, up, down, _ptr(y), _stream(stream)))
+    return y
+
+
+def _bessel_i0_as(value):
+    """Abramowitz-Stegun polynomial I0, the approximation the reference's
+    default design uses (filtering_typed.cpp:159-174)."""
+    a = abs(value)
+    if a < 3.75:
+        y = (value / 3.75) ** 2
+        return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 + y *
+                          (0.0360768 + y * 0.0045813)))))
+    y = 3.75 / a
+    return math.exp(a) / math.sqrt(a) * (0.39894228 + y * (0.01328592 + y * (0.00225319 + y * (
+        -0.00157565 + y * (0.00916281 + y * (-0.02057706 + y * (0.02635537 + y * (
+            -0.01647633 + y * 0.00392377))))))))
+
+
+def design_resample_poly_coefficients(up, down, window="kaiser", beta=5.0, design_window=None):
+    """Default resample_poly filter (filtering_typed.cpp:176-218), FP64 on the
+    host: 2*10*max(up,down)+1 taps of a windowed sinc at cutoff 1/max(up,down),
+    normalised to unit sum and scaled by up (up/down gcd-reduced first)."""
+    g = math.gcd(up, down)
+    u, d = up // g, down // g
+    half = 10 * max(u, d)
+    count = 2 * half + 1
+    idx = np.arange(count)
+    if window == "kaiser":
+        den = _bessel_i0_as(beta)
+        w = np.array([_bessel_i0_as(beta * math.sqrt(max(0.0, 1.0 - ((i - half) / half) ** 2))) / den
+                      for i in range(count)])
+    elif window == "hamming":
+        w = 0.54 - 0.46 * np.cos(6.283185307179586476925286766559 * idx / (count - 1))
+    elif window == "explicit":
+        w = np.asarray(design_window, np.float64)
+        if w.size != count:
+            raise ValueError("resample_poly design window shape")
+    else:
+        raise ValueError("resample_poly window mode")
+    cutoff = 1.0 / max(u, d)
+    off = idx - half
+    with np.errstate(invalid="ignore", divide="ignore"):
+        sinc = np.where(off == 0, cutoff, np.sin(np.pi * cutoff * off) / (np.pi * off))
+    c = sinc * w
+    return c / np.sum(c) * u
+
+
+def resample_poly(x, up, down, axis=0, *, coefficients=None, window="kaiser", beta=5.0,
+                  design_window=None, widen=True, stream=None):
+    """resample_poly_device (filtering_typed.h:765-770).
+
+    Custom coefficients are scaled by the gcd-reduced `up` in float32 as the
+    reference does for FP32 input and give a float32 result.  Without them the
+    FP64 default design is rounded to float32 taps and the result is FP64
+    storage (float64 / complex128) when widen=True, as the reference returns.
+    """
+    x = _signal(x)
+    if up < 1 or down < 1:
+        raise ValueError("resample_poly factors")
+    outer, n, inner, ax = _lines(x.shape, axis)
+    g = math.gcd(up, down)
+    u, d = up // g, down // g
+    if u == 1 and d == 1:
+        return x.clone()
+    if x.dim() > 2:
+        raise ValueError("resample_poly nonidentity rank")
+    if coefficients is not None:
+        t = np.asarray(coefficients.cpu() if isinstance(coefficients, torch.Tensor) else coefficients,
+                       np.float32)
+        taps = (t * np.float32(u)).astype(np.float32) if u < 65536 else (t.astype(np.float64) * u)
+        fp64_out = False
+    else:
+        taps = design_resample_poly_coefficients(up, down, window, beta, design_window)
+        fp64_out = True
+    taps_d = torch.as_tensor(np.ascontiguousarray(taps, np.float32), device=x.device)
+    out_n = (n * u + d - 1) // d
+    shape = list(x.shape)
+    shape[ax] = out_n
+    y = torch.empty(shape, dtype=x.dtype, device=x.device)
+    fn = lib().zkx_resample_poly_c64 if x.is_complex() else lib().zkx_resample_poly_f32
+    check(fn(_ptr(taps_d), taps_d.numel(), _ptr(x), outer, n, inner, up, down, _ptr(y),
+             _stream(stream)))
+    if fp64_out and widen:
+        return y.to(torch.complex128 if x.is_complex() else torch.float64)
+    return y
+
+
+def firfilter(b, x, axis=-1, zi=None, *, stream=None):
+    """firfilter_device (filtering_typed.h:552-565): causal FIR y = b * x along
+    `axis`, first N samples; with zi (broadcast to the state shape, axis
+    length K-1) also returns zf.  float32 only, as the reference."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda or x.dtype != torch.float32:
+        raise ValueError("x must be a float32 device tensor")
+    x = x.contiguous()
+    b = _taps(b)
+    if b.numel() == 0:
+        raise ValueError("firfilter coefficients")
+    outer, n, inner, ax = _lines(x.shape, axis)
+    y = torch.empty_like(x)
+    zf = zi_t = None
+    if zi is not None:
+        state_shape = list(x.shape)
+        state_shape[ax] = b.numel() - 1
+        zi_t = torch.as_tensor(zi, dtype=torch.float32, device=x.device)
+        try:
+            zi_t = zi_t.broadcast_to(state_shape).contiguous()
+        except RuntimeError as e:
+            raise ValueError("firfilter zi broadcast shape") from e
+        zf = torch.empty(state_shape, dtype=torch.float32, device=x.device)
+    check(lib().zkx_firfilter_f32(_ptr(b), b.numel(), _ptr(x), outer, n, inner, _ptr(zi_t), _ptr(y),
+                                  _ptr(zf), _stream(stream)))
+    return (y, zf) if zi is not None else y
+
+
+__all__ = ["upfirdn", "resample_poly", "firfilter", "design_resample_poly_coefficients"]
+
+
+def fftconvolve(in1, in2, mode="full", *, stream=None):
+    """fftconvolve of complex64 rows in1 [..., n1] with one 1-D in2 [n2] along
+    the last axis (cuSignal convolve.py:159-320 semantics for modes
+    'full' / 'same' / 'valid').  Real inputs are promoted to complex64."""
+    modes = {"full": 0, "same": 1, "valid": 2}
+    if mode not in modes:
+        raise ValueError("acceptable mode flags are 'valid', 'same', or 'full'")
+    if not isinstance(in1, torch.Tensor) or not in1.is_cuda:
+        raise ValueError("in1 must be a device tensor")
+    a = in1.to(torch.complex64).contiguous()
+    b = torch.as_tensor(in2, device=a.device).to(torch.complex64).contiguous().reshape(-1)
+    if a.numel() == 0 or b.numel() == 0:
+        return torch.empty(0, dtype=torch.complex64, device=a.device)
+    n1 = a.shape[-1]
+    rows = a.numel() // n1
+    out_len = lib().zkx_fftconvolve_out_len(n1, b.numel(), modes[mode])
+    y = torch.empty(tuple(a.shape[:-1]) + (out_len,), dtype=torch.complex64, device=a.device)
+    check(lib().zkx_fftconvolve_c64(_ptr(a), rows, n1, _ptr(b), b.numel(), modes[mode], _ptr(y),
+                                    _stream(stream)))
+    return y
+
+
+__all__.append("fftconvolve")
