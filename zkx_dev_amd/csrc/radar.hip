@@ -193,13 +193,15 @@ fft_rows_kernel(const float2* __restrict__ x, long long rows, const float2* __re
 // ---------------------------------------------------------------------------
 // Doppler: column FFT along pulses, TC range bins per block
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int WIDE = 0>
 struct ColCfg {
     static constexpr int E = fft::elems_for(N);
     static constexpr int T = N / E;
     // range bins per block: enough for coalesced rows, at most 1024 threads
-    static constexpr int TC = T <= 16 ? (256 / T > 64 ? 64 : 256 / T)
-                                      : (T >= 512 ? 1 : (1024 / T < 16 ? 1024 / T : 16));
+    static constexpr int TC0 = T <= 16 ? (256 / T > 64 ? 64 : 256 / T)
+                                       : (T >= 512 ? 1 : (1024 / T < 16 ? 1024 / T : 16));
+    // the wide variant doubles the bins per block where the block stays <= 512 threads
+    static constexpr int TC = (WIDE && TC0 * 2 * T <= 512 && TC0 * 2 <= 128) ? TC0 * 2 : TC0;
     static constexpr int THREADS = T * TC;
     static constexpr size_t LDS_BYTES = sizeof(float) * (2 * N * TC + N);
 };
@@ -209,13 +211,13 @@ struct ColCfg {
 // reference CFAR's integral image (radartools_typed.cpp:501-510), fused here so
 // the power map never makes an extra HBM round trip for it.
 // Window over P pulses (symmetric), computed per block in FP64 into LDS.
-template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+template <int N, int WIDE>
+__global__ void __launch_bounds__((ColCfg<N, WIDE>::THREADS))
 doppler_kernel(const float2* __restrict__ x, int P, int R, int window,
                const float* __restrict__ explicit_window, const float2* __restrict__ tw,
                float2* __restrict__ y, float* __restrict__ power, float* __restrict__ colsum)
 {
-    using C = ColCfg<N>;
+    using C = ColCfg<N, WIDE>;
     extern __shared__ float lds[];
     float* wtab = lds + 2 * N * C::TC;
     const int used = P < N ? P : N;
@@ -524,63 +526,201 @@ __global__ void cfar1d_threshold_kernel(const float* __restrict__ x, const float
 constexpr int kWinTileR = 32;
 constexpr int kWinTileC = 64;
 
+// Separable window sums: a haloed tile of x in LDS; horizontal sums of the
+// outer (2*cm+1) and inner (2*g1+1) column windows per halo row, accumulated
+// in FP64 and kept as float (relative error <= 2^-24); then FP64 vertical
+// sums of those over the outer (2*rm+1) and inner (2*g0+1) row windows.  Each
+// thread slides its window along 8 consecutive outputs.  The FP64 threshold
+// T = (alpha/N)(outer - inner) is compared with x in FP64.
+constexpr int kWinRun = 8;
+
 __global__ void __launch_bounds__(256)
 cfar_window_kernel(const float* __restrict__ x, int rows, int cols, int g0, int g1, int r0, int r1,
                    double alpha_over_n, float* __restrict__ thr, unsigned char* __restrict__ det)
 {
-    extern __shared__ double S[];  // (TR + 2*rm + 1) x (TC + 2*cm + 1) prefix table
+    extern __shared__ float sw[];
     const int rm = g0 + r0, cm = g1 + r1;
     const int HR = kWinTileR + 2 * rm, HC = kWinTileC + 2 * cm;
-    const int W = HC + 1;
+    const int XS = HC | 1;                   // odd row strides keep lanes on distinct banks
+    constexpr int HS = kWinTileC + 1;
+    float* xs = sw;                          // [HR][XS]
+    float* ho = xs + HR * XS;                // [HR][HS] outer horizontal sums
+    float* hi = ho + HR * HS;                // [HR][HS] inner horizontal sums
     const int tr0 = blockIdx.y * kWinTileR, tc0 = blockIdx.x * kWinTileC;
     const long long base = static_cast<long long>(blockIdx.z) * rows * cols;
     const float* X = x + base;
-    // load the haloed tile (zero outside the map) into rows 1.., cols 1..
-    for (int i = threadIdx.x; i < HR * HC; i += blockDim.x) {
-        const int hr = i / HC, hc = i % HC;
-        const int r = tr0 - rm + hr, c = tc0 - cm + hc;
-        S[(hr + 1) * W + hc + 1] = (r >= 0 && r < rows && c >= 0 && c < cols)
-                                       ? static_cast<double>(X[static_cast<long long>(r) * cols + c]) : 0.0;
-    }
-    for (int i = threadIdx.x; i < W; i += blockDim.x) S[i] = 0.0;
-    for (int i = threadIdx.x; i < HR + 1; i += blockDim.x) S[i * W] = 0.0;
-    __syncthreads();
-    // prefix along rows (each thread one table row), then along columns
-    for (int hr = threadIdx.x; hr < HR; hr += blockDim.x) {
-        double run = 0.0;
-        double* rowp = S + (hr + 1) * W;
-        for (int hc = 1; hc <= HC; ++hc) {
-            run += rowp[hc];
-            rowp[hc] = run;
+    {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int hr = wave; hr < HR; hr += nw) {
+            const int r = tr0 - rm + hr;
+            const bool rin = r >= 0 && r < rows;
+            const float* xrow = X + static_cast<long long>(rin ? r : 0) * cols;
+            for (int hc = lane; hc < HC; hc += 64) {
+                const int c = tc0 - cm + hc;
+                xs[hr * XS + hc] = (rin && c >= 0 && c < cols) ? xrow[c] : 0.0f;
+            }
         }
     }
     __syncthreads();
-    for (int hc = threadIdx.x; hc < HC; hc += blockDim.x) {
-        double run = 0.0;
-        for (int hr = 1; hr <= HR; ++hr) {
-            run += S[hr * W + hc + 1];
-            S[hr * W + hc + 1] = run;
+    constexpr int CPR = kWinTileC / kWinRun;  // column runs per row
+    for (int task = threadIdx.x; task < HR * CPR; task += blockDim.x) {
+        const int hr = task % HR, c0 = (task / HR) * kWinRun;
+        const float* xr = xs + hr * XS + c0;
+        double so = 0.0, si = 0.0;
+        for (int d = 0; d <= 2 * cm; ++d) so += xr[d];
+        for (int d = cm - g1; d <= cm + g1; ++d) si += xr[d];
+        ho[hr * HS + c0] = static_cast<float>(so);
+        hi[hr * HS + c0] = static_cast<float>(si);
+#pragma unroll
+        for (int j = 1; j < kWinRun; ++j) {
+            so += static_cast<double>(xr[j + 2 * cm]) - static_cast<double>(xr[j - 1]);
+            si += static_cast<double>(xr[j + cm + g1]) - static_cast<double>(xr[j - 1 + cm - g1]);
+            ho[hr * HS + c0 + j] = static_cast<float>(so);
+            hi[hr * HS + c0 + j] = static_cast<float>(si);
         }
     }
     __syncthreads();
-    auto box = [&](int ra, int rb, int ca, int cb) {  // inclusive table coords (0-based in halo tile)
-        return S[(rb + 1) * W + cb + 1] - S[ra * W + cb + 1] - S[(rb + 1) * W + ca] + S[ra * W + ca];
-    };
-    for (int i = threadIdx.x; i < kWinTileR * kWinTileC; i += blockDim.x) {
-        const int lr = i / kWinTileC, lc = i % kWinTileC;
-        const int r = tr0 + lr, c = tc0 + lc;
-        if (r >= rows || c >= cols) continue;
-        const long long o = base + static_cast<long long>(r) * cols + c;
-        double T = 0.0;
-        if (r >= rm && r < rows - rm && c >= cm && c < cols - cm) {
-            const int hr = lr + rm, hc = lc + cm;  // cell position inside the halo tile
-            const double outer = box(hr - rm, hr + rm, hc - cm, hc + cm);
-            const double inner = box(hr - g0, hr + g0, hc - g1, hc + g1);
-            T = alpha_over_n * (outer - inner);
+    constexpr int RPC = kWinTileR / kWinRun;  // row runs per column
+    for (int task = threadIdx.x; task < kWinTileC * RPC; task += blockDim.x) {
+        const int lc = task % kWinTileC, lr0 = (task / kWinTileC) * kWinRun;
+        const int c = tc0 + lc;
+        double so = 0.0, si = 0.0;
+        for (int d = 0; d <= 2 * rm; ++d) so += ho[(lr0 + d) * HS + lc];
+        for (int d = rm - g0; d <= rm + g0; ++d) si += hi[(lr0 + d) * HS + lc];
+#pragma unroll
+        for (int j = 0; j < kWinRun; ++j) {
+            if (j > 0) {
+                so += static_cast<double>(ho[(lr0 + j + 2 * rm) * HS + lc]) -
+                      static_cast<double>(ho[(lr0 + j - 1) * HS + lc]);
+                si += static_cast<double>(hi[(lr0 + j + rm + g0) * HS + lc]) -
+                      static_cast<double>(hi[(lr0 + j - 1 + rm - g0) * HS + lc]);
+            }
+            const int r = tr0 + lr0 + j;
+            if (r >= rows || c >= cols) continue;
+            double T = 0.0;
+            if (r >= rm && r < rows - rm && c >= cm && c < cols - cm) T = alpha_over_n * (so - si);
+            const long long o = base + static_cast<long long>(r) * cols + c;
+            thr[o] = static_cast<float>(T);
+            det[o] = static_cast<double>(xs[(lr0 + j + rm) * XS + lc + cm]) > T ? 1 : 0;
         }
-        const double xv = static_cast<double>(x[o]);
-        thr[o] = static_cast<float>(T);
-        det[o] = xv > T ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Doppler FFT + |.|^2 + windowed CA-CFAR in one kernel
+// ---------------------------------------------------------------------------
+// A block transforms NCM consecutive range bins: its TCO = NCM - 2*cm output
+// bins plus the cm bins of CFAR halo on each side (their FFTs are recomputed
+// by the neighbouring blocks), so the power map lives only in LDS.  The CFAR
+// covers all N Doppler rows of the tile, so it needs no row halo (rows within
+// g0+r0 of the edges are border cells with T = 0).  Window sums follow
+// cfar_window_kernel: FP64 horizontal sums kept as float, FP64 vertical sums,
+// FP64 threshold and comparison.
+template <int N>
+struct FusedCfg {
+    static constexpr int E = fft::elems_for(N);
+    static constexpr int T = N / E;
+    static constexpr int NCM = 64;  // bins transformed per block
+    static constexpr int THREADS = NCM * T;
+    static constexpr int FFT_FLOATS = 2 * N * NCM + N;
+};
+
+template <int N>
+__global__ void __launch_bounds__((FusedCfg<N>::THREADS))
+doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, const float2* __restrict__ tw, int g0,
+                    int g1, int r0, int r1, double alpha_over_n, float* __restrict__ thr,
+                    unsigned char* __restrict__ det, float2* __restrict__ y, float* __restrict__ power)
+{
+    using C = FusedCfg<N>;
+    extern __shared__ float lds[];
+    const int rm = g0 + r0, cm = g1 + r1;
+    const int TCO = C::NCM - 2 * cm;
+    const int c0 = blockIdx.x * TCO;  // first output bin of the block
+    const int cb = c0 - cm;           // bin of slot 0
+    float* wtab = lds + 2 * N * C::NCM;
+    const int used = P < N ? P : N;
+    for (int p = threadIdx.x; p < used; p += blockDim.x) wtab[p] = static_cast<float>(window_value(window, p, P));
+    __syncthreads();
+    const int s = threadIdx.x % C::NCM, t = threadIdx.x / C::NCM;
+    const int col = cb + s;
+    const bool inb = col >= 0 && col < R;
+    const long long dwell = blockIdx.y;
+    const float2* src = x + dwell * static_cast<long long>(P) * R;
+    fft::ColLds<C::NCM> L{lds, lds + N * C::NCM, s};
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int p = t + k * C::T;
+        v[k] = (inb && p < used) ? cscale(src[static_cast<long long>(p) * R + col], wtab[p]) : make_float2(0.f, 0.f);
+    }
+    fft::transform<N, C::E, false>(v, L, tw, t);
+    // |Y|^2 tile [N][NCM+1] over the FFT buffers, then horizontal sums [N][TCO+1]
+    constexpr int PS = C::NCM + 1;
+    const int HS = TCO + 1;
+    float* pw = lds;
+    float* ho = pw + N * PS;
+    float* hi = ho + N * HS;
+    const long long obase = dwell * static_cast<long long>(N) * R;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int f = t + k * C::T;
+        const float pv = v[k].x * v[k].x + v[k].y * v[k].y;
+        pw[f * PS + s] = pv;
+        if (inb && s >= cm && s < cm + TCO) {
+            const long long o = obase + static_cast<long long>(f) * R + col;
+            if (y) y[o] = v[k];
+            if (power) power[o] = pv;
+        }
+    }
+    __syncthreads();
+    constexpr int RUN = 8;
+    const int runs = (TCO + RUN - 1) / RUN;
+    for (int task = threadIdx.x; task < N * runs; task += blockDim.x) {
+        const int f = task % N, o0 = (task / N) * RUN;
+        const float* pr = pw + f * PS + o0;  // slot o0 .. : window of output o is slots o .. o+2cm
+        double so = 0.0, si = 0.0;
+        for (int d = 0; d <= 2 * cm; ++d) so += pr[d];
+        for (int d = cm - g1; d <= cm + g1; ++d) si += pr[d];
+        for (int j = 0; j < RUN && o0 + j < TCO; ++j) {
+            if (j > 0) {
+                so += static_cast<double>(pr[j + 2 * cm]) - static_cast<double>(pr[j - 1]);
+                si += static_cast<double>(pr[j + cm + g1]) - static_cast<double>(pr[j - 1 + cm - g1]);
+            }
+            ho[f * HS + o0 + j] = static_cast<float>(so);
+            hi[f * HS + o0 + j] = static_cast<float>(si);
+        }
+    }
+    __syncthreads();
+    constexpr int VRUN = 16;
+    constexpr int vruns = N / VRUN > 0 ? N / VRUN : 1;
+    constexpr int VR = N / vruns;
+    for (int task = threadIdx.x; task < TCO * vruns; task += blockDim.x) {
+        const int o = task % TCO, f0 = (task / TCO) * VR;
+        const int c = c0 + o;
+        if (c >= R) continue;
+        double so = 0.0, si = 0.0;
+        bool primed = false;
+        for (int j = 0; j < VR; ++j) {
+            const int f = f0 + j;
+            double T = 0.0;
+            if (f >= rm && f < N - rm && c >= cm && c < R - cm) {
+                if (!primed) {
+                    so = 0.0;
+                    si = 0.0;
+                    for (int d = -rm; d <= rm; ++d) so += ho[(f + d) * HS + o];
+                    for (int d = -g0; d <= g0; ++d) si += hi[(f + d) * HS + o];
+                    primed = true;
+                } else {
+                    so += static_cast<double>(ho[(f + rm) * HS + o]) - static_cast<double>(ho[(f - rm - 1) * HS + o]);
+                    si += static_cast<double>(hi[(f + g0) * HS + o]) - static_cast<double>(hi[(f - g0 - 1) * HS + o]);
+                }
+                T = alpha_over_n * (so - si);
+            }
+            const long long oi = obase + static_cast<long long>(f) * R + c;
+            thr[oi] = static_cast<float>(T);
+            det[oi] = static_cast<double>(pw[f * PS + o + cm]) > T ? 1 : 0;
+        }
     }
 }
 
@@ -673,20 +813,31 @@ int launch_pc_fused(const float2* x, long long rows, int S, const float2* H, flo
     return ZKX_SUCCESS;
 }
 
+int g_doppler_variant = 0;
+
+template <int N, int WIDE>
+int launch_doppler_v(const float2* x, int batch, int P, int R, int window, const float* ew, float2* y,
+                     float* power, float* colsum, const float2* tw, hipStream_t st)
+{
+    using C = ColCfg<N, WIDE>;
+    dim3 grid((R + C::TC - 1) / C::TC, batch);
+    int rc = allow_lds(doppler_kernel<N, WIDE>, C::LDS_BYTES);
+    if (rc) return rc;
+    hipLaunchKernelGGL((doppler_kernel<N, WIDE>), grid, dim3(C::THREADS), C::LDS_BYTES, st, x, P, R, window, ew,
+                       tw, y, power, colsum);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
 template <int N>
 int launch_doppler(const float2* x, int batch, int P, int R, int window, const float* ew, float2* y,
                    float* power, float* colsum, hipStream_t st)
 {
-    using C = ColCfg<N>;
     const float2* tw = nullptr;
     int rc = twiddles(N, &tw);
     if (rc) return rc;
-    dim3 grid((R + C::TC - 1) / C::TC, batch);
-    if ((rc = allow_lds(doppler_kernel<N>, C::LDS_BYTES))) return rc;
-    hipLaunchKernelGGL((doppler_kernel<N>), grid, dim3(C::THREADS), C::LDS_BYTES, st, x, P, R, window, ew,
-                       tw, y, power, colsum);
-    ZKX_LAUNCH_CHECK();
-    return ZKX_SUCCESS;
+    if (g_doppler_variant == 1) return launch_doppler_v<N, 1>(x, batch, P, R, window, ew, y, power, colsum, tw, st);
+    return launch_doppler_v<N, 0>(x, batch, P, R, window, ew, y, power, colsum, tw, st);
 }
 
 #define ZKX_POW2_CASES(F, ...)                     \
@@ -804,7 +955,8 @@ int cfar_window_2d(const float* x, int batch, int rows, int cols, int g0, int g1
     const double n = static_cast<double>(count);
     const double alpha = n * (std::pow(pfa, -1.0 / n) - 1.0);
     const int rm = g0 + r0, cm = g1 + r1;
-    const size_t lds = sizeof(double) * static_cast<size_t>(kWinTileR + 2 * rm + 1) * (kWinTileC + 2 * cm + 1);
+    const size_t lds = sizeof(float) * static_cast<size_t>(kWinTileR + 2 * rm) *
+                       (((kWinTileC + 2 * cm) | 1) + 2 * (kWinTileC + 1));
     if (lds > 160 * 1024) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "ca_cfar window too large for the LDS tile");
     int rc = allow_lds(cfar_window_kernel, lds);
     if (rc) return rc;
@@ -813,6 +965,47 @@ int cfar_window_2d(const float* x, int batch, int rows, int cols, int g0, int g1
                        det);
     ZKX_LAUNCH_CHECK();
     return ZKX_SUCCESS;
+}
+
+template <int N>
+int launch_doppler_cfar(const float2* x, int batch, int P, int R, int window, int g0, int g1, int r0, int r1,
+                        double pfa, float* thr, unsigned char* det, float2* y, float* power, hipStream_t st,
+                        bool* done)
+{
+    using C = FusedCfg<N>;
+    *done = false;
+    const int cm = g1 + r1;
+    if (C::NCM - 2 * cm < 8) return ZKX_SUCCESS;  // halo too wide for the tile: caller falls back
+    const int TCO = C::NCM - 2 * cm;
+    const size_t cfar_floats = static_cast<size_t>(N) * (C::NCM + 1) + 2 * static_cast<size_t>(N) * (TCO + 1);
+    const size_t lds = sizeof(float) * std::max<size_t>(C::FFT_FLOATS, cfar_floats);
+    if (lds > 160 * 1024) return ZKX_SUCCESS;
+    const float2* tw = nullptr;
+    int rc = twiddles(N, &tw);
+    if (rc) return rc;
+    if ((rc = allow_lds(doppler_cfar_kernel<N>, lds))) return rc;
+    const int count = 2 * r0 * (2 * r1 + 2 * g1 + 1) + 2 * (2 * g0 + 1) * r1;
+    const double n = static_cast<double>(count);
+    const double alpha = n * (std::pow(pfa, -1.0 / n) - 1.0);
+    dim3 grid((R + TCO - 1) / TCO, batch);
+    hipLaunchKernelGGL((doppler_cfar_kernel<N>), grid, dim3(C::THREADS), lds, st, x, P, R, window, tw, g0, g1, r0,
+                       r1, alpha / n, thr, det, y, power);
+    ZKX_LAUNCH_CHECK();
+    *done = true;
+    return ZKX_SUCCESS;
+}
+
+int doppler_cfar(int n, const float2* x, int batch, int P, int R, int window, int g0, int g1, int r0, int r1,
+                 double pfa, float* thr, unsigned char* det, float2* y, float* power, hipStream_t st, bool* done)
+{
+    *done = false;
+    switch (n) {
+    case 16: return launch_doppler_cfar<16>(x, batch, P, R, window, g0, g1, r0, r1, pfa, thr, det, y, power, st, done);
+    case 32: return launch_doppler_cfar<32>(x, batch, P, R, window, g0, g1, r0, r1, pfa, thr, det, y, power, st, done);
+    case 64: return launch_doppler_cfar<64>(x, batch, P, R, window, g0, g1, r0, r1, pfa, thr, det, y, power, st, done);
+    case 128: return launch_doppler_cfar<128>(x, batch, P, R, window, g0, g1, r0, r1, pfa, thr, det, y, power, st, done);
+    default: return ZKX_SUCCESS;
+    }
 }
 
 int validate_cfar2d(int rows, int cols, int g0, int g1, int r0, int r1, double pfa)
@@ -853,6 +1046,11 @@ int zkx_set_kernel_variant(const char* op, int variant)
         g_pc_variant = variant;
         return ZKX_SUCCESS;
     }
+    if (name == "pulse_doppler") {
+        if (variant < 0 || variant > 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_doppler variant out of range");
+        g_doppler_variant = variant;
+        return ZKX_SUCCESS;
+    }
     ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unknown kernel variant op");
 }
 
@@ -861,6 +1059,7 @@ int zkx_get_kernel_variant(const char* op)
     if (!op) return -1;
     const std::string name(op);
     if (name == "pulse_compression") return g_pc_variant;
+    if (name == "pulse_doppler") return g_doppler_variant;
     return -1;
 }
 
@@ -1039,6 +1238,12 @@ int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int r
     rc = pc_fused(R, reinterpret_cast<const float2*>(x), static_cast<long long>(batch) * P, R,
                   reinterpret_cast<const float2*>(spectrum), pc, st);
     if (rc) return rc;
+    if (cfar_mode == 1) {
+        bool done = false;
+        rc = doppler_cfar(P, pc, batch, P, R, doppler_window, guard_doppler, guard_range, ref_doppler, ref_range, pfa,
+                          threshold, detections, reinterpret_cast<float2*>(rd_map), nullptr, st, &done);
+        if (rc || done) return rc;
+    }
     rc = doppler(P, pc, batch, P, R, doppler_window, nullptr, reinterpret_cast<float2*>(rd_map), pw,
                  cfar_mode == 0 ? I : nullptr, st);
     if (rc) return rc;
