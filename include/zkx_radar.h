@@ -181,6 +181,19 @@ int zkx_widen_f32_to_f64(const float* in, double* out, size_t count, void* strea
  *   workspace  device buffer of zkx_dwell_workspace_bytes(...) bytes
  */
 size_t zkx_dwell_workspace_bytes(int batch, int num_pulses, int range_bins);
+
+/*
+ * Task1 steps 3-4 fused: Doppler FFT along pulses (nfft = num_pulses, a power
+ * of two <= 128), |.|^2 and the FP64 windowed CA-CFAR, with the power map kept
+ * in LDS.  x [batch, P, R] is the range-compressed cube; rd_map and power
+ * (each [batch, P, R], may be NULL) optionally receive the complex map and
+ * |.|^2.  Threshold / detections as zkx_ca_cfar_window_f32.  Requires
+ * 2*(guard_range + ref_range) <= 56.
+ */
+int zkx_doppler_cfar_c64(const zkx_complex64* x, int batch, int num_pulses, int range_bins,
+                         int doppler_window, int guard_doppler, int guard_range, int ref_doppler,
+                         int ref_range, double pfa, float* threshold, uint8_t* detections,
+                         zkx_complex64* rd_map, float* power, void* stream);
 int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int range_bins,
                         const zkx_complex64* spectrum, int doppler_window,
                         int guard_doppler, int guard_range, int ref_doppler, int ref_range,

@@ -38,6 +38,7 @@ SIGNATURES = {
     "zkx_widen_c64_to_c128": (I, [P, P, SZ, P]),
     "zkx_widen_f32_to_f64": (I, [P, P, SZ, P]),
     "zkx_dwell_workspace_bytes": (SZ, [I, I, I]),
+    "zkx_doppler_cfar_c64": (I, [P, I, I, I, I, I, I, I, I, D, P, P, P, P, P]),
     "zkx_dwell_batch_c64": (I, [P, I, I, I, P, I, I, I, I, I, D, I, P, P, P, P, P]),
     "zkx_upfirdn_out_len": (LL, [LL, LL, I, I]),
     "zkx_upfirdn_f32": (I, [P, I, P, I, I, I, I, I, P, P]),

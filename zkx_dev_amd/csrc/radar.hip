@@ -635,7 +635,16 @@ doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, cons
     extern __shared__ float lds[];
     const int rm = g0 + r0, cm = g1 + r1;
     const int TCO = C::NCM - 2 * cm;
-    const int c0 = blockIdx.x * TCO;  // first output bin of the block
+    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so hand
+    // each XCD a contiguous run of tiles; neighbouring tiles then share their
+    // overlapping halo columns in one L2.
+    const int nx = gridDim.x;
+    const int total = nx * gridDim.y;
+    const int lin = blockIdx.y * nx + blockIdx.x;
+    const int full = (total / 8) * 8;
+    const int logical = lin < full ? (lin % 8) * (full / 8) + lin / 8 : lin;
+    const int tile = logical % nx;
+    const int c0 = tile * TCO;  // first output bin of the block
     const int cb = c0 - cm;           // bin of slot 0
     float* wtab = lds + 2 * N * C::NCM;
     const int used = P < N ? P : N;
@@ -644,7 +653,7 @@ doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, cons
     const int s = threadIdx.x % C::NCM, t = threadIdx.x / C::NCM;
     const int col = cb + s;
     const bool inb = col >= 0 && col < R;
-    const long long dwell = blockIdx.y;
+    const long long dwell = logical / nx;
     const float2* src = x + dwell * static_cast<long long>(P) * R;
     fft::ColLds<C::NCM> L{lds, lds + N * C::NCM, s};
     float2 v[C::E];
@@ -682,44 +691,59 @@ doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, cons
         double so = 0.0, si = 0.0;
         for (int d = 0; d <= 2 * cm; ++d) so += pr[d];
         for (int d = cm - g1; d <= cm + g1; ++d) si += pr[d];
-        for (int j = 0; j < RUN && o0 + j < TCO; ++j) {
+        // all window reads of the run before any store, then the stores
+        float vo[RUN], vi[RUN];
+#pragma unroll
+        for (int j = 0; j < RUN; ++j) {
             if (j > 0) {
                 so += static_cast<double>(pr[j + 2 * cm]) - static_cast<double>(pr[j - 1]);
                 si += static_cast<double>(pr[j + cm + g1]) - static_cast<double>(pr[j - 1 + cm - g1]);
             }
-            ho[f * HS + o0 + j] = static_cast<float>(so);
-            hi[f * HS + o0 + j] = static_cast<float>(si);
+            vo[j] = static_cast<float>(so);
+            vi[j] = static_cast<float>(si);
+        }
+#pragma unroll
+        for (int j = 0; j < RUN; ++j) {
+            if (o0 + j < TCO) {
+                ho[f * HS + o0 + j] = vo[j];
+                hi[f * HS + o0 + j] = vi[j];
+            }
         }
     }
     __syncthreads();
-    constexpr int VRUN = 16;
-    constexpr int vruns = N / VRUN > 0 ? N / VRUN : 1;
-    constexpr int VR = N / vruns;
+    // vertical FP64 sliding sums: a task is one output bin and VR consecutive
+    // Doppler rows; x and the window reads are issued up front and the rows
+    // are unrolled, so no row waits on its own LDS load
+    constexpr int VR = N < 16 ? N : 16;
+    constexpr int vruns = N / VR;
     for (int task = threadIdx.x; task < TCO * vruns; task += blockDim.x) {
         const int o = task % TCO, f0 = (task / TCO) * VR;
         const int c = c0 + o;
         if (c >= R) continue;
+        float xv[VR];
+#pragma unroll
+        for (int j = 0; j < VR; ++j) xv[j] = pw[(f0 + j) * PS + o + cm];
+        const bool colin = c >= cm && c < R - cm;
+        const int fi = f0 > rm ? f0 : rm;
+        const int fe = (f0 + VR) < (N - rm) ? (f0 + VR) : (N - rm);
         double so = 0.0, si = 0.0;
-        bool primed = false;
+        if (colin && fi < fe) {
+            for (int d = -rm; d <= rm; ++d) so += ho[(fi + d) * HS + o];
+            for (int d = -g0; d <= g0; ++d) si += hi[(fi + d) * HS + o];
+        }
+        float* tp = thr + obase + static_cast<long long>(f0) * R + c;
+        unsigned char* dp = det + obase + static_cast<long long>(f0) * R + c;
+#pragma unroll
         for (int j = 0; j < VR; ++j) {
             const int f = f0 + j;
-            double T = 0.0;
-            if (f >= rm && f < N - rm && c >= cm && c < R - cm) {
-                if (!primed) {
-                    so = 0.0;
-                    si = 0.0;
-                    for (int d = -rm; d <= rm; ++d) so += ho[(f + d) * HS + o];
-                    for (int d = -g0; d <= g0; ++d) si += hi[(f + d) * HS + o];
-                    primed = true;
-                } else {
-                    so += static_cast<double>(ho[(f + rm) * HS + o]) - static_cast<double>(ho[(f - rm - 1) * HS + o]);
-                    si += static_cast<double>(hi[(f + g0) * HS + o]) - static_cast<double>(hi[(f - g0 - 1) * HS + o]);
-                }
-                T = alpha_over_n * (so - si);
+            const bool live = colin && f >= fi && f < fe;
+            if (live && f > fi) {
+                so += static_cast<double>(ho[(f + rm) * HS + o]) - static_cast<double>(ho[(f - rm - 1) * HS + o]);
+                si += static_cast<double>(hi[(f + g0) * HS + o]) - static_cast<double>(hi[(f - g0 - 1) * HS + o]);
             }
-            const long long oi = obase + static_cast<long long>(f) * R + c;
-            thr[oi] = static_cast<float>(T);
-            det[oi] = static_cast<double>(pw[f * PS + o + cm]) > T ? 1 : 0;
+            const double T = live ? alpha_over_n * (so - si) : 0.0;
+            tp[static_cast<long long>(j) * R] = static_cast<float>(T);
+            dp[static_cast<long long>(j) * R] = static_cast<double>(xv[j]) > T ? 1 : 0;
         }
     }
 }
@@ -1202,6 +1226,28 @@ int zkx_ca_cfar_window_f32(const float* x, int rows, int cols, int guard_rows, i
     if (rc) return rc;
     return cfar_window_2d(x, 1, rows, cols, guard_rows, guard_cols, ref_rows, ref_cols, pfa, threshold, detections,
                           (hipStream_t)stream);
+}
+
+int zkx_doppler_cfar_c64(const zkx_complex64* x, int batch, int num_pulses, int range_bins, int doppler_window,
+                         int guard_doppler, int guard_range, int ref_doppler, int ref_range, double pfa,
+                         float* threshold, uint8_t* detections, zkx_complex64* rd_map, float* power, void* stream)
+{
+    clear_last_error();
+    const int P = num_pulses, R = range_bins;
+    if (batch < 1 || P < 2 || R < 1 || !x || !threshold || !detections)
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "doppler_cfar shape is invalid");
+    if (!is_pow2(P) || P > 128) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "doppler_cfar needs power-of-two pulses <= 128");
+    if (!valid_window(doppler_window) || explicit_kind(doppler_window))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "doppler_cfar window must be none, hann or hamming");
+    int rc = validate_cfar2d(P, R, guard_doppler, guard_range, ref_doppler, ref_range, pfa);
+    if (rc) return rc;
+    bool done = false;
+    rc = doppler_cfar(P, reinterpret_cast<const float2*>(x), batch, P, R, doppler_window, guard_doppler, guard_range,
+                      ref_doppler, ref_range, pfa, threshold, detections, reinterpret_cast<float2*>(rd_map), power,
+                      (hipStream_t)stream, &done);
+    if (rc) return rc;
+    if (!done) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "doppler_cfar range window too wide for the fused tile");
+    return ZKX_SUCCESS;
 }
 
 size_t zkx_dwell_workspace_bytes(int batch, int num_pulses, int range_bins)

@@ -215,6 +215,27 @@ def ca_cfar_windowed(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=Non
     return threshold, detections
 
 
+def doppler_cfar(x, guard_cells=(2, 2), reference_cells=(8, 8), pfa=1e-6, *, window="hann",
+                 threshold=None, detections=None, rd_map=None, power=None, stream=None):
+    """Task1 steps 3-4 fused: Doppler FFT (nfft = num_pulses), |.|^2 and the
+    FP64 windowed CA-CFAR for x [..., P, R] complex64.  Returns (threshold,
+    detections) shaped like x."""
+    x = _device_tensor(x, torch.complex64, "x")
+    P, R = x.shape[-2], x.shape[-1]
+    B = x.numel() // (P * R)
+    if threshold is None:
+        threshold = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+    if detections is None:
+        detections = torch.empty(x.shape, dtype=torch.uint8, device=x.device)
+    code = lib().zkx_doppler_cfar_c64(_ptr(x), B, P, R, WINDOWS[window], guard_cells[0], guard_cells[1],
+                                      reference_cells[0], reference_cells[1], float(pfa), _ptr(threshold),
+                                      _ptr(detections), _ptr(rd_map), _ptr(power), _stream(stream))
+    if code in (1001, 1002):
+        raise ValueError(lib().zkx_last_error_message().decode())
+    check(code)
+    return threshold, detections
+
+
 def widen_complex(t, stream=None):
     out = torch.empty(t.shape, dtype=torch.complex128, device=t.device)
     check(lib().zkx_widen_c64_to_c128(_ptr(t), _ptr(out), t.numel(), _stream(stream)))
@@ -225,12 +246,13 @@ class DwellPipeline:
     """Task1 steps 2-4 for batches of same-shape dwells, template resident.
 
     run(x[B, P, R]) -> (threshold[B, P, R] float32, detections[B, P, R] uint8)
-    cfar_mode 0 reproduces the reference's integral-image thresholds bit for
-    bit; 1 uses FP64 windowed sums.
+    cfar_mode 1 (default) fuses Doppler, |.|^2 and an FP64 windowed CFAR in
+    one kernel; 0 reproduces the reference's FP32 integral-image thresholds
+    bit for bit (separate Doppler and CFAR kernels).
     """
 
     def __init__(self, template, num_pulses, range_bins, *, normalize=True, pc_window="hamming",
-                 doppler_window="hann", guard=(2, 2), reference=(8, 8), pfa=1e-6, cfar_mode=0,
+                 doppler_window="hann", guard=(2, 2), reference=(8, 8), pfa=1e-6, cfar_mode=1,
                  max_batch=1, device="cuda"):
         self.P, self.R = int(num_pulses), int(range_bins)
         self.guard, self.reference, self.pfa = tuple(guard), tuple(reference), float(pfa)
@@ -271,4 +293,4 @@ class DwellPipeline:
 
 __all__ = ["pulse_compression", "pc_template_spectrum", "pulse_compression_prepared",
            "pulse_doppler", "doppler_power", "cfar_alpha", "ca_cfar", "ca_cfar_windowed",
-           "widen_complex", "DwellPipeline", "ZkxError"]
+           "doppler_cfar", "widen_complex", "DwellPipeline", "ZkxError"]
