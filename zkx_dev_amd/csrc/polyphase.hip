@@ -56,16 +56,38 @@ struct PolyPlan {
 
 constexpr int kWaves = 4;
 
-// Shared layout: taps table [Pn][SL][M] floats, x tile [XL], out tile [TT*Pn].
+// Taps table tab[r][s][i] = h[phi_r + j*up], j = i*D - s + Jmax - 1, zero
+// outside the phase, built once per call in global memory.
+__global__ void taps_table_kernel(const float* __restrict__ h, PolyPlan p, int M, float* __restrict__ tab)
+{
+    const int SL = p.Jmax + (M - 1) * p.D;
+    const int count = p.Pn * SL * M;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
+        const int r = e / (SL * M), rem = e % (SL * M), s = rem / M, i = rem % M;
+        const long long mr = static_cast<long long>(r) * p.down + p.m_off;
+        const int phi = static_cast<int>(mr % p.up);
+        const int j = i * p.D - s + p.Jmax - 1;
+        const long long k = static_cast<long long>(phi) + static_cast<long long>(j) * p.up;
+        tab[e] = (j >= 0 && k < p.K) ? h[k] : 0.0f;
+    }
+}
+
+// Shared layout: x tile [XL], out tile [TT*Pn].  The taps of a wavefront's
+// residue are uniform across its lanes and come through scalar loads, so the
+// only LDS read per shift is the one x sample that feeds M FMAs.
 template <class V, int M>
 __global__ void __launch_bounds__(64 * kWaves)
-polyphase_kernel(const float* __restrict__ h, const V* __restrict__ x, V* __restrict__ y, PolyPlan p, int TT, int XL)
+polyphase_kernel(const float* __restrict__ tab, const V* __restrict__ x, V* __restrict__ y, PolyPlan p, int TT, int XL)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
     const int SL = p.Jmax + (M - 1) * p.D;
-    float* tab = reinterpret_cast<float*>(smraw);
-    V* xs = reinterpret_cast<V*>(smraw + ((sizeof(float) * p.Pn * SL * M + 15) / 16) * 16);
-    V* ys = xs + XL;
+    // Lane l reads tile element l*S + k for a wave-uniform k, S = M*D.  The
+    // tile is stored transposed, element e at (e mod S)*Lc + e/S, so those 64
+    // reads hit 64 consecutive words instead of one bank.
+    const int S = M * p.D;
+    const int Lc = (XL + S - 1) / S;
+    V* xs = reinterpret_cast<V*>(smraw);
+    V* ys = xs + static_cast<long long>(S) * Lc;
     const int tid = threadIdx.x;
     const long long line = blockIdx.y;
     const V* xl = x + line * p.n_in;
@@ -74,38 +96,47 @@ polyphase_kernel(const float* __restrict__ h, const V* __restrict__ x, V* __rest
     // m of residue r at T: T*up*D + m_r, m_r = r*down + m_off
     const long long q0 = p.m_off / p.up;  // base of residue 0 at T = 0
     const long long xstart = T0 * p.D + q0 - (p.Jmax - 1);
-    // taps table: tab[r][s][i] = h[phi_r + j*up], j = i*D - s + Jmax - 1, zero outside the phase
-    for (int e = tid; e < p.Pn * SL * M; e += blockDim.x) {
-        const int r = e / (SL * M), rem = e % (SL * M), s = rem / M, i = rem % M;
-        const long long mr = static_cast<long long>(r) * p.down + p.m_off;
-        const int phi = static_cast<int>(mr % p.up);
-        const int j = i * p.D - s + p.Jmax - 1;
-        const long long k = static_cast<long long>(phi) + static_cast<long long>(j) * p.up;
-        tab[e] = (j >= 0 && k < p.K) ? h[k] : 0.0f;
-    }
-    for (int e = tid; e < XL; e += blockDim.x) {
-        const long long u = xstart + e;
-        xs[e] = (u >= 0 && u < p.n_in) ? xl[u] : VOps<V>::zero();
+    for (int base = tid; base < S * Lc; base += 16 * static_cast<int>(blockDim.x)) {
+        V tmp[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = base + u * static_cast<int>(blockDim.x);
+            const long long g = xstart + e;
+            tmp[u] = (e < XL && g >= 0 && g < p.n_in) ? xl[g] : VOps<V>::zero();
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = base + u * static_cast<int>(blockDim.x);
+            if (e < S * Lc) xs[(e % S) * Lc + e / S] = tmp[u];
+        }
     }
     __syncthreads();
-    const int wave = tid / 64, lane = tid % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;
     const int NC = TT / (64 * M);
     for (int task = wave; task < p.Pn * NC; task += kWaves) {
         const int r = task / NC, tc = task % NC;
         const long long mr = static_cast<long long>(r) * p.down + p.m_off;
         const int dq = static_cast<int>(mr / p.up - q0);
-        const int t0 = tc * 64 * M + lane * M;
-        const float* tr = tab + r * SL * M;
-        const V* xb = xs + t0 * p.D + dq;
+        const float* tr = tab + static_cast<long long>(r) * SL * M;
+        // lane's first output t0 = tc*64*M + lane*M, tile element t0*D + dq + s
+        //   = (tc*64 + lane)*S + (dq + s)
+        const V* xb = xs + tc * 64 + lane;
         V acc[M];
 #pragma unroll
         for (int i = 0; i < M; ++i) acc[i] = VOps<V>::zero();
+        int k = dq + SL - 1;  // dq + s, walked down with its quotient / remainder by S
+        int kq = k / S, km = k % S;
         for (int s = SL - 1; s >= 0; --s) {
-            const V xv = xb[s];
+            const V xv = xb[km * Lc + kq];
             const float* trs = tr + s * M;
 #pragma unroll
             for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[i], xv, acc[i]);
+            if (--km < 0) {
+                km = S - 1;
+                --kq;
+            }
         }
+        const int t0 = tc * 64 * M + lane * M;
 #pragma unroll
         for (int i = 0; i < M; ++i) ys[(t0 + i) * p.Pn + r] = acc[i];
     }
@@ -172,36 +203,65 @@ unsigned grid_for(long long n)
     return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(b, 65535)));
 }
 
+thread_local ThreadWorkspace t_taps_ws;
+
 template <class V, int M>
 int launch_tiled(const float* h, const V* x, V* y, const PolyPlan& p, long long lines, hipStream_t st, bool* done)
 {
     *done = false;
     const int NC = kWaves / std::gcd(p.Pn, kWaves);
     const int TT = NC * 64 * M;
-    const int SL = p.Jmax + (M - 1) * p.D;
+    const long long SL = p.Jmax + static_cast<long long>(M - 1) * p.D;
     const long long q_last = ((static_cast<long long>(p.Pn) - 1) * p.down + p.m_off) / p.up - p.m_off / p.up;
     const long long XL = static_cast<long long>(TT - 1) * p.D + q_last + p.Jmax + (M - 1) * p.D + 1;
-    const size_t tab_bytes = ((sizeof(float) * static_cast<size_t>(p.Pn) * SL * M + 15) / 16) * 16;
-    const size_t lds = tab_bytes + sizeof(V) * (static_cast<size_t>(XL) + static_cast<size_t>(TT) * p.Pn);
-    if (lds > 150 * 1024 || XL > std::numeric_limits<int>::max()) return ZKX_SUCCESS;  // caller falls back
-    if (lds > 64 * 1024)
-        ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(polyphase_kernel<V, M>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const long long S = static_cast<long long>(M) * p.D;
+    const long long XLp = (XL + S - 1) / S * S;  // transposed tile rounds up to whole columns
+    const size_t lds = sizeof(V) * (static_cast<size_t>(XLp) + static_cast<size_t>(TT) * p.Pn);
+    const long long tab_count = static_cast<long long>(p.Pn) * SL * M;
+    // very long phase tables or tiles go to the generic kernel
+    if (lds > 150 * 1024 || XL > std::numeric_limits<int>::max() || tab_count > (1 << 20)) return ZKX_SUCCESS;
     const long long Tcount = (p.n_out + p.Pn - 1) / p.Pn;
     const long long blocks = (Tcount + TT - 1) / TT;
     if (blocks > std::numeric_limits<int>::max() || lines > 65535) return ZKX_SUCCESS;
+    void* wsp = nullptr;
+    int rc = t_taps_ws.reserve(sizeof(float) * static_cast<size_t>(tab_count), &wsp);
+    if (rc) return rc;
+    float* tab = static_cast<float*>(wsp);
+    hipLaunchKernelGGL(taps_table_kernel, dim3(static_cast<unsigned>(std::min<long long>((tab_count + 255) / 256, 1024))),
+                       dim3(256), 0, st, h, p, M, tab);
+    ZKX_LAUNCH_CHECK();
+    if (lds > 64 * 1024)
+        ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(polyphase_kernel<V, M>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     hipLaunchKernelGGL((polyphase_kernel<V, M>), dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(lines)),
-                       dim3(64 * kWaves), lds, st, h, x, y, p, TT, static_cast<int>(XL));
+                       dim3(64 * kWaves), lds, st, tab, x, y, p, TT, static_cast<int>(XL));
     ZKX_LAUNCH_CHECK();
     *done = true;
     return ZKX_SUCCESS;
 }
 
-// Register blocking depth: enough reuse that the FMAs, not LDS, set the pace,
-// without padding waste ((M-1)*D zero taps per output group) outgrowing it.
+// LDS bytes of one tiled block for a register-blocking depth M.
+template <class V>
+size_t tiled_lds_bytes(const PolyPlan& p, int M)
+{
+    const int NC = kWaves / std::gcd(p.Pn, kWaves);
+    const long long TT = static_cast<long long>(NC) * 64 * M;
+    const long long SL = p.Jmax + static_cast<long long>(M - 1) * p.D;
+    const long long q_last = ((static_cast<long long>(p.Pn) - 1) * p.down + p.m_off) / p.up - p.m_off / p.up;
+    const long long XL = (TT - 1) * p.D + q_last + p.Jmax + (M - 1) * p.D + 1;
+    (void)SL;
+    const long long S = static_cast<long long>(M) * p.D;
+    return sizeof(V) * static_cast<size_t>((XL + S - 1) / S * S + TT * p.Pn);
+}
+
+// Register blocking depth: M = 8 gives more FMAs per LDS read of x, M = 4
+// less zero-tap padding ((M-1)*D per output group) and a smaller tile; take
+// 8 only when the taps per phase dwarf the padding and the tile still leaves
+// room for three blocks per CU.
+template <class V>
 int pick_m(const PolyPlan& p)
 {
-    return (p.Jmax >= 16 * p.D) ? 8 : 4;
+    return (p.Jmax >= 16 * p.D && tiled_lds_bytes<V>(p, 8) <= 52 * 1024) ? 8 : 4;
 }
 
 template <class V>
@@ -210,7 +270,7 @@ int run_polyphase(const float* h, const V* x, V* y, const PolyPlan& p, int outer
     if (p.n_out <= 0 || outer <= 0 || inner <= 0) return ZKX_SUCCESS;
     if (inner == 1) {
         bool done = false;
-        int rc = pick_m(p) == 8 ? launch_tiled<V, 8>(h, x, y, p, outer, st, &done)
+        int rc = pick_m<V>(p) == 8 ? launch_tiled<V, 8>(h, x, y, p, outer, st, &done)
                                 : launch_tiled<V, 4>(h, x, y, p, outer, st, &done);
         if (rc) return rc;
         if (done) return ZKX_SUCCESS;

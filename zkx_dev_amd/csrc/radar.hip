@@ -550,14 +550,25 @@ cfar_window_kernel(const float* __restrict__ x, int rows, int cols, int g0, int 
     const long long base = static_cast<long long>(blockIdx.z) * rows * cols;
     const float* X = x + base;
     {
+        // rows of the haloed tile, 64 columns per wave pass, 8 loads in flight
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        for (int hr = wave; hr < HR; hr += nw) {
-            const int r = tr0 - rm + hr;
-            const bool rin = r >= 0 && r < rows;
-            const float* xrow = X + static_cast<long long>(rin ? r : 0) * cols;
-            for (int hc = lane; hc < HC; hc += 64) {
-                const int c = tc0 - cm + hc;
-                xs[hr * XS + hc] = (rin && c >= 0 && c < cols) ? xrow[c] : 0.0f;
+        const int segs = (HC + 63) / 64;
+        const int items = HR * segs;  // (row, 64-column segment) pairs
+        for (int b0 = wave; b0 < items; b0 += 8 * nw) {
+            float tmp[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int it = b0 + u * nw;
+                const int hr = it / segs, hc = (it % segs) * 64 + lane;
+                const int r = tr0 - rm + hr, c = tc0 - cm + hc;
+                tmp[u] = (it < items && hc < HC && r >= 0 && r < rows && c >= 0 && c < cols)
+                             ? X[static_cast<long long>(r) * cols + c] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int it = b0 + u * nw;
+                const int hr = it / segs, hc = (it % segs) * 64 + lane;
+                if (it < items && hc < HC) xs[hr * XS + hc] = tmp[u];
             }
         }
     }

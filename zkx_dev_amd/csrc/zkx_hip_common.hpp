@@ -60,6 +60,27 @@ void set_last_error(int code, const char* msg);
         return (code);                                                            \
     } while (0)
 
+// Copy `count` elements produced by load(e) into LDS dst[e] with U loads in
+// flight per thread: all U global loads of a round are issued before any of
+// their LDS stores, so the round trip is paid once per U elements.
+template <int U, class V, class F>
+__device__ __forceinline__ void stage_to_lds(V* dst, int count, F load)
+{
+    for (int base = threadIdx.x; base < count; base += U * static_cast<int>(blockDim.x)) {
+        V tmp[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * static_cast<int>(blockDim.x);
+            if (e < count) tmp[u] = load(e);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * static_cast<int>(blockDim.x);
+            if (e < count) dst[e] = tmp[u];
+        }
+    }
+}
+
 inline bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
 inline int ilog2(long long n)
 {
