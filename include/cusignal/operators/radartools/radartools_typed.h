@@ -1,0 +1,233 @@
+// The reference's radar operator API (ZKX/cusignal_cpp/include/cusignal/
+// operators/radartools/radartools_typed.h) on libzkx_radar.so: same option and
+// result structs, same function names and overloads, same dtype rules and
+// exceptions, for float element types (TypedComplex<float> / float), which is
+// the radar hot path.  The computation is the gfx950 HIP library behind the C
+// ABI; nothing here computes on the host except cfar_alpha's scalar.
+// The reference's *_typed_cpu functions are its CPU reference path and are not
+// part of this library.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <type_traits>
+#include <vector>
+
+#include "../filtering/filtering_typed.h"
+
+namespace cusignal {
+
+enum class CfarDetection : std::uint8_t { no = 0, yes = 1 };
+
+struct CaCfarOptions {
+    std::vector<int> guard_cells;
+    std::vector<int> reference_cells;
+    double pfa = 1.0e-3;
+};
+
+struct CaCfarDeviceResult {
+    DeviceArray<float> threshold;
+    DeviceArray<CfarDetection> detections;
+    std::vector<int> shape;
+};
+
+enum class RadarWindowKind { none, hann, hamming, explicit_fp32, explicit_fp64 };
+
+struct PulseCompressionOptions {
+    int num_pulses{0};
+    int samples_per_pulse{0};
+    bool normalize{false};
+    int nfft{-1};
+    RadarWindowKind window{RadarWindowKind::none};
+};
+
+struct PulseDopplerOptions {
+    int num_pulses{0};
+    int samples_per_pulse{0};
+    int nfft{-1};
+    RadarWindowKind window{RadarWindowKind::none};
+};
+
+struct RadarComplexDeviceResult {
+    DeviceArray<ComplexFloat> fp32;
+    DeviceArray<ComplexDouble> fp64;
+    std::vector<int> shape;
+    bool is_fp64{false};
+};
+
+namespace detail {
+template <class T>
+constexpr void require_float()
+{
+    static_assert(std::is_same_v<T, float>, "libzkx_radar implements the float (complex64) operator path");
+}
+inline bool radar_result_is_fp64(RadarWindowKind w)
+{
+    // radartools_typed.cpp:271-278 for float inputs
+    return w != RadarWindowKind::none && w != RadarWindowKind::explicit_fp32;
+}
+inline DeviceArray<ComplexDouble> widen(const DeviceArray<ComplexFloat>& v)
+{
+    DeviceArray<ComplexDouble> out(v.size());
+    check_abi(zkx_widen_c64_to_c128(reinterpret_cast<const zkx_complex64*>(v.data()),
+                                    reinterpret_cast<zkx_complex128*>(out.data()), v.size(), nullptr));
+    return out;
+}
+}  // namespace detail
+
+// radartools_typed.h:237-262
+template <class T>
+RadarComplexDeviceResult pulse_compression_device(const DeviceArray<TypedComplex<T>>& x,
+                                                  const DeviceArray<TypedComplex<T>>& pulse_template,
+                                                  const PulseCompressionOptions& options,
+                                                  const DeviceArray<float>* explicit_window = nullptr)
+{
+    detail::require_float<T>();
+    const int nfft = options.nfft == -1 ? options.samples_per_pulse : options.nfft;
+    if (options.num_pulses < 1 || options.samples_per_pulse < 1 || nfft < 1 ||
+        x.size() != static_cast<std::size_t>(options.num_pulses) * options.samples_per_pulse || pulse_template.empty())
+        throw std::invalid_argument("pulse_compression shape is invalid");
+    const bool needs_window = options.window == RadarWindowKind::explicit_fp32 ||
+                              options.window == RadarWindowKind::explicit_fp64;
+    if (needs_window != (explicit_window != nullptr) ||
+        (explicit_window && explicit_window->size() != pulse_template.size()))
+        throw std::invalid_argument("pulse_compression window is invalid");
+    DeviceArray<ComplexFloat> computed(static_cast<std::size_t>(options.num_pulses) * nfft);
+    detail::check_abi(zkx_pulse_compression_c64(
+        reinterpret_cast<const zkx_complex64*>(x.data()), options.num_pulses, options.samples_per_pulse,
+        reinterpret_cast<const zkx_complex64*>(pulse_template.data()), static_cast<int>(pulse_template.size()), nfft,
+        options.normalize ? 1 : 0, static_cast<int>(options.window), explicit_window ? explicit_window->data() : nullptr,
+        reinterpret_cast<zkx_complex64*>(computed.data()), nullptr));
+    RadarComplexDeviceResult result;
+    result.shape = {options.num_pulses, nfft};
+    result.is_fp64 = detail::radar_result_is_fp64(options.window);
+    if (result.is_fp64) result.fp64 = detail::widen(computed);
+    else result.fp32 = std::move(computed);
+    return result;
+}
+
+// radartools_typed.h:304-311 (preallocated FP32 output, no widening)
+template <class T>
+void pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x, const PulseDopplerOptions& options,
+                          DeviceArray<ComplexFloat>& output, const DeviceArray<float>* explicit_window = nullptr);
+
+// Extension of the preallocated overload: also writes |output|^2 into
+// `power` from the same kernel, replacing the separate power_kernel the
+// reference's step 4 launches (Task1/pipeline/src/steps/step4/step4.cu:11-19,58-61).
+template <class T>
+void pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x, const PulseDopplerOptions& options,
+                          DeviceArray<ComplexFloat>& output, DeviceArray<float>& power,
+                          const DeviceArray<float>* explicit_window = nullptr)
+{
+    detail::require_float<T>();
+    const int nfft = options.nfft == -1 ? options.num_pulses : options.nfft;
+    if (options.num_pulses < 1 || options.samples_per_pulse < 1 || nfft < 1 ||
+        x.size() != static_cast<std::size_t>(options.num_pulses) * options.samples_per_pulse)
+        throw std::invalid_argument("pulse_doppler shape is invalid");
+    const bool needs_window = options.window == RadarWindowKind::explicit_fp32 ||
+                              options.window == RadarWindowKind::explicit_fp64;
+    if (needs_window != (explicit_window != nullptr) ||
+        (explicit_window && explicit_window->size() != static_cast<std::size_t>(options.num_pulses)))
+        throw std::invalid_argument("pulse_doppler window is invalid");
+    const std::size_t count = static_cast<std::size_t>(nfft) * options.samples_per_pulse;
+    if (output.size() != count || power.size() != count)
+        throw std::invalid_argument("pulse_doppler preallocated output shape mismatch");
+    detail::check_abi(zkx_pulse_doppler_c64(reinterpret_cast<const zkx_complex64*>(x.data()), options.num_pulses,
+                                            options.samples_per_pulse, nfft, static_cast<int>(options.window),
+                                            explicit_window ? explicit_window->data() : nullptr,
+                                            reinterpret_cast<zkx_complex64*>(output.data()), power.data(), nullptr));
+}
+
+template <class T>
+void pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x, const PulseDopplerOptions& options,
+                          DeviceArray<ComplexFloat>& output, const DeviceArray<float>* explicit_window)
+{
+    detail::require_float<T>();
+    const int nfft = options.nfft == -1 ? options.num_pulses : options.nfft;
+    if (options.num_pulses < 1 || options.samples_per_pulse < 1 || nfft < 1 ||
+        x.size() != static_cast<std::size_t>(options.num_pulses) * options.samples_per_pulse)
+        throw std::invalid_argument("pulse_doppler shape is invalid");
+    const bool needs_window = options.window == RadarWindowKind::explicit_fp32 ||
+                              options.window == RadarWindowKind::explicit_fp64;
+    if (needs_window != (explicit_window != nullptr) ||
+        (explicit_window && explicit_window->size() != static_cast<std::size_t>(options.num_pulses)))
+        throw std::invalid_argument("pulse_doppler window is invalid");
+    if (output.size() != static_cast<std::size_t>(nfft) * options.samples_per_pulse)
+        throw std::invalid_argument("pulse_doppler preallocated output shape mismatch");
+    detail::check_abi(zkx_pulse_doppler_c64(reinterpret_cast<const zkx_complex64*>(x.data()), options.num_pulses,
+                                            options.samples_per_pulse, nfft, static_cast<int>(options.window),
+                                            explicit_window ? explicit_window->data() : nullptr,
+                                            reinterpret_cast<zkx_complex64*>(output.data()), nullptr, nullptr));
+}
+
+// radartools_typed.h:276-289
+template <class T>
+RadarComplexDeviceResult pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x,
+                                              const PulseDopplerOptions& options,
+                                              const DeviceArray<float>* explicit_window = nullptr)
+{
+    const int nfft = options.nfft == -1 ? options.num_pulses : options.nfft;
+    if (nfft < 1 || options.samples_per_pulse < 1) throw std::invalid_argument("pulse_doppler shape is invalid");
+    DeviceArray<ComplexFloat> computed(static_cast<std::size_t>(nfft) * options.samples_per_pulse);
+    pulse_doppler_device(x, options, computed, explicit_window);
+    RadarComplexDeviceResult result;
+    result.shape = {nfft, options.samples_per_pulse};
+    result.is_fp64 = detail::radar_result_is_fp64(options.window);
+    if (result.is_fp64) result.fp64 = detail::widen(computed);
+    else result.fp32 = std::move(computed);
+    return result;
+}
+
+// radartools_typed.h:207-222: alpha = N (pfa^(-1/N) - 1) in FP64
+template <class T>
+double cfar_alpha_typed_cpu(T pfa, int reference)
+{
+    detail::require_float<T>();
+    double alpha = 0.0;
+    detail::check_abi(zkx_cfar_alpha(static_cast<double>(pfa), reference, &alpha));
+    return alpha;
+}
+
+template <class T>
+DeviceArray<double> cfar_alpha_device(T pfa, int reference)
+{
+    return DeviceArray<double>::from_host({cfar_alpha_typed_cpu(pfa, reference)});
+}
+
+// radartools_typed.h:182-195 (preallocated outputs)
+template <class T>
+void ca_cfar_device(const DeviceArray<T>& x, const std::vector<int>& shape, const CaCfarOptions& options,
+                    DeviceArray<float>& threshold, DeviceArray<CfarDetection>& detections)
+{
+    detail::require_float<T>();
+    if (shape.size() > 2) throw std::invalid_argument("ca_cfar supports only scalar, 1D, or 2D input");
+    if (threshold.size() != x.size() || detections.size() != x.size())
+        throw std::invalid_argument("ca_cfar preallocated output shape must match input");
+    if (shape.empty()) {
+        if (x.size() != 1) throw std::invalid_argument("ca_cfar scalar input must contain one element");
+        const float v = x.to_host()[0];
+        threshold = DeviceArray<float>::from_host({0.0f});
+        detections = DeviceArray<CfarDetection>::from_host({v > 0.0f ? CfarDetection::yes : CfarDetection::no});
+        return;
+    }
+    if (options.guard_cells.size() != shape.size() || options.reference_cells.size() != shape.size())
+        throw std::invalid_argument("ca_cfar cell parameter rank mismatch");
+    detail::check_abi(zkx_ca_cfar_f32(x.data(), static_cast<int>(shape.size()), shape.data(),
+                                      options.guard_cells.data(), options.reference_cells.data(), options.pfa,
+                                      threshold.data(), reinterpret_cast<uint8_t*>(detections.data()), nullptr,
+                                      nullptr));
+}
+
+// radartools_typed.h:154-170
+template <class T>
+CaCfarDeviceResult ca_cfar_device(const DeviceArray<T>& x, const std::vector<int>& shape, const CaCfarOptions& options)
+{
+    CaCfarDeviceResult result;
+    result.shape = shape;
+    result.threshold = DeviceArray<float>(x.size());
+    result.detections = DeviceArray<CfarDetection>(x.size());
+    ca_cfar_device(x, shape, options, result.threshold, result.detections);
+    return result;
+}
+
+}  // namespace cusignal

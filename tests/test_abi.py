@@ -38,3 +38,16 @@ def test_python_binding_covers_header():
     from zkx_dev_amd import _lib
     missing = [n for n in declared_functions() if n.startswith("zkx_") and n not in _lib.SIGNATURES]
     assert not missing, missing
+
+
+@pytest.mark.parametrize("name", ["task1_chain", "filter_chain"])
+def test_cpp_operator_api_compiles(name):
+    """The C++ operator API headers (include/cusignal) compile for a caller
+    written like the reference's Task1 steps."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "tests", "cpp", name + ".cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                        "-I", os.path.join(root, "include"), "-I", "/opt/rocm/include", src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -53,5 +53,21 @@ def build(verbose=False):
     return LIB
 
 
+def build_client(src, out, verbose=False):
+    """Compile a host C++ program written against include/cusignal (the
+    operator API) with g++, linked to libzkx_radar.so by rpath."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", inc, "-I", "/opt/rocm/include",
+           src, "-o", out, "-L", LIBDIR, "-lzkx_radar", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../../../zkx_dev_amd/lib",
+           "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {out}", file=sys.stderr)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
