@@ -206,16 +206,21 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         from oracle import oracle as orc
-        d = synth.make_dwell(P, R, TAPS, seed=1)
-        c0 = time.perf_counter()
-        pc = orc.pulse_compression(d.x, d.template, window="hamming", normalize=True)
-        rd = orc.pulse_doppler(pc.astype(np.complex64), window="hann")
-        pwr_h = (np.abs(rd) ** 2).astype(np.float32)
-        orc.ca_cfar(pwr_h, GUARD, REF, PFA)
-        cpu_s = time.perf_counter() - c0
-        cpu = {"value": 1.0 / cpu_s, "unit": "dwells/s", "cores": 1, "kind": "port",
-               "sample": "1 dwell 128x4096: direct FP64 circular correlation (256 taps), direct "
-                         "DFT Doppler, FP32 integral-image CFAR (the reference *_typed_cpu chain)",
+        # a bounded sample: distinct seeded dwells until about 10 s of CPU work
+        dwells_done, cpu_s = 0, 0.0
+        while dwells_done < 3 or (cpu_s < 10.0 and dwells_done < 30):
+            d = synth.make_dwell(P, R, TAPS, seed=100 + dwells_done)
+            c0 = time.perf_counter()
+            pc = orc.pulse_compression(d.x, d.template, window="hamming", normalize=True)
+            rd = orc.pulse_doppler(pc.astype(np.complex64), window="hann")
+            pwr_h = (np.abs(rd) ** 2).astype(np.float32)
+            orc.ca_cfar(pwr_h, GUARD, REF, PFA)
+            cpu_s += time.perf_counter() - c0
+            dwells_done += 1
+        cpu = {"value": dwells_done / cpu_s, "unit": "dwells/s", "cores": 1, "kind": "port",
+               "sample": f"{dwells_done} seeded dwells 128x4096 (about 10 s): direct FP64 circular "
+                         "correlation (256 taps), direct DFT Doppler, FP32 integral-image CFAR "
+                         "(the reference *_typed_cpu chain), input generation excluded",
                "seconds": cpu_s}
 
     out = {
