@@ -201,6 +201,65 @@ int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int r
                         zkx_complex64* rd_map, void* workspace, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* heterogeneous dwell batch engine                                         */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Many dwells of different pulse counts, range lengths and templates in one
+ * run: two persistent launches (pulse compression; Doppler FFT + |.|^2 + FP64
+ * windowed CA-CFAR) replayed from a HIP graph.  No reference counterpart: the
+ * reference runs the Task1 chain one dwell per operator call
+ * (Task1/pipeline/src/steps/step2/step2.cu:20-36, step3/step3.cu:17-35,
+ * step4/step4.cu:66-84).  Per dwell the results are those of
+ * pulse_compression (nfft = R, the engine's window and normalisation) ->
+ * pulse_doppler (nfft = P, the engine's window) -> zkx_ca_cfar_window_f32 on
+ * |.|^2 with the engine's guard/reference cells and pfa.
+ *
+ * Dwell p has P = num_pulses (a power of two in [16, 256]) pulses of
+ * R = range_bins samples at x + input_offset ([P][R] complex64); its outputs
+ * are [P][R] at threshold/detections (+ rd_map) + output_offset.  Templates
+ * are registered once and stay resident (with their spectra at every FFT
+ * length); template taps T <= R.  R need not be a power of two: such pulses
+ * are compressed by overlap-save segments of a power-of-two FFT over the
+ * periodic pulse, which gives the same length-R circular correlation.
+ *
+ * Lanes: each lane owns its device job tables, workspace and graph; calls on
+ * one lane must be issued from one host thread at a time and keep one
+ * stream (a change of stream waits for the previous one).  Runs on
+ * different lanes and streams overlap on the device.
+ */
+typedef struct zkx_engine zkx_engine;
+typedef struct {
+    int num_pulses;
+    int range_bins;
+    int template_id;
+    int reserved;
+    long long input_offset;  /* complex64 elements */
+    long long output_offset; /* cells */
+} zkx_dwell_desc;
+
+int zkx_engine_create(zkx_engine** engine, int pc_window, int normalize, int doppler_window,
+                      int guard_doppler, int guard_range, int ref_doppler, int ref_range, double pfa,
+                      int lanes);
+int zkx_engine_destroy(zkx_engine* engine);
+/* h: complex64 taps in device or host memory; returns the template id */
+int zkx_engine_add_template(zkx_engine* engine, const zkx_complex64* h, int taps, int* id);
+/* use_graph 0/1; grid sizes in blocks (0 keeps the current value) */
+int zkx_engine_set_options(zkx_engine* engine, int use_graph, int pc_grid, int dc_grid);
+int zkx_engine_run(zkx_engine* engine, int lane, const zkx_complex64* x, const zkx_dwell_desc* dwells,
+                   int n, float* threshold, uint8_t* detections, zkx_complex64* rd_map, void* stream);
+/* FFT length and segments per pulse the engine would use for each dwell */
+int zkx_engine_plan(zkx_engine* engine, const zkx_dwell_desc* dwells, int n, int* fft_length,
+                    int* segments);
+/* time candidate FFT lengths per (R, T) class and grid sizes on this batch */
+int zkx_engine_autotune(zkx_engine* engine, int lane, const zkx_complex64* x,
+                        const zkx_dwell_desc* dwells, int n, float* threshold, uint8_t* detections,
+                        void* stream);
+/* tuning table as text ("grid PC DC" / "pc R T N" lines); length excludes the NUL */
+int zkx_engine_tuning_export(zkx_engine* engine, char* buf, int cap, int* length);
+int zkx_engine_tuning_import(zkx_engine* engine, const char* text);
+
+/* ---------------------------------------------------------------------- */
 /* polyphase filtering                                                      */
 /* ---------------------------------------------------------------------- */
 

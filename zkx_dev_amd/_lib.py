@@ -48,6 +48,15 @@ SIGNATURES = {
     "zkx_firfilter_f32": (I, [P, I, P, I, I, I, P, P, P, P]),
     "zkx_fftconvolve_out_len": (LL, [LL, LL, I]),
     "zkx_fftconvolve_c64": (I, [P, I, I, P, I, I, P, P]),
+    "zkx_engine_create": (I, [ctypes.POINTER(P), I, I, I, I, I, I, I, D, I]),
+    "zkx_engine_destroy": (I, [P]),
+    "zkx_engine_add_template": (I, [P, P, I, ctypes.POINTER(I)]),
+    "zkx_engine_set_options": (I, [P, I, I, I]),
+    "zkx_engine_run": (I, [P, I, P, P, I, P, P, P, P]),
+    "zkx_engine_plan": (I, [P, P, I, P, P]),
+    "zkx_engine_autotune": (I, [P, I, P, P, I, P, P, P]),
+    "zkx_engine_tuning_export": (I, [P, ctypes.c_char_p, I, ctypes.POINTER(I)]),
+    "zkx_engine_tuning_import": (I, [P, ctypes.c_char_p]),
 }
 
 _lib = None

@@ -10,6 +10,7 @@
 //   windows            radartools_typed.cpp:238-251 (symmetric Hann/Hamming)
 //   Task1 chain        ZKX/Task1/pipeline/src/steps/step{2,3,4}/step{2,3,4}.cu
 #include "fft_lds.hpp"
+#include "radar_tiles.hpp"
 #include "runtime.hpp"
 
 #include <algorithm>
@@ -20,15 +21,8 @@
 namespace zkx {
 namespace {
 
-constexpr double kPi = 3.141592653589793238462643383279502884;
-
-// Symmetric window value in FP64 (kind 1 Hann, 2 Hamming), rounded by callers.
-__host__ __device__ inline double window_value(int kind, int i, int len)
-{
-    if (kind == ZKX_WINDOW_NONE || len <= 1) return 1.0;
-    const double c = cos(2.0 * kPi * static_cast<double>(i) / static_cast<double>(len - 1));
-    return kind == ZKX_WINDOW_HANN ? 0.5 - 0.5 * c : 0.54 - 0.46 * c;
-}
+using tiles::kPi;
+using tiles::window_value;
 
 // ---------------------------------------------------------------------------
 // row FFT kernels (one signal per slot, FPB slots per block)
@@ -618,23 +612,10 @@ cfar_window_kernel(const float* __restrict__ x, int rows, int cols, int g0, int 
 }
 
 // ---------------------------------------------------------------------------
-// Doppler FFT + |.|^2 + windowed CA-CFAR in one kernel
+// Doppler FFT + |.|^2 + windowed CA-CFAR in one kernel (tile body in
+// radar_tiles.hpp)
 // ---------------------------------------------------------------------------
-// A block transforms NCM consecutive range bins: its TCO = NCM - 2*cm output
-// bins plus the cm bins of CFAR halo on each side (their FFTs are recomputed
-// by the neighbouring blocks), so the power map lives only in LDS.  The CFAR
-// covers all N Doppler rows of the tile, so it needs no row halo (rows within
-// g0+r0 of the edges are border cells with T = 0).  Window sums follow
-// cfar_window_kernel: FP64 horizontal sums kept as float, FP64 vertical sums,
-// FP64 threshold and comparison.
-template <int N>
-struct FusedCfg {
-    static constexpr int E = fft::elems_for(N);
-    static constexpr int T = N / E;
-    static constexpr int NCM = 64;  // bins transformed per block
-    static constexpr int THREADS = NCM * T;
-    static constexpr int FFT_FLOATS = 2 * N * NCM + N;
-};
+using tiles::FusedCfg;
 
 template <int N>
 __global__ void __launch_bounds__((FusedCfg<N>::THREADS))
@@ -642,10 +623,7 @@ doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, cons
                     int g1, int r0, int r1, double alpha_over_n, float* __restrict__ thr,
                     unsigned char* __restrict__ det, float2* __restrict__ y, float* __restrict__ power)
 {
-    using C = FusedCfg<N>;
     extern __shared__ float lds[];
-    const int rm = g0 + r0, cm = g1 + r1;
-    const int TCO = C::NCM - 2 * cm;
     // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so hand
     // each XCD a contiguous run of tiles; neighbouring tiles then share their
     // overlapping halo columns in one L2.
@@ -655,108 +633,23 @@ doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, cons
     const int full = (total / 8) * 8;
     const int logical = lin < full ? (lin % 8) * (full / 8) + lin / 8 : lin;
     const int tile = logical % nx;
-    const int c0 = tile * TCO;  // first output bin of the block
-    const int cb = c0 - cm;           // bin of slot 0
-    float* wtab = lds + 2 * N * C::NCM;
-    const int used = P < N ? P : N;
-    for (int p = threadIdx.x; p < used; p += blockDim.x) wtab[p] = static_cast<float>(window_value(window, p, P));
-    __syncthreads();
-    const int s = threadIdx.x % C::NCM, t = threadIdx.x / C::NCM;
-    const int col = cb + s;
-    const bool inb = col >= 0 && col < R;
     const long long dwell = logical / nx;
-    const float2* src = x + dwell * static_cast<long long>(P) * R;
-    fft::ColLds<C::NCM> L{lds, lds + N * C::NCM, s};
-    float2 v[C::E];
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-        const int p = t + k * C::T;
-        v[k] = (inb && p < used) ? cscale(src[static_cast<long long>(p) * R + col], wtab[p]) : make_float2(0.f, 0.f);
-    }
-    fft::transform<N, C::E, false>(v, L, tw, t);
-    // |Y|^2 tile [N][NCM+1] over the FFT buffers, then horizontal sums [N][TCO+1]
-    constexpr int PS = C::NCM + 1;
-    const int HS = TCO + 1;
-    float* pw = lds;
-    float* ho = pw + N * PS;
-    float* hi = ho + N * HS;
     const long long obase = dwell * static_cast<long long>(N) * R;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-        const int f = t + k * C::T;
-        const float pv = v[k].x * v[k].x + v[k].y * v[k].y;
-        pw[f * PS + s] = pv;
-        if (inb && s >= cm && s < cm + TCO) {
-            const long long o = obase + static_cast<long long>(f) * R + col;
-            if (y) y[o] = v[k];
-            if (power) power[o] = pv;
-        }
-    }
-    __syncthreads();
-    constexpr int RUN = 8;
-    const int runs = (TCO + RUN - 1) / RUN;
-    for (int task = threadIdx.x; task < N * runs; task += blockDim.x) {
-        const int f = task % N, o0 = (task / N) * RUN;
-        const float* pr = pw + f * PS + o0;  // slot o0 .. : window of output o is slots o .. o+2cm
-        double so = 0.0, si = 0.0;
-        for (int d = 0; d <= 2 * cm; ++d) so += pr[d];
-        for (int d = cm - g1; d <= cm + g1; ++d) si += pr[d];
-        // all window reads of the run before any store, then the stores
-        float vo[RUN], vi[RUN];
-#pragma unroll
-        for (int j = 0; j < RUN; ++j) {
-            if (j > 0) {
-                so += static_cast<double>(pr[j + 2 * cm]) - static_cast<double>(pr[j - 1]);
-                si += static_cast<double>(pr[j + cm + g1]) - static_cast<double>(pr[j - 1 + cm - g1]);
-            }
-            vo[j] = static_cast<float>(so);
-            vi[j] = static_cast<float>(si);
-        }
-#pragma unroll
-        for (int j = 0; j < RUN; ++j) {
-            if (o0 + j < TCO) {
-                ho[f * HS + o0 + j] = vo[j];
-                hi[f * HS + o0 + j] = vi[j];
-            }
-        }
-    }
-    __syncthreads();
-    // vertical FP64 sliding sums: a task is one output bin and VR consecutive
-    // Doppler rows; x and the window reads are issued up front and the rows
-    // are unrolled, so no row waits on its own LDS load
-    constexpr int VR = N < 16 ? N : 16;
-    constexpr int vruns = N / VR;
-    for (int task = threadIdx.x; task < TCO * vruns; task += blockDim.x) {
-        const int o = task % TCO, f0 = (task / TCO) * VR;
-        const int c = c0 + o;
-        if (c >= R) continue;
-        float xv[VR];
-#pragma unroll
-        for (int j = 0; j < VR; ++j) xv[j] = pw[(f0 + j) * PS + o + cm];
-        const bool colin = c >= cm && c < R - cm;
-        const int fi = f0 > rm ? f0 : rm;
-        const int fe = (f0 + VR) < (N - rm) ? (f0 + VR) : (N - rm);
-        double so = 0.0, si = 0.0;
-        if (colin && fi < fe) {
-            for (int d = -rm; d <= rm; ++d) so += ho[(fi + d) * HS + o];
-            for (int d = -g0; d <= g0; ++d) si += hi[(fi + d) * HS + o];
-        }
-        float* tp = thr + obase + static_cast<long long>(f0) * R + c;
-        unsigned char* dp = det + obase + static_cast<long long>(f0) * R + c;
-#pragma unroll
-        for (int j = 0; j < VR; ++j) {
-            const int f = f0 + j;
-            const bool live = colin && f >= fi && f < fe;
-            if (live && f > fi) {
-                so += static_cast<double>(ho[(f + rm) * HS + o]) - static_cast<double>(ho[(f - rm - 1) * HS + o]);
-                si += static_cast<double>(hi[(f + g0) * HS + o]) - static_cast<double>(hi[(f - g0 - 1) * HS + o]);
-            }
-            const double T = live ? alpha_over_n * (so - si) : 0.0;
-            tp[static_cast<long long>(j) * R] = static_cast<float>(T);
-            dp[static_cast<long long>(j) * R] = static_cast<double>(xv[j]) > T ? 1 : 0;
-        }
-    }
+    tiles::DcTile a;
+    a.src = x + dwell * static_cast<long long>(P) * R;
+    a.P = P;
+    a.R = R;
+    a.window = window;
+    a.g0 = g0;
+    a.g1 = g1;
+    a.r0 = r0;
+    a.r1 = r1;
+    a.alpha_over_n = alpha_over_n;
+    a.thr = thr + obase;
+    a.det = det + obase;
+    a.y = y ? y + obase : nullptr;
+    a.power = power ? power + obase : nullptr;
+    tiles::doppler_cfar_tile<N>(a, tile, threadIdx.x, blockDim.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -1061,6 +954,11 @@ int fft_rows_dispatch(int n, const float2* x, long long rows, bool inverse, floa
                       hipStream_t st)
 {
     return fft_rows(n, x, rows, inverse, scale, y, st);
+}
+
+int pc_template_dispatch(int n, const float2* h, int taps, int normalize, int window, float2* H, hipStream_t st)
+{
+    return pc_template(n, h, taps, normalize, window, nullptr, H, st);
 }
 
 }  // namespace zkx

@@ -1,0 +1,223 @@
+// Tile bodies shared by the single-shape radar kernels (radar.hip) and the
+// heterogeneous batch engine (batch.hip):
+//   doppler_cfar_tile<N>  windowed N-point Doppler FFT of NCM range bins,
+//                         |Y|^2 and FP64 windowed CA-CFAR of the inner bins;
+//   pc_segment_row<N>     pulse compression of one N-sample segment of a
+//                         pulse by FFT -> x conj(H) -> IFFT (circular when
+//                         N == R, overlap-save over the periodic pulse
+//                         otherwise).
+// Each body is written for `nthr` cooperating threads with local index `tid`
+// and its own LDS region, so several tiles of the same N can share a block;
+// every thread of the block must make the call (the bodies hold barriers).
+#pragma once
+
+#include "fft_lds.hpp"
+
+namespace zkx {
+namespace tiles {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// Symmetric window value in FP64 (kind 1 Hann, 2 Hamming), rounded by callers
+// (radartools_typed.cpp:238-251 of the reference defines the windows).
+__host__ __device__ inline double window_value(int kind, int i, int len)
+{
+    if (kind == ZKX_WINDOW_NONE || len <= 1) return 1.0;
+    const double c = cos(2.0 * kPi * static_cast<double>(i) / static_cast<double>(len - 1));
+    return kind == ZKX_WINDOW_HANN ? 0.5 - 0.5 * c : 0.54 - 0.46 * c;
+}
+
+// ---------------------------------------------------------------------------
+// Doppler FFT + |.|^2 + windowed CA-CFAR tile
+// ---------------------------------------------------------------------------
+// A tile transforms NCM consecutive range bins: its TCO = NCM - 2*cm output
+// bins plus the cm bins of CFAR halo on each side (their FFTs are recomputed
+// by the neighbouring tiles), so the power map lives only in LDS.  The CFAR
+// covers all N Doppler rows of the tile, so it needs no row halo (rows within
+// g0+r0 of the edges are border cells with T = 0).  Window sums: FP64
+// horizontal sums kept as float, FP64 vertical sliding sums, FP64 threshold
+// and comparison.
+template <int N>
+struct FusedCfg {
+    static constexpr int E = fft::elems_for(N);
+    static constexpr int T = N / E;
+    static constexpr int NCM = 64;  // bins transformed per tile
+    static constexpr int THREADS = NCM * T;
+    static constexpr int FFT_FLOATS = 2 * N * NCM + N;
+    // LDS floats of one tile for a CFAR halo of cm bins
+    static constexpr int floats(int cm)
+    {
+        const int cf = N * (NCM + 1) + 2 * N * (NCM - 2 * cm + 1);
+        return FFT_FLOATS > cf ? FFT_FLOATS : cf;
+    }
+};
+
+struct DcTile {
+    const float2* src;  // the dwell, [P][R]
+    int P, R, window;
+    int g0, g1, r0, r1;
+    double alpha_over_n;
+    float* thr;           // [N][R] of this dwell
+    unsigned char* det;   // [N][R]
+    float2* y;            // optional range-Doppler map [N][R]
+    float* power;         // optional |Y|^2 [N][R]
+};
+
+template <int N>
+__device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int tid, int nthr, float* lds,
+                                                  const float2* __restrict__ tw)
+{
+    using C = FusedCfg<N>;
+    const int P = a.P, R = a.R;
+    const int g0 = a.g0, g1 = a.g1;
+    const int rm = a.g0 + a.r0, cm = a.g1 + a.r1;
+    const int TCO = C::NCM - 2 * cm;
+    const int c0 = tile * TCO;  // first output bin of the tile
+    const int cb = c0 - cm;     // bin of slot 0
+    float* wtab = lds + 2 * N * C::NCM;
+    const int used = P < N ? P : N;
+    for (int p = tid; p < used; p += nthr) wtab[p] = static_cast<float>(window_value(a.window, p, P));
+    __syncthreads();
+    const int s = tid % C::NCM, t = tid / C::NCM;
+    const int col = cb + s;
+    const bool inb = col >= 0 && col < R;
+    const float2* src = a.src;
+    fft::ColLds<C::NCM> L{lds, lds + N * C::NCM, s};
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int p = t + k * C::T;
+        v[k] = (inb && p < used) ? cscale(src[static_cast<long long>(p) * R + col], wtab[p]) : make_float2(0.f, 0.f);
+    }
+    fft::transform<N, C::E, false>(v, L, tw, t);
+    // |Y|^2 tile [N][NCM+1] over the FFT buffers, then horizontal sums [N][TCO+1]
+    constexpr int PS = C::NCM + 1;
+    const int HS = TCO + 1;
+    float* pw = lds;
+    float* ho = pw + N * PS;
+    float* hi = ho + N * HS;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int f = t + k * C::T;
+        const float pv = v[k].x * v[k].x + v[k].y * v[k].y;
+        pw[f * PS + s] = pv;
+        if (inb && s >= cm && s < cm + TCO) {
+            const long long o = static_cast<long long>(f) * R + col;
+            if (a.y) a.y[o] = v[k];
+            if (a.power) a.power[o] = pv;
+        }
+    }
+    __syncthreads();
+    constexpr int RUN = 8;
+    const int runs = (TCO + RUN - 1) / RUN;
+    for (int task = tid; task < N * runs; task += nthr) {
+        const int f = task % N, o0 = (task / N) * RUN;
+        const float* pr = pw + f * PS + o0;  // slot o0 .. : window of output o is slots o .. o+2cm
+        double so = 0.0, si = 0.0;
+        for (int d = 0; d <= 2 * cm; ++d) so += pr[d];
+        for (int d = cm - g1; d <= cm + g1; ++d) si += pr[d];
+        // all window reads of the run before any store, then the stores
+        float vo[RUN], vi[RUN];
+#pragma unroll
+        for (int j = 0; j < RUN; ++j) {
+            if (j > 0) {
+                so += static_cast<double>(pr[j + 2 * cm]) - static_cast<double>(pr[j - 1]);
+                si += static_cast<double>(pr[j + cm + g1]) - static_cast<double>(pr[j - 1 + cm - g1]);
+            }
+            vo[j] = static_cast<float>(so);
+            vi[j] = static_cast<float>(si);
+        }
+#pragma unroll
+        for (int j = 0; j < RUN; ++j) {
+            if (o0 + j < TCO) {
+                ho[f * HS + o0 + j] = vo[j];
+                hi[f * HS + o0 + j] = vi[j];
+            }
+        }
+    }
+    __syncthreads();
+    // vertical FP64 sliding sums: a task is one output bin and VR consecutive
+    // Doppler rows; x and the window reads are issued up front and the rows
+    // are unrolled, so no row waits on its own LDS load
+    constexpr int VR = N < 16 ? N : 16;
+    constexpr int vruns = N / VR;
+    for (int task = tid; task < TCO * vruns; task += nthr) {
+        const int o = task % TCO, f0 = (task / TCO) * VR;
+        const int c = c0 + o;
+        if (c >= R) continue;
+        float xv[VR];
+#pragma unroll
+        for (int j = 0; j < VR; ++j) xv[j] = pw[(f0 + j) * PS + o + cm];
+        const bool colin = c >= cm && c < R - cm;
+        const int fi = f0 > rm ? f0 : rm;
+        const int fe = (f0 + VR) < (N - rm) ? (f0 + VR) : (N - rm);
+        double so = 0.0, si = 0.0;
+        if (colin && fi < fe) {
+            for (int d = -rm; d <= rm; ++d) so += ho[(fi + d) * HS + o];
+            for (int d = -g0; d <= g0; ++d) si += hi[(fi + d) * HS + o];
+        }
+        float* tp = a.thr + static_cast<long long>(f0) * R + c;
+        unsigned char* dp = a.det + static_cast<long long>(f0) * R + c;
+#pragma unroll
+        for (int j = 0; j < VR; ++j) {
+            const int f = f0 + j;
+            const bool live = colin && f >= fi && f < fe;
+            if (live && f > fi) {
+                so += static_cast<double>(ho[(f + rm) * HS + o]) - static_cast<double>(ho[(f - rm - 1) * HS + o]);
+                si += static_cast<double>(hi[(f + g0) * HS + o]) - static_cast<double>(hi[(f - g0 - 1) * HS + o]);
+            }
+            const double T = live ? a.alpha_over_n * (so - si) : 0.0;
+            tp[static_cast<long long>(j) * R] = static_cast<float>(T);
+            dp[static_cast<long long>(j) * R] = static_cast<double>(xv[j]) > T ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pulse-compression segment
+// ---------------------------------------------------------------------------
+// Output bins [k0, k0 + count) of the circular correlation of one pulse
+// (R samples, periodic) with a template of at most N - count + 1 taps:
+//   y[k] = sum_t x[(k + t) mod R] conj(h[t])
+// The N samples x[(k0 + i) mod R], i < N, are transformed, multiplied by
+// conj(H) (H = FFT_N of the windowed, normalised template) and transformed
+// back; outputs i < count are exact (an overlap-save segment).  With N == R
+// and k0 == 0 this is the plain circular correlation of length R.
+// `t` indexes the T = N/E threads of the row; `row_lds` is the row's region.
+// (No __restrict__ on these pointers: with it the scheduler hoists the
+// spectrum loads above the forward transform and the row needs ~200 VGPRs.)
+template <int N>
+__device__ __forceinline__ void pc_segment_row(const float2* pulse, int R, int k0, int count,
+                                               bool active, const float2* H, float2* out,
+                                               int t, float* row_lds, const float2* tw)
+{
+    using C = fft::RowCfg<N>;
+    fft::RowLds L{row_lds, row_lds + C::SLOT_FLOATS};
+    float2 v[C::E];
+    // positions k0 + t + k*T reduced mod R by one conditional subtraction per
+    // step: callers guarantee k0 < R and T = N/E <= R
+    int pos = k0 + t;
+    pos -= pos >= R ? R : 0;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        v[k] = active ? pulse[pos] : make_float2(0.f, 0.f);
+        pos += C::T;
+        pos -= pos >= R ? R : 0;
+    }
+    fft::transform<N, C::E, false>(v, L, tw, t);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], H[t + k * C::T]);
+    fft::transform<N, C::E, true>(v, L, tw, t);
+    if (active) {
+        const float sc = 1.0f / static_cast<float>(N);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const int i = t + k * C::T;
+            if (i < count) out[k0 + i] = cscale(v[k], sc);
+        }
+    }
+}
+
+}  // namespace tiles
+}  // namespace zkx

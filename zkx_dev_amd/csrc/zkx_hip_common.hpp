@@ -81,6 +81,8 @@ __device__ __forceinline__ void stage_to_lds(V* dst, int count, F load)
     }
 }
 
+__host__ __device__ constexpr int ilog2_c(int n) { return n <= 1 ? 0 : 1 + ilog2_c(n / 2); }
+
 inline bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
 inline int ilog2(long long n)
 {
