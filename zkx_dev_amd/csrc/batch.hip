@@ -266,12 +266,20 @@ void pc_geometry(int R, int T, int nlog, int* L, int* S)
     *S = (R + l - 1) / l;
 }
 
+// Tuning classes: range length and the template length rounded up to a
+// multiple of 128 (a length tuned for the class serves every T in it).
+std::pair<int, int> tune_key(int R, int T) { return {R, (T + 127) / 128 * 128}; }
+
 // FFT length for a (R, T) class: the tuned choice, else the cheapest by
 // S * N * (log2 N + 2) per pulse (transform pair plus load/store).
 int choose_nlog(const zkx_engine& e, int R, int T)
 {
-    auto it = e.pc_choice.find({R, T});
-    if (it != e.pc_choice.end()) return it->second;
+    auto it = e.pc_choice.find(tune_key(R, T));
+    if (it != e.pc_choice.end()) {
+        int L, S;
+        pc_geometry(R, T, it->second, &L, &S);
+        if (L) return it->second;
+    }
     int best = -1;
     double best_cost = 0.0;
     for (int nl = kMinLog; nl <= kMaxLog; ++nl) {
@@ -695,7 +703,7 @@ int zkx_engine_autotune(zkx_engine* e, int lane, const zkx_complex64* x, const z
             rc = ZKX_INVALID_ARGUMENT;
             break;
         }
-        classes[{dwells[i].range_bins, e->templates[tid].ntaps}].push_back(dwells[i]);
+        classes[zkx::tune_key(dwells[i].range_bins, e->templates[tid].ntaps)].push_back(dwells[i]);
     }
     for (auto it = classes.begin(); it != classes.end() && !rc; ++it) {
         const std::vector<int> cands = zkx::candidate_nlogs(it->first.first, it->first.second);

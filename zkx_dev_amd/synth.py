@@ -76,3 +76,67 @@ def complex_noise_tones(channels, n, tones=3, seed=20261204):
             sig += np.exp(2j * np.pi * f * t)
         out[c] = sig
     return out
+
+
+@dataclass
+class UnevenBatch:
+    """Dwells of mixed shape packed back to back in one flat buffer."""
+    shapes: list            # [(P, R), ...]
+    taps: list              # template length per dwell
+    templates: list         # complex64 numpy templates
+    truth: list             # [(range_bins, doppler_bins), ...]
+    x: object               # flat complex64 torch tensor on the device
+    offsets: list           # element offset of each dwell in x
+
+
+def uneven_shapes(n, seed, pulses=(32, 64, 128, 256), r_lo=1024, r_hi=16384, r_step=256,
+                  t_lo=64, t_hi=1024):
+    """Seeded shapes of the uneven batch: P uniform over `pulses`, R uniform
+    over r_lo..r_hi in steps of r_step, T uniform in [t_lo, min(t_hi, R)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        P = int(rng.choice(pulses))
+        R = int(rng.integers(r_lo // r_step, r_hi // r_step + 1)) * r_step
+        T = int(rng.integers(t_lo, min(t_hi, R) + 1))
+        out.append((P, R, T))
+    return out
+
+
+def make_uneven_batch_device(specs, seed=20261206, snr_db=-10.0, targets=(1, 8), device="cuda",
+                             amplitude=1.0, bw_over_fs=0.5, margin=10):
+    """Dwells for specs [(P, R, T), ...] generated on the device (torch):
+    complex Gaussian noise at the given per-sample SNR plus 1-8 seeded point
+    targets per dwell (LFM echo at range bin d, Doppler tone at bin f)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    total = sum(P * R for P, R, _ in specs)
+    x = torch.empty(total, dtype=torch.complex64, device=device)
+    noise_power = amplitude ** 2 / (10.0 ** (snr_db / 10.0))
+    sigma = float(np.sqrt(noise_power / 2.0))
+    offsets, truth, templates = [], [], []
+    off = 0
+    for P, R, T in specs:
+        view = x[off:off + P * R].view(P, R)
+        re = torch.randn((P, R), generator=gen, device=device) * sigma
+        im = torch.randn((P, R), generator=gen, device=device) * sigma
+        view.copy_(torch.complex(re, im))
+        s = lfm_chirp(T, bw_over_fs)
+        k = int(rng.integers(targets[0], targets[1] + 1))
+        hi_r = max(margin + 1, R - margin - T)
+        rb = sorted(int(v) for v in rng.choice(np.arange(margin, hi_r), size=k, replace=hi_r - margin < k))
+        hi_f = max(margin + 1, P - margin)
+        fb = [int(v) for v in rng.integers(margin, hi_f, size=k)]
+        p = torch.arange(P, device=device, dtype=torch.float64)[:, None]
+        sd = torch.from_numpy(s.astype(np.complex128)).to(device)
+        for d, f in zip(rb, fb):
+            m = min(T, R - d)
+            tone = torch.exp(2j * np.pi * f * p / P)
+            view[:, d:d + m] += (amplitude * sd[None, :m] * tone).to(torch.complex64)
+        offsets.append(off)
+        truth.append((rb, fb))
+        templates.append(s)
+        off += P * R
+    return UnevenBatch([(P, R) for P, R, _ in specs], [T for _, _, T in specs], templates, truth, x, offsets)
