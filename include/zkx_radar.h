@@ -260,6 +260,47 @@ int zkx_engine_tuning_export(zkx_engine* engine, char* buf, int cap, int* length
 int zkx_engine_tuning_import(zkx_engine* engine, const char* text);
 
 /* ---------------------------------------------------------------------- */
+/* continuous-batching server                                               */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Requests name a dwell resident in the device input pool (complex64
+ * elements at input_offset, [P][R]) and a template registered with the
+ * engine.  A scheduler thread admits queued requests in arrival order while
+ * the HBM budget allows (13 bytes per sample: 8-byte pulse-compressed
+ * workspace + 4-byte threshold + 1-byte detections) and runs them as one
+ * engine batch on the first free lane (one stream per lane), so batches
+ * formed while others run overlap on the device.  No reference counterpart
+ * (the reference's Task1 chain handles one dwell per call,
+ * Task1/pipeline/src/task1_runner.cu:224-301).  The server uses engine lanes
+ * 0 .. lanes-1; the caller must not run those lanes itself meanwhile.
+ * keep_outputs: results stay in the server's arena until zkx_server_release;
+ * otherwise they are freed when the batch ends (latency-only use).
+ */
+typedef struct zkx_server zkx_server;
+int zkx_server_create(zkx_server** server, zkx_engine* engine, const zkx_complex64* input_pool,
+                      long long pool_elems, long long hbm_budget_bytes, int lanes, int max_batch_dwells,
+                      int keep_outputs);
+int zkx_server_destroy(zkx_server* server);
+int zkx_server_submit(zkx_server* server, long long input_offset, int num_pulses, int range_bins,
+                      int template_id, long long* ticket);
+int zkx_server_wait(zkx_server* server, long long ticket, int timeout_ms, int* done);
+int zkx_server_result(zkx_server* server, long long ticket, double* latency_us, const float** threshold,
+                      const uint8_t** detections);
+/* copy a kept request's outputs ([P][R] float, uint8) into caller device buffers */
+int zkx_server_copy_result(zkx_server* server, long long ticket, float* threshold, uint8_t* detections,
+                           void* stream);
+int zkx_server_release(zkx_server* server, long long ticket);
+/* hold (1) or resume (0) batch formation; queued requests wait while held */
+int zkx_server_hold(zkx_server* server, int hold);
+/* stats[5]: batches launched, dwells finished, queued, reserved bytes, peak reserved bytes */
+int zkx_server_stats(zkx_server* server, long long* stats);
+/* open-loop load from the calling thread: request i submitted at arrival_us[i] */
+int zkx_server_replay(zkx_server* server, const double* arrival_us, const long long* input_offset,
+                      const int* num_pulses, const int* range_bins, const int* template_id, int n,
+                      double* latency_us);
+
+/* ---------------------------------------------------------------------- */
 /* polyphase filtering                                                      */
 /* ---------------------------------------------------------------------- */
 

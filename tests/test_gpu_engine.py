@@ -166,3 +166,19 @@ def test_validation(setup):
     assert e.value.code == 1001
     with pytest.raises(ValueError):
         eng.run(x[:10], descs, thr, det)
+
+
+def test_small_grids_match(setup):
+    """Few persistent blocks, so every block runs many jobs of different
+    sizes back to back (LDS reuse between jobs) — results must not change."""
+    eng, _, _, descs, total, x = setup
+    a = _outputs(total)
+    eng.run(x, descs, *a)
+    torch.cuda.synchronize()
+    eng.set_options(use_graph=True, pc_grid=8, dc_grid=8)
+    b = _outputs(total)
+    eng.run(x, descs, *b)
+    torch.cuda.synchronize()
+    eng.set_options(use_graph=True, pc_grid=512, dc_grid=256)
+    for u, v in zip(a, b):
+        assert torch.equal(u.view(torch.uint8), v.view(torch.uint8))

@@ -57,6 +57,16 @@ SIGNATURES = {
     "zkx_engine_autotune": (I, [P, I, P, P, I, P, P, P]),
     "zkx_engine_tuning_export": (I, [P, ctypes.c_char_p, I, ctypes.POINTER(I)]),
     "zkx_engine_tuning_import": (I, [P, ctypes.c_char_p]),
+    "zkx_server_create": (I, [ctypes.POINTER(P), P, P, LL, LL, I, I, I]),
+    "zkx_server_destroy": (I, [P]),
+    "zkx_server_submit": (I, [P, LL, I, I, I, ctypes.POINTER(LL)]),
+    "zkx_server_wait": (I, [P, LL, I, ctypes.POINTER(I)]),
+    "zkx_server_result": (I, [P, LL, ctypes.POINTER(D), ctypes.POINTER(P), ctypes.POINTER(P)]),
+    "zkx_server_release": (I, [P, LL]),
+    "zkx_server_copy_result": (I, [P, LL, P, P, P]),
+    "zkx_server_stats": (I, [P, P]),
+    "zkx_server_hold": (I, [P, I]),
+    "zkx_server_replay": (I, [P, P, P, P, P, P, I, P]),
 }
 
 _lib = None

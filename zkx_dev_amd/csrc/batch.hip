@@ -657,14 +657,19 @@ int zkx_engine_plan(zkx_engine* e, const zkx_dwell_desc* dwells, int n, int* fft
     if (!e || (n > 0 && (!dwells || !fft_length || !segments))) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lk(e->mu);
     for (int i = 0; i < n; ++i) {
-        const int tid = dwells[i].template_id;
+        const zkx_dwell_desc& d = dwells[i];
+        if (!zkx::is_pow2_int(d.num_pulses) || d.num_pulses < 16 || d.num_pulses > 256 || d.range_bins < 1)
+            ZKX_FAIL(ZKX_INVALID_SHAPE, "engine dwells need a power-of-two pulse count in [16, 256]");
+        const int tid = d.template_id;
         if (tid < 0 || tid >= static_cast<int>(e->templates.size()))
             ZKX_FAIL(ZKX_INVALID_ARGUMENT, "engine dwell names an unknown template");
         const int T = e->templates[tid].ntaps;
-        const int nl = zkx::choose_nlog(*e, dwells[i].range_bins, T);
+        if (T > d.range_bins) ZKX_FAIL(ZKX_INVALID_SHAPE, "engine template longer than the range length");
+        const int nl = zkx::choose_nlog(*e, d.range_bins, T);
+        if (nl < 0) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "no FFT length serves this range length and template");
         int L = 0, S = 0;
-        if (nl >= 0) zkx::pc_geometry(dwells[i].range_bins, T, nl, &L, &S);
-        fft_length[i] = nl >= 0 ? (1 << nl) : 0;
+        zkx::pc_geometry(d.range_bins, T, nl, &L, &S);
+        fft_length[i] = 1 << nl;
         segments[i] = S;
     }
     return ZKX_SUCCESS;

@@ -76,6 +76,10 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
     const int cb = c0 - cm;     // bin of slot 0
     float* wtab = lds + 2 * N * C::NCM;
     const int used = P < N ? P : N;
+    // a block that runs several tiles in a row must not overwrite the window
+    // table (inside the previous tile's sum buffers) while that tile's last
+    // pass still reads them
+    __syncthreads();
     for (int p = tid; p < used; p += nthr) wtab[p] = static_cast<float>(window_value(a.window, p, P));
     __syncthreads();
     const int s = tid % C::NCM, t = tid / C::NCM;
