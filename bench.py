@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--cfar-mode", type=int, default=1)
     ap.add_argument("--plain-dwells", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dwell", "filters", "batched", "serve"], default="dwell")
+    ap.add_argument("--workload", choices=["dwell", "filters", "batched", "serve", "split"], default="dwell")
     ap.add_argument("--dwells", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
     return ap.parse_args()
@@ -244,6 +244,70 @@ def bench_serve(args):
     eng.close()
 
 
+def bench_split(args):
+    """The split-precision configuration: 16 channels x 512 pulses x 8192 range
+    complex64 (pulse-compressed data: noise plus seeded Doppler tones), the
+    512-point Doppler DFT as a split-BF16 MFMA contraction (FP32 accumulate)
+    beside the FP32 Stockham FFT, with the error of each against FP64 on one
+    channel.  Algorithmic work of the contraction: 8 P^2 R C flops."""
+    import torch
+    from oracle import oracle as orc
+    from zkx_dev_amd import radar
+
+    C, Pp, Rr = 16, 512, 8192
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(20261209)
+    x = torch.complex(torch.randn((C, Pp, Rr), generator=g, device=dev),
+                      torch.randn((C, Pp, Rr), generator=g, device=dev)) * (0.5 ** 0.5)
+    p = torch.arange(Pp, device=dev, dtype=torch.float32)[:, None]
+    for c in range(C):
+        for f in (17 + c, 200 + 3 * c, 400 - c):
+            x[c, :, (f * 37) % Rr] += torch.exp(2j * np.pi * f * p / Pp)[:, 0]
+    y = torch.empty_like(x)
+    pw = torch.empty((C, Pp, Rr), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    reps = max(3, args.steps // 5)
+    t3 = timed(lambda: radar.doppler_dft_mfma(x, window="hann", terms=3, out=y, power=pw), reps)
+    t1 = timed(lambda: radar.doppler_dft_mfma(x, window="hann", terms=1, out=y, power=pw), reps)
+    fy = torch.empty((Pp, Rr), dtype=torch.complex64, device=dev)
+
+    def fft_all():
+        for c in range(C):
+            radar.pulse_doppler(x[c], window="hann", widen=False, out=fy)
+    tf = timed(fft_all, reps)
+    # error on channel 0 against FP64
+    ref = orc.pulse_doppler_fft64(x[0].cpu().numpy(), window="hann")
+    e3 = orc.rel_l2(radar.doppler_dft_mfma(x[:1], window="hann", terms=3)[0].cpu().numpy(), ref)
+    e1 = orc.rel_l2(radar.doppler_dft_mfma(x[:1], window="hann", terms=1)[0].cpu().numpy(), ref)
+    radar.pulse_doppler(x[0], window="hann", widen=False, out=fy)
+    ef = orc.rel_l2(fy.cpu().numpy(), ref)
+    flops = 8.0 * Pp * Pp * Rr * C
+    peak = 2.5e15
+    print(json.dumps({
+        "metric": "split-BF16 Doppler DFT time and error vs FP64", "workload": "split",
+        "config": {"workload": "16 ch x 512 pulses x 8192 range complex64, 512-pt Doppler DFT, Hann"},
+        "dtype": "bf16x3 (split) with fp32 accumulate",
+        "mfma_split_ms": t3 * 1e3, "mfma_bf16_ms": t1 * 1e3, "fft_fp32_ms": tf * 1e3,
+        "roofline": {"bound": "mfma", "achieved": flops / t3 / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / t3 / peak, "issued_tflops": 3 * flops / t3 / 1e12,
+                     "algorithmic_flops": flops},
+        "rel_l2_vs_fp64": {"split_bf16": e3, "bf16": e1, "fft_fp32": ef},
+        "cube_bytes": C * Pp * Rr * 8}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -258,6 +322,10 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "split":
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        bench_split(args)
         return
     import torch
     from zkx_dev_amd import radar, synth

@@ -301,6 +301,22 @@ int zkx_server_replay(zkx_server* server, const double* arrival_us, const long l
                       double* latency_us);
 
 /* ---------------------------------------------------------------------- */
+/* split-BF16 matrix-core Doppler DFT                                       */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Doppler DFT (nfft = num_pulses) of channels x [P, R] complex64 cubes as a
+ * dense contraction on the MFMA units: Y' = A X' with A the (2P x 2P) real
+ * form of the windowed DFT matrix.  terms = 3: split-BF16 operands
+ * (hi*hi + hi*lo + lo*hi, FP32 accumulate); terms = 1: plain BF16.  The same
+ * transform as zkx_pulse_doppler_c64 with nfft = P (pulse_doppler_device,
+ * radartools_typed.cpp:714-808), computed as a matrix product; P need only be
+ * a multiple of 16 (<= 2048).  out [channels, P, R]; power optional |Y|^2.
+ */
+int zkx_doppler_dft_mfma_c64(const zkx_complex64* x, int channels, int num_pulses, int range_bins,
+                             int window, int terms, zkx_complex64* out, float* power, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* polyphase filtering                                                      */
 /* ---------------------------------------------------------------------- */
 

@@ -57,6 +57,7 @@ SIGNATURES = {
     "zkx_engine_autotune": (I, [P, I, P, P, I, P, P, P]),
     "zkx_engine_tuning_export": (I, [P, ctypes.c_char_p, I, ctypes.POINTER(I)]),
     "zkx_engine_tuning_import": (I, [P, ctypes.c_char_p]),
+    "zkx_doppler_dft_mfma_c64": (I, [P, I, I, I, I, I, P, P, P]),
     "zkx_server_create": (I, [ctypes.POINTER(P), P, P, LL, LL, I, I, I]),
     "zkx_server_destroy": (I, [P]),
     "zkx_server_submit": (I, [P, LL, I, I, I, ctypes.POINTER(LL)]),

@@ -150,6 +150,27 @@ def doppler_power(x, *, nfft=-1, window="none", explicit_window=None, stream=Non
     return power
 
 
+def doppler_dft_mfma(x, *, window="none", terms=3, out=None, power=None, stream=None):
+    """Doppler DFT over pulses (nfft = P) on the matrix cores with split-BF16
+    operands (terms=3) or plain BF16 (terms=1), FP32 accumulate.
+    x: complex64 [P, R] or [C, P, R]; returns the complex map (and fills
+    `power` with |Y|^2 when given).  P must be a multiple of 16."""
+    x = _device_tensor(x, torch.complex64, "x")
+    cube = x if x.dim() == 3 else x.unsqueeze(0)
+    C, P, R = cube.shape
+    codes = {"none": 0, "hann": 1, "hamming": 2}
+    if window not in codes:
+        raise ValueError("doppler_dft_mfma window must be none, hann or hamming")
+    y = out if out is not None else torch.empty_like(cube)
+    if y.shape != cube.shape or y.dtype != torch.complex64 or not y.is_contiguous():
+        raise ValueError("out must be a contiguous complex64 tensor of the input's shape")
+    if power is not None and (power.numel() != cube.numel() or power.dtype != torch.float32):
+        raise ValueError("power must be float32 with the input's element count")
+    check(lib().zkx_doppler_dft_mfma_c64(_ptr(cube), C, P, R, codes[window], int(terms), _ptr(y), _ptr(power),
+                                         _stream(stream)))
+    return y if x.dim() == 3 else y[0]
+
+
 def cfar_alpha(pfa, reference):
     """cfar_alpha_typed_cpu (radartools_typed.cpp:592-603): N*(pfa^(-1/N)-1), FP64."""
     out = ctypes.c_double()
