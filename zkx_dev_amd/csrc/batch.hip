@@ -364,7 +364,8 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
             if (!is_pow2_int(d.num_pulses) || d.num_pulses < (1 << kMinDopplerLog) ||
                 d.num_pulses > (1 << kMaxDopplerLog))
                 ZKX_FAIL(ZKX_INVALID_SHAPE, "engine dwells need a power-of-two pulse count in [16, 256]");
-            if (d.range_bins < 1) ZKX_FAIL(ZKX_INVALID_SHAPE, "engine dwell range length must be positive");
+            if (d.range_bins < 1 || static_cast<long long>(d.num_pulses) * d.range_bins >= (1LL << 31))
+                ZKX_FAIL(ZKX_INVALID_SHAPE, "engine dwell range length must be positive and P*R < 2^31");
             if (d.template_id < 0 || d.template_id >= static_cast<int>(e.templates.size()))
                 ZKX_FAIL(ZKX_INVALID_ARGUMENT, "engine dwell names an unknown template");
             if (d.input_offset < 0 || d.output_offset < 0)
@@ -658,7 +659,8 @@ int zkx_engine_plan(zkx_engine* e, const zkx_dwell_desc* dwells, int n, int* fft
     std::lock_guard<std::mutex> lk(e->mu);
     for (int i = 0; i < n; ++i) {
         const zkx_dwell_desc& d = dwells[i];
-        if (!zkx::is_pow2_int(d.num_pulses) || d.num_pulses < 16 || d.num_pulses > 256 || d.range_bins < 1)
+        if (!zkx::is_pow2_int(d.num_pulses) || d.num_pulses < 16 || d.num_pulses > 256 || d.range_bins < 1 ||
+            static_cast<long long>(d.num_pulses) * d.range_bins >= (1LL << 31))
             ZKX_FAIL(ZKX_INVALID_SHAPE, "engine dwells need a power-of-two pulse count in [16, 256]");
         const int tid = d.template_id;
         if (tid < 0 || tid >= static_cast<int>(e->templates.size()))

@@ -904,9 +904,9 @@ int launch_doppler_cfar(const float2* x, int batch, int P, int R, int window, in
     *done = false;
     const int cm = g1 + r1;
     if (C::NCM - 2 * cm < 8) return ZKX_SUCCESS;  // halo too wide for the tile: caller falls back
+    if (static_cast<long long>(N) * R >= (1LL << 31)) return ZKX_SUCCESS;  // tile offsets are 32-bit
     const int TCO = C::NCM - 2 * cm;
-    const size_t cfar_floats = static_cast<size_t>(N) * (C::NCM + 1) + 2 * static_cast<size_t>(N) * (TCO + 1);
-    const size_t lds = sizeof(float) * std::max<size_t>(C::FFT_FLOATS, cfar_floats);
+    const size_t lds = sizeof(float) * static_cast<size_t>(C::floats(cm));
     if (lds > 160 * 1024) return ZKX_SUCCESS;
     const float2* tw = nullptr;
     int rc = twiddles(N, &tw);
@@ -984,6 +984,7 @@ int zkx_set_kernel_variant(const char* op, int variant)
         g_doppler_variant = variant;
         return ZKX_SUCCESS;
     }
+
     ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unknown kernel variant op");
 }
 

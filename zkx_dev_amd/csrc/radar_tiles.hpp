@@ -47,7 +47,7 @@ struct FusedCfg {
     // LDS floats of one tile for a CFAR halo of cm bins
     static constexpr int floats(int cm)
     {
-        const int cf = N * (NCM + 1) + 2 * N * (NCM - 2 * cm + 1);
+        const int cf = N * (NCM + 1) + 2 * (NCM - 2 * cm) * (N + 1);
         return FFT_FLOATS > cf ? FFT_FLOATS : cf;
     }
 };
@@ -57,15 +57,36 @@ struct DcTile {
     int P, R, window;
     int g0, g1, r0, r1;
     double alpha_over_n;
-    float* thr;           // [N][R] of this dwell
+    float* thr;           // [N][R] of this dwell (P*R < 2^31: 32-bit offsets)
     unsigned char* det;   // [N][R]
     float2* y;            // optional range-Doppler map [N][R]
     float* power;         // optional |Y|^2 [N][R]
 };
 
+// Raw (unwindowed) samples of a tile's NCM columns, zero outside the dwell:
+// thread (slot s, row group t) gets rows t + k*T of column tile*TCO - cm + s.
 template <int N>
-__device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int tid, int nthr, float* lds,
-                                                  const float2* __restrict__ tw)
+__device__ __forceinline__ void dc_load(const DcTile& a, int tile, int tid, float2 (&buf)[FusedCfg<N>::E])
+{
+    using C = FusedCfg<N>;
+    const int cm = a.g1 + a.r1;
+    const int TCO = C::NCM - 2 * cm;
+    const int s = tid % C::NCM, t = tid / C::NCM;
+    const int col = tile * TCO - cm + s;
+    const bool inb = col >= 0 && col < a.R;
+    const int used = a.P < N ? a.P : N;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int p = t + k * C::T;
+        buf[k] = (inb && p < used) ? a.src[p * a.R + col] : make_float2(0.f, 0.f);
+    }
+}
+
+// The tile body on samples already in `buf` (from dc_load, issued before the
+// window-table barrier so the loads are in flight while the table is built).
+template <int N>
+__device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile, int tid, int nthr, float* lds,
+                                                      const float2* tw, const float2 (&buf)[FusedCfg<N>::E])
 {
     using C = FusedCfg<N>;
     const int P = a.P, R = a.R;
@@ -85,21 +106,23 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
     const int s = tid % C::NCM, t = tid / C::NCM;
     const int col = cb + s;
     const bool inb = col >= 0 && col < R;
-    const float2* src = a.src;
     fft::ColLds<C::NCM> L{lds, lds + N * C::NCM, s};
     float2 v[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
         const int p = t + k * C::T;
-        v[k] = (inb && p < used) ? cscale(src[static_cast<long long>(p) * R + col], wtab[p]) : make_float2(0.f, 0.f);
+        v[k] = p < used ? cscale(buf[k], wtab[p]) : make_float2(0.f, 0.f);
     }
     fft::transform<N, C::E, false>(v, L, tw, t);
-    // |Y|^2 tile [N][NCM+1] over the FFT buffers, then horizontal sums [N][TCO+1]
+    // |Y|^2 tile pw[N][NCM+1] over the FFT buffers, then the horizontal sums
+    // stored column-major, ho/hi[TCO][N+1], so that every LDS stride the
+    // vertical pass uses is a compile-time constant (immediate offsets).
+    // Global offsets are 32-bit from the dwell's base (P*R < 2^31).
     constexpr int PS = C::NCM + 1;
-    const int HS = TCO + 1;
+    constexpr int NS = N + 1;
     float* pw = lds;
     float* ho = pw + N * PS;
-    float* hi = ho + N * HS;
+    float* hi = ho + (C::NCM - 2 * cm) * NS;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
@@ -107,7 +130,7 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
         const float pv = v[k].x * v[k].x + v[k].y * v[k].y;
         pw[f * PS + s] = pv;
         if (inb && s >= cm && s < cm + TCO) {
-            const long long o = static_cast<long long>(f) * R + col;
+            const int o = f * R + col;
             if (a.y) a.y[o] = v[k];
             if (a.power) a.power[o] = pv;
         }
@@ -132,11 +155,13 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
             vo[j] = static_cast<float>(so);
             vi[j] = static_cast<float>(si);
         }
+        float* hoc = ho + o0 * NS + f;
+        float* hic = hi + o0 * NS + f;
 #pragma unroll
         for (int j = 0; j < RUN; ++j) {
             if (o0 + j < TCO) {
-                ho[f * HS + o0 + j] = vo[j];
-                hi[f * HS + o0 + j] = vi[j];
+                hoc[j * NS] = vo[j];
+                hic[j * NS] = vi[j];
             }
         }
     }
@@ -151,31 +176,44 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
         const int c = c0 + o;
         if (c >= R) continue;
         float xv[VR];
+        const float* pc = pw + f0 * PS + o + cm;
 #pragma unroll
-        for (int j = 0; j < VR; ++j) xv[j] = pw[(f0 + j) * PS + o + cm];
+        for (int j = 0; j < VR; ++j) xv[j] = pc[j * PS];
         const bool colin = c >= cm && c < R - cm;
         const int fi = f0 > rm ? f0 : rm;
         const int fe = (f0 + VR) < (N - rm) ? (f0 + VR) : (N - rm);
+        const float* hoc = ho + o * NS;
+        const float* hic = hi + o * NS;
         double so = 0.0, si = 0.0;
         if (colin && fi < fe) {
-            for (int d = -rm; d <= rm; ++d) so += ho[(fi + d) * HS + o];
-            for (int d = -g0; d <= g0; ++d) si += hi[(fi + d) * HS + o];
+            for (int d = -rm; d <= rm; ++d) so += hoc[fi + d];
+            for (int d = -g0; d <= g0; ++d) si += hic[fi + d];
         }
-        float* tp = a.thr + static_cast<long long>(f0) * R + c;
-        unsigned char* dp = a.det + static_cast<long long>(f0) * R + c;
+        int off = f0 * R + c;
 #pragma unroll
         for (int j = 0; j < VR; ++j) {
             const int f = f0 + j;
             const bool live = colin && f >= fi && f < fe;
             if (live && f > fi) {
-                so += static_cast<double>(ho[(f + rm) * HS + o]) - static_cast<double>(ho[(f - rm - 1) * HS + o]);
-                si += static_cast<double>(hi[(f + g0) * HS + o]) - static_cast<double>(hi[(f - g0 - 1) * HS + o]);
+                so += static_cast<double>(hoc[f + rm]) - static_cast<double>(hoc[f - rm - 1]);
+                si += static_cast<double>(hic[f + g0]) - static_cast<double>(hic[f - g0 - 1]);
             }
             const double T = live ? a.alpha_over_n * (so - si) : 0.0;
-            tp[static_cast<long long>(j) * R] = static_cast<float>(T);
-            dp[static_cast<long long>(j) * R] = static_cast<double>(xv[j]) > T ? 1 : 0;
+            a.thr[off] = static_cast<float>(T);
+            a.det[off] = static_cast<double>(xv[j]) > T ? 1 : 0;
+            off += R;
         }
     }
+}
+
+// One tile, loading its own samples.
+template <int N>
+__device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int tid, int nthr, float* lds,
+                                                  const float2* __restrict__ tw)
+{
+    float2 buf[FusedCfg<N>::E];
+    dc_load<N>(a, tile, tid, buf);
+    doppler_cfar_tile_buf<N>(a, tile, tid, nthr, lds, tw, buf);
 }
 
 // ---------------------------------------------------------------------------
