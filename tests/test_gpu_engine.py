@@ -182,3 +182,29 @@ def test_small_grids_match(setup):
     eng.set_options(use_graph=True, pc_grid=512, dc_grid=256)
     for u, v in zip(a, b):
         assert torch.equal(u.view(torch.uint8), v.view(torch.uint8))
+
+
+def test_engine_other_cfar_window():
+    """A window other than the compiled-in one (runtime cells).  The engine
+    sizes LDS for P up to 256, which needs a range halo of at least 10 bins."""
+    from zkx_dev_amd import engine, synth
+    guard, ref, pfa = (1, 3), (4, 8), 1e-5
+    eng = engine.DwellEngine(guard=guard, reference=ref, pfa=pfa)
+    mix = [(32, 1280, 100), (64, 1024, 64)]
+    dwells = [synth.make_dwell(p, r, t, n_targets=2, seed=900 + i) for i, (p, r, t) in enumerate(mix)]
+    tids = [eng.add_template(torch.from_numpy(d.template)) for d in dwells]
+    descs, total = engine.make_descs([(p, r) for p, r, _ in mix], tids)
+    x = torch.from_numpy(np.concatenate([d.x.reshape(-1) for d in dwells])).cuda()
+    thr, det, rd = _outputs(total)
+    eng.run(x, descs, thr, det, rd)
+    torch.cuda.synchronize()
+    for (p, r, t), d, row in zip(mix, dwells, descs):
+        o = int(row["output_offset"])
+        m = rd[o:o + p * r].cpu().numpy().reshape(p, r)
+        pw = (m.real.astype(np.float64) ** 2 + m.imag.astype(np.float64) ** 2).astype(np.float32)
+        exact = orc.ca_cfar_exact(pw, guard, ref, pfa)
+        assert np.allclose(thr[o:o + p * r].cpu().numpy().reshape(p, r), exact, rtol=1e-5, atol=1e-30)
+        near = np.abs(pw.astype(np.float64) - exact) <= 1e-6 * np.abs(exact)
+        bad = (det[o:o + p * r].cpu().numpy().reshape(p, r) != (pw.astype(np.float64) > exact)) & ~near
+        assert not bad.any()
+    eng.close()

@@ -70,3 +70,27 @@ def test_batch_equals_single(mods, mode):
     for b in range(4):
         t1, d1 = pipe.run(xb[b])
         assert torch.equal(t1[0], tb[b]) and torch.equal(d1[0], db[b])
+
+
+@pytest.mark.parametrize("guard,ref", [((1, 3), (4, 6)), ((2, 2), (8, 8)), ((0, 1), (3, 2))])
+def test_fused_doppler_cfar_windows_vs_fp64(mods, guard, ref):
+    """The fused kernel with the configured window (compiled-in cells) and
+    with other windows (runtime cells), against the FP64 chain."""
+    radar, synth = mods
+    d = synth.make_dwell(64, 1024, 64, n_targets=3, seed=31)
+    x = torch.from_numpy(d.x).cuda()
+    pc = radar.pulse_compression(x, torch.from_numpy(d.template).cuda(), normalize=True, window="hamming",
+                                 widen=False)
+    rd = torch.empty((64, 1024), dtype=torch.complex64, device="cuda")
+    thr, det = radar.doppler_cfar(pc, guard, ref, 1e-5, window="hann", rd_map=rd)
+    torch.cuda.synchronize()
+    m = rd.cpu().numpy()
+    ref64 = orc.pulse_doppler_fft64(orc.pulse_compression_fft64(d.x, d.template, window="hamming",
+                                                                normalize=True), window="hann")
+    assert orc.rel_l2(m, ref64) < 1e-5
+    pw = (m.real.astype(np.float64) ** 2 + m.imag.astype(np.float64) ** 2).astype(np.float32)
+    exact = orc.ca_cfar_exact(pw, guard, ref, 1e-5)
+    assert np.allclose(thr.cpu().numpy(), exact, rtol=1e-5, atol=1e-30)
+    near = np.abs(pw.astype(np.float64) - exact) <= 1e-6 * np.abs(exact)
+    bad = (det.cpu().numpy() != (pw.astype(np.float64) > exact)) & ~near
+    assert not bad.any()

@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kPcThreads, 4) pc_engine_kernel(const RunArgs*
     }
 }
 
-template <int N>
+template <int N, bool FIXED>
 __device__ __forceinline__ void dc_job(const RunArgs& ra, const Job& job, float* lds)
 {
     using C = tiles::FusedCfg<N>;
@@ -161,9 +161,15 @@ __device__ __forceinline__ void dc_job(const RunArgs& ra, const Job& job, float*
     // sub-blocks without a tile run the body on an out-of-range tile (no loads
     // or stores) so that every thread meets the same barriers
     const int tile = sub < job.count ? job.first + sub : (1 << 24);
-    tiles::doppler_cfar_tile<N>(a, tile, tid, TH, lds + sub * C::floats(ra.g1 + ra.r1), ra.tw[ilog2_c(N)]);
+    float* sl = lds + sub * C::floats(ra.g1 + ra.r1);
+    if constexpr (FIXED)
+        tiles::doppler_cfar_tile<N, 2, 2, 8, 8>(a, tile, tid, TH, sl, ra.tw[ilog2_c(N)]);
+    else
+        tiles::doppler_cfar_tile<N>(a, tile, tid, TH, sl, ra.tw[ilog2_c(N)]);
 }
 
+// FIXED: the engine's CFAR window is guard 2x2, reference 8x8 (compiled in)
+template <bool FIXED>
 __global__ void __launch_bounds__(kDcThreads) dc_engine_kernel(const RunArgs* __restrict__ rap)
 {
     extern __shared__ float lds[];
@@ -173,11 +179,11 @@ __global__ void __launch_bounds__(kDcThreads) dc_engine_kernel(const RunArgs* __
     for (; j < end; j += step) {
         const Job job = ra.dc_jobs[j];
         switch (job.nlog) {
-        case 4: dc_job<16>(ra, job, lds); break;
-        case 5: dc_job<32>(ra, job, lds); break;
-        case 6: dc_job<64>(ra, job, lds); break;
-        case 7: dc_job<128>(ra, job, lds); break;
-        case 8: dc_job<256>(ra, job, lds); break;
+        case 4: dc_job<16, FIXED>(ra, job, lds); break;
+        case 5: dc_job<32, FIXED>(ra, job, lds); break;
+        case 6: dc_job<64, FIXED>(ra, job, lds); break;
+        case 7: dc_job<128, FIXED>(ra, job, lds); break;
+        case 8: dc_job<256, FIXED>(ra, job, lds); break;
         default: break;
         }
     }
@@ -232,6 +238,7 @@ struct zkx_engine {
     int g0 = 2, g1 = 2, r0 = 8, r1 = 8;
     double pfa = 1e-6, alpha_over_n = 0.0;
     size_t dc_lds = 0;
+    bool fixed_window = false;
     bool use_graph = true;
     int pc_grid = 0, dc_grid = 0;
     std::vector<zkx::TemplateRec> templates;
@@ -320,7 +327,10 @@ int capture_graph(zkx_engine& e, Lane& lane)
     if (err == hipSuccess) {
         const RunArgs* ra = reinterpret_cast<const RunArgs*>(lane.dev);
         hipLaunchKernelGGL(pc_engine_kernel, dim3(e.pc_grid), dim3(kPcThreads), kPcLdsBytes, cs, ra);
-        hipLaunchKernelGGL(dc_engine_kernel, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, cs, ra);
+        if (e.fixed_window)
+            hipLaunchKernelGGL(dc_engine_kernel<true>, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, cs, ra);
+        else
+            hipLaunchKernelGGL(dc_engine_kernel<false>, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, cs, ra);
         err = hipStreamEndCapture(cs, &g);
     }
     if (err == hipSuccess) err = hipGraphInstantiate(&lane.exec, g, nullptr, nullptr, 0);
@@ -474,7 +484,10 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
         const RunArgs* rap = reinterpret_cast<const RunArgs*>(lane.dev);
         hipLaunchKernelGGL(pc_engine_kernel, dim3(e.pc_grid), dim3(kPcThreads), kPcLdsBytes, st, rap);
         ZKX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(dc_engine_kernel, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, st, rap);
+        if (e.fixed_window)
+            hipLaunchKernelGGL(dc_engine_kernel<true>, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, st, rap);
+        else
+            hipLaunchKernelGGL(dc_engine_kernel<false>, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, st, rap);
         ZKX_LAUNCH_CHECK();
     }
     ZKX_HIP_CHECK(hipEventRecord(lane.ev[s], st));
@@ -559,8 +572,12 @@ int zkx_engine_create(zkx_engine** out, int pc_window, int normalize, int dopple
         e->dc_grid = zkx::round8(e->cus);
         for (int l = zkx::kMinLog; l <= zkx::kMaxLog && !rc; ++l) rc = zkx::twiddles(1 << l, &e->tw[l]);
         if (rc) break;
-        err = hipFuncSetAttribute(reinterpret_cast<const void*>(zkx::dc_engine_kernel),
+        e->fixed_window = guard_doppler == 2 && guard_range == 2 && ref_doppler == 8 && ref_range == 8;
+        err = hipFuncSetAttribute(reinterpret_cast<const void*>(zkx::dc_engine_kernel<true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute(reinterpret_cast<const void*>(zkx::dc_engine_kernel<false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (err == hipSuccess)
             err = hipFuncSetAttribute(reinterpret_cast<const void*>(zkx::pc_engine_kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(zkx::kPcLdsBytes));

@@ -617,7 +617,7 @@ cfar_window_kernel(const float* __restrict__ x, int rows, int cols, int g0, int 
 // ---------------------------------------------------------------------------
 using tiles::FusedCfg;
 
-template <int N>
+template <int N, int CG0 = -1, int CG1 = -1, int CR0 = -1, int CR1 = -1>
 __global__ void __launch_bounds__((FusedCfg<N>::THREADS))
 doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, const float2* __restrict__ tw, int g0,
                     int g1, int r0, int r1, double alpha_over_n, float* __restrict__ thr,
@@ -649,7 +649,7 @@ doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, cons
     a.det = det + obase;
     a.y = y ? y + obase : nullptr;
     a.power = power ? power + obase : nullptr;
-    tiles::doppler_cfar_tile<N>(a, tile, threadIdx.x, blockDim.x, lds, tw);
+    tiles::doppler_cfar_tile<N, CG0, CG1, CR0, CR1>(a, tile, threadIdx.x, blockDim.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -911,13 +911,21 @@ int launch_doppler_cfar(const float2* x, int batch, int P, int R, int window, in
     const float2* tw = nullptr;
     int rc = twiddles(N, &tw);
     if (rc) return rc;
-    if ((rc = allow_lds(doppler_cfar_kernel<N>, lds))) return rc;
+    // the configured window (guard 2x2, reference 8x8) runs a kernel with the
+    // cell counts compiled in; any other window the runtime-cell kernel
+    const bool fixed = g0 == 2 && g1 == 2 && r0 == 8 && r1 == 8;
+    if ((rc = fixed ? allow_lds(doppler_cfar_kernel<N, 2, 2, 8, 8>, lds) : allow_lds(doppler_cfar_kernel<N>, lds)))
+        return rc;
     const int count = 2 * r0 * (2 * r1 + 2 * g1 + 1) + 2 * (2 * g0 + 1) * r1;
     const double n = static_cast<double>(count);
     const double alpha = n * (std::pow(pfa, -1.0 / n) - 1.0);
     dim3 grid((R + TCO - 1) / TCO, batch);
-    hipLaunchKernelGGL((doppler_cfar_kernel<N>), grid, dim3(C::THREADS), lds, st, x, P, R, window, tw, g0, g1, r0,
-                       r1, alpha / n, thr, det, y, power);
+    if (fixed)
+        hipLaunchKernelGGL((doppler_cfar_kernel<N, 2, 2, 8, 8>), grid, dim3(C::THREADS), lds, st, x, P, R, window, tw,
+                           g0, g1, r0, r1, alpha / n, thr, det, y, power);
+    else
+        hipLaunchKernelGGL((doppler_cfar_kernel<N>), grid, dim3(C::THREADS), lds, st, x, P, R, window, tw, g0, g1, r0,
+                           r1, alpha / n, thr, det, y, power);
     ZKX_LAUNCH_CHECK();
     *done = true;
     return ZKX_SUCCESS;

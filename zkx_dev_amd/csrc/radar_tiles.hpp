@@ -84,14 +84,16 @@ __device__ __forceinline__ void dc_load(const DcTile& a, int tile, int tid, floa
 
 // The tile body on samples already in `buf` (from dc_load, issued before the
 // window-table barrier so the loads are in flight while the table is built).
-template <int N>
+// CG0..CR1 >= 0 fix the CFAR cells at compile time (every window loop then
+// unrolls with constant LDS offsets); -1 reads them from `a`.
+template <int N, int CG0 = -1, int CG1 = -1, int CR0 = -1, int CR1 = -1>
 __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile, int tid, int nthr, float* lds,
                                                       const float2* tw, const float2 (&buf)[FusedCfg<N>::E])
 {
     using C = FusedCfg<N>;
     const int P = a.P, R = a.R;
-    const int g0 = a.g0, g1 = a.g1;
-    const int rm = a.g0 + a.r0, cm = a.g1 + a.r1;
+    const int g0 = CG0 >= 0 ? CG0 : a.g0, g1 = CG1 >= 0 ? CG1 : a.g1;
+    const int rm = g0 + (CR0 >= 0 ? CR0 : a.r0), cm = g1 + (CR1 >= 0 ? CR1 : a.r1);
     const int TCO = C::NCM - 2 * cm;
     const int c0 = tile * TCO;  // first output bin of the tile
     const int cb = c0 - cm;     // bin of slot 0
@@ -207,13 +209,13 @@ __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile,
 }
 
 // One tile, loading its own samples.
-template <int N>
+template <int N, int CG0 = -1, int CG1 = -1, int CR0 = -1, int CR1 = -1>
 __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int tid, int nthr, float* lds,
                                                   const float2* __restrict__ tw)
 {
     float2 buf[FusedCfg<N>::E];
     dc_load<N>(a, tile, tid, buf);
-    doppler_cfar_tile_buf<N>(a, tile, tid, nthr, lds, tw, buf);
+    doppler_cfar_tile_buf<N, CG0, CG1, CR0, CR1>(a, tile, tid, nthr, lds, tw, buf);
 }
 
 // ---------------------------------------------------------------------------
