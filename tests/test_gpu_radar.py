@@ -37,7 +37,10 @@ def rng_c64(rng, shape, scale=1.0):
 
 
 @pytest.mark.parametrize("P,S,T,nfft", [(8, 256, 64, -1), (4, 256, 64, 512), (3, 300, 40, 256),
-                                        (2, 4096, 256, -1), (5, 64, 100, 128), (2, 16, 5, 16)])
+                                        (2, 4096, 256, -1), (5, 64, 100, 128), (2, 16, 5, 16),
+                                        # other lengths: overlap-save segments over the period nfft
+                                        (4, 300, 40, -1), (3, 100, 120, 77), (2, 1000, 64, 1500), (5, 50, 8, 1),
+                                        (2, 20000, 256, -1), (3, 5, 3, 9)])
 @pytest.mark.parametrize("window", ["none", "hann", "hamming"])
 @pytest.mark.parametrize("normalize", [False, True])
 def test_pulse_compression_vs_fp64(R, P, S, T, nfft, window, normalize):
@@ -83,7 +86,9 @@ def test_pulse_compression_delta_template_is_shift(R):
 
 
 @pytest.mark.parametrize("P,Rg,nfft", [(16, 64, -1), (32, 100, -1), (128, 256, -1), (128, 96, 256),
-                                       (64, 33, 32), (512, 48, -1), (8, 5, 8), (2, 7, 4)])
+                                       (64, 33, 32), (512, 48, -1), (8, 5, 8), (2, 7, 4),
+                                       # other lengths: split-BF16 matrix-core DFT (error ~4e-6)
+                                       (100, 64, -1), (48, 30, 60), (130, 20, 100), (3, 9, 5), (1000, 8, -1)])
 @pytest.mark.parametrize("window", ["none", "hann", "hamming"])
 def test_pulse_doppler_vs_fp64(R, P, Rg, nfft, window):
     rng = np.random.default_rng(P * 7 + Rg)
@@ -92,6 +97,16 @@ def test_pulse_doppler_vs_fp64(R, P, Rg, nfft, window):
     ref = orc.pulse_doppler_fft64(x, nfft=nfft, window=window)
     assert y.shape == ref.shape
     assert orc.rel_l2(y, ref) < TOL
+
+
+def test_pulse_doppler_explicit_window_other_length(R):
+    rng = np.random.default_rng(77)
+    P, Rg, nfft = 40, 16, 50
+    x = rng_c64(rng, (P, Rg))
+    w = rng.uniform(0.2, 1.0, P).astype(np.float32)
+    y = R.pulse_doppler(dev(x), nfft=nfft, window="explicit_fp32", explicit_window=dev(w), widen=False)
+    ref = np.fft.fft(x.astype(np.complex128) * w.astype(np.float64)[:, None], nfft, axis=0)
+    assert orc.rel_l2(y.cpu().numpy(), ref) < TOL
 
 
 def test_pulse_doppler_power_and_tone(R):

@@ -107,7 +107,7 @@ __device__ __forceinline__ void pc_job(const RunArgs& ra, const Job& job, float*
     const int count = min(d.L, d.R - k0);
     const float2* pulse = ra.x + d.x_off + static_cast<long long>(p) * d.R;
     float2* out = ra.ws + d.w_off + static_cast<long long>(p) * d.R;
-    tiles::pc_segment_row<N>(pulse, d.R, k0, count, active, d.H, out, t, lds + slot * 2 * C::SLOT_FLOATS,
+    tiles::pc_segment_row<N>(pulse, d.R, d.R, k0, count, active, d.H, out, t, lds + slot * 2 * C::SLOT_FLOATS,
                              ra.tw[ilog2_c(N)]);
 }
 

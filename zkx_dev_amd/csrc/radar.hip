@@ -29,11 +29,12 @@ using tiles::window_value;
 // ---------------------------------------------------------------------------
 using fft::RowCfg;
 
-// Template spectrum H = FFT_N(h~), h~ = window*h / ||window*h|| (normalize)
-// truncated/zero-padded to N.  One block, FPB slots but only slot 0 works.
+// Template spectrum H = FFT_N(h~), h~ = window*h / ||window*h|| (normalize
+// over all taps) keeping the first `keep` taps, zero-padded to N.  One block,
+// FPB slots but only slot 0 works.
 template <int N>
 __global__ void __launch_bounds__(RowCfg<N>::THREADS)
-pc_template_kernel(const float2* __restrict__ h, int taps, int normalize, int window,
+pc_template_kernel(const float2* __restrict__ h, int taps, int keep, int normalize, int window,
                    const float* __restrict__ explicit_window, const float2* __restrict__ tw,
                    float2* __restrict__ spectrum)
 {
@@ -63,7 +64,7 @@ pc_template_kernel(const float2* __restrict__ h, int taps, int normalize, int wi
     for (int k = 0; k < C::E; ++k) {
         const int i = t + k * C::T;
         float2 val = make_float2(0.f, 0.f);
-        if (slot == 0 && i < taps) {
+        if (slot == 0 && i < taps && i < keep) {
             const double w = expl ? static_cast<double>(explicit_window[i]) : window_value(window, i, taps);
             val = make_float2(static_cast<float>(h[i].x * w * scale), static_cast<float>(h[i].y * w * scale));
         }
@@ -106,6 +107,29 @@ pc_fused_kernel(const float2* __restrict__ x, long long rows, int S, const float
 #pragma unroll
         for (int k = 0; k < C::E; ++k) dst[t + k * C::T] = cscale(v[k], s);
     }
+}
+
+// Pulse compression of any nfft by overlap-save segments: each of the
+// P * nseg rows is one N-point segment (tiles::pc_segment_row) of a pulse's
+// circular correlation over period nfft (the first `valid` = min(S, nfft)
+// samples of the pulse, zeros after them); L outputs per segment.
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+pc_segments_kernel(const float2* __restrict__ x, long long rows, int nseg, int L, int nfft, int valid, int S,
+                   const float2* __restrict__ H, const float2* __restrict__ tw, float2* __restrict__ y)
+{
+    using C = RowCfg<N>;
+    extern __shared__ float lds[];
+    const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
+    const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
+    const bool active = row < rows;
+    const long long r = active ? row : 0;
+    const long long p = r / nseg;
+    const int seg = static_cast<int>(r - p * nseg);
+    const int k0 = seg * L;
+    const int count = min(L, nfft - k0);
+    tiles::pc_segment_row<N>(x + p * S, nfft, valid, k0, count, active, H, y + p * nfft, t,
+                             lds + slot * 2 * C::SLOT_FLOATS, tw);
 }
 
 // Persistent form: each block walks rows blockIdx.x*FPB+slot, +gridDim.x*FPB,
@@ -688,7 +712,7 @@ int launch_fft_rows(const float2* x, long long rows, bool inverse, float scale, 
 }
 
 template <int N>
-int launch_pc_template(const float2* h, int taps, int normalize, int window, const float* ew, float2* H,
+int launch_pc_template(const float2* h, int taps, int keep, int normalize, int window, const float* ew, float2* H,
                        hipStream_t st)
 {
     using C = RowCfg<N>;
@@ -696,8 +720,24 @@ int launch_pc_template(const float2* h, int taps, int normalize, int window, con
     int rc = twiddles(N, &tw);
     if (rc) return rc;
     if ((rc = allow_lds(pc_template_kernel<N>, C::LDS_BYTES))) return rc;
-    hipLaunchKernelGGL((pc_template_kernel<N>), dim3(1), dim3(C::THREADS), C::LDS_BYTES, st, h, taps,
+    hipLaunchKernelGGL((pc_template_kernel<N>), dim3(1), dim3(C::THREADS), C::LDS_BYTES, st, h, taps, keep,
                        normalize, window, ew, tw, H);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+template <int N>
+int launch_pc_segments(const float2* x, long long rows, int nseg, int L, int nfft, int valid, int S,
+                       const float2* H, float2* y, hipStream_t st)
+{
+    using C = RowCfg<N>;
+    const float2* tw = nullptr;
+    int rc = twiddles(N, &tw);
+    if (rc) return rc;
+    if ((rc = allow_lds(pc_segments_kernel<N>, C::LDS_BYTES))) return rc;
+    const long long blocks = (rows + C::FPB - 1) / C::FPB;
+    hipLaunchKernelGGL((pc_segments_kernel<N>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS), C::LDS_BYTES,
+                       st, x, rows, nseg, L, nfft, valid, S, H, tw, y);
     ZKX_LAUNCH_CHECK();
     return ZKX_SUCCESS;
 }
@@ -793,10 +833,41 @@ int fft_rows(int n, const float2* x, long long rows, bool inverse, float scale, 
 }
 
 int pc_template(int n, const float2* h, int taps, int normalize, int window, const float* ew, float2* H,
-                hipStream_t st)
+                hipStream_t st, int keep = -1)
 {
-    ZKX_POW2_CASES(launch_pc_template, h, taps, normalize, window, ew, H, st)
+    if (keep < 0) keep = taps;
+    ZKX_POW2_CASES(launch_pc_template, h, taps, keep, normalize, window, ew, H, st)
     ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression nfft must be a power of two in [2, 8192]");
+}
+
+int pc_segments(int n, const float2* x, long long rows, int nseg, int L, int nfft, int valid, int S,
+                const float2* H, float2* y, hipStream_t st)
+{
+    ZKX_POW2_CASES(launch_pc_segments, x, rows, nseg, L, nfft, valid, S, H, y, st)
+    ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "segment FFT length must be a power of two in [16, 8192]");
+}
+
+// Segment FFT length for a circular correlation of period nfft with T taps:
+// N > T, N/16 <= nfft (one-subtraction wrap), N <= 8192, cheapest by
+// segments * N * (log2 N + 2).  Returns 0 if none fits.
+int pc_segment_length(int nfft, int T, int* L_out, int* nseg_out)
+{
+    int best = 0;
+    double best_cost = 0.0;
+    for (int nl = 4; nl <= 13; ++nl) {
+        const int n = 1 << nl;
+        if (n <= T || n / 16 > nfft) continue;
+        const int L = std::min(n - T + 1, nfft);
+        const int ns = (nfft + L - 1) / L;
+        const double cost = static_cast<double>(ns) * n * (nl + 2);
+        if (!best || cost < best_cost) {
+            best = n;
+            best_cost = cost;
+            *L_out = L;
+            *nseg_out = ns;
+        }
+    }
+    return best;
 }
 
 int pc_fused(int n, const float2* x, long long rows, int S, const float2* H, float2* y, hipStream_t st)
@@ -958,6 +1029,9 @@ int validate_cfar2d(int rows, int cols, int g0, int g1, int r0, int r1, double p
 }  // namespace
 
 // exported for other translation units of the library
+int doppler_dft(const float2* x, int channels, int P, int R, int nfft, int window, const float* ew, int terms,
+                float2* out, float* power, hipStream_t st);
+
 int fft_rows_dispatch(int n, const float2* x, long long rows, bool inverse, float scale, float2* y,
                       hipStream_t st)
 {
@@ -1048,10 +1122,25 @@ int zkx_pulse_compression_c64(const zkx_complex64* x, int num_pulses, int sample
     if (!valid_window(window)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unsupported pulse_compression window");
     if (explicit_kind(window) != (explicit_window != nullptr))
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_compression window is invalid");
-    if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft)
-        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression nfft must be a power of two in [2, 8192]");
     // per-call spectrum buffer, kept per host thread like the reference's plan pool
     static thread_local ThreadWorkspace ws;
+    if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft) {
+        // any other nfft: overlap-save segments of a power-of-two FFT over
+        // the period nfft (the reference uses Bluestein for these lengths)
+        const int keep = std::min(template_len, nfft);
+        int L = 0, nseg = 0;
+        const int N = pc_segment_length(nfft, keep, &L, &nseg);
+        if (!N) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression: template too long for this nfft");
+        void* H = nullptr;
+        int rc = ws.reserve(sizeof(float2) * static_cast<size_t>(N), &H);
+        if (rc) return rc;
+        rc = pc_template(N, reinterpret_cast<const float2*>(tmpl), template_len, normalize, window, explicit_window,
+                         static_cast<float2*>(H), (hipStream_t)stream, keep);
+        if (rc) return rc;
+        return pc_segments(N, reinterpret_cast<const float2*>(x), static_cast<long long>(num_pulses) * nseg, nseg, L,
+                           nfft, std::min(samples_per_pulse, nfft), samples_per_pulse,
+                           static_cast<const float2*>(H), reinterpret_cast<float2*>(out), (hipStream_t)stream);
+    }
     void* H = nullptr;
     int rc = ws.reserve(sizeof(float2) * static_cast<size_t>(nfft), &H);
     if (rc) return rc;
@@ -1075,8 +1164,14 @@ int zkx_pulse_doppler_c64(const zkx_complex64* x, int num_pulses, int samples_pe
     if (!valid_window(window)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unsupported pulse_doppler window");
     if (explicit_kind(window) != (explicit_window != nullptr))
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_doppler window is invalid");
-    if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft)
-        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_doppler nfft must be a power of two in [2, 8192]");
+    if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft) {
+        // other lengths: the DFT as a split-BF16 matrix-core contraction
+        // (the reference uses Bluestein for these lengths)
+        if (nfft > 4096 || num_pulses > 4096)
+            ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_doppler: non-power-of-two nfft must be <= 4096");
+        return doppler_dft(reinterpret_cast<const float2*>(x), 1, num_pulses, samples_per_pulse, nfft, window,
+                           explicit_window, 3, reinterpret_cast<float2*>(out), power, (hipStream_t)stream);
+    }
     return doppler(nfft, reinterpret_cast<const float2*>(x), 1, num_pulses, samples_per_pulse, window,
                    explicit_window, reinterpret_cast<float2*>(out), power, nullptr, (hipStream_t)stream);
 }

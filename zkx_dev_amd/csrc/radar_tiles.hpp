@@ -221,8 +221,9 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
 // ---------------------------------------------------------------------------
 // Pulse-compression segment
 // ---------------------------------------------------------------------------
-// Output bins [k0, k0 + count) of the circular correlation of one pulse
-// (R samples, periodic) with a template of at most N - count + 1 taps:
+// Output bins [k0, k0 + count) of the circular correlation of one pulse of
+// period R (its first `valid` samples in memory, zeros after them) with a
+// template of at most N - count + 1 taps:
 //   y[k] = sum_t x[(k + t) mod R] conj(h[t])
 // The N samples x[(k0 + i) mod R], i < N, are transformed, multiplied by
 // conj(H) (H = FFT_N of the windowed, normalised template) and transformed
@@ -232,7 +233,7 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
 // (No __restrict__ on these pointers: with it the scheduler hoists the
 // spectrum loads above the forward transform and the row needs ~200 VGPRs.)
 template <int N>
-__device__ __forceinline__ void pc_segment_row(const float2* pulse, int R, int k0, int count,
+__device__ __forceinline__ void pc_segment_row(const float2* pulse, int R, int valid, int k0, int count,
                                                bool active, const float2* H, float2* out,
                                                int t, float* row_lds, const float2* tw)
 {
@@ -245,7 +246,7 @@ __device__ __forceinline__ void pc_segment_row(const float2* pulse, int R, int k
     pos -= pos >= R ? R : 0;
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
-        v[k] = active ? pulse[pos] : make_float2(0.f, 0.f);
+        v[k] = (active && pos < valid) ? pulse[pos] : make_float2(0.f, 0.f);
         pos += C::T;
         pos -= pos >= R ? R : 0;
     }
