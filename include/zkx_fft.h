@@ -12,10 +12,13 @@
  *   - plans are opaque handles;
  *   - FFTResult codes 0 ok, 1 invalid value, 2 alloc failed, 3 exec failed,
  *     99 internal error.
- * This build supports C2C transforms of power-of-two length 2..8192 (one
- * LDS-resident pass per transform); other lengths and the R2C/C2R/2-D plan
- * kinds return FFT_INVALID_VALUE.  All work runs on the default stream, as
- * the reference's backends do.
+ * Lengths: powers of two 1..8192 (one LDS-resident pass per row) and any
+ * other length up to 4096 (Bluestein, one fused kernel per row); longer
+ * plans return FFT_INVALID_VALUE.  Plan kinds: 1-D C2C, R2C, C2R (with the
+ * reference backend's half-spectrum mirroring and scaling,
+ * fft_thrust.cu:1096-1160) and 2-D (complex data for every type, as
+ * fft_thrust.cu:1014-1058).  fft_execute* run on the default stream, as the
+ * reference's backends do; fft_execute_on_stream takes a stream.
  */
 #ifndef ZKX_FFT_H
 #define ZKX_FFT_H
@@ -39,7 +42,7 @@ typedef enum {
 /* fft_interface.h:189-205 */
 FFTResult fft_plan_create_1d(FFTPlanHandle* plan, int nx, int batch, FFTType type,
                              FFTDirection direction);
-/* fft_interface.h:207-222 (2-D plans: not supported by this build) */
+/* fft_interface.h:207-222: 2-D plan over complex [ny][nx] */
 FFTResult fft_plan_create_2d(FFTPlanHandle* plan, int nx, int ny, FFTType type,
                              FFTDirection direction);
 /* fft_interface.h:224-240: out-of-place execute */

@@ -42,11 +42,58 @@ def test_fft_parseval_and_roundtrip(Fm):
 
 def test_fft_plan_errors(Fm):
     with pytest.raises(Fm.FFTError):
-        Fm.Plan(1000)
+        Fm.Plan(5000)  # Bluestein lengths up to 4096
     with pytest.raises(Fm.FFTError):
         Fm.Plan(16384)
     with pytest.raises(Fm.FFTError):
         Fm.Plan(64, batch=0)
+    with pytest.raises(Fm.FFTError):
+        Fm.Plan(64, type=7)
+
+
+@pytest.mark.parametrize("n", [1, 3, 5, 7, 12, 100, 127, 1000, 1500, 4095])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fft_bluestein_vs_numpy(Fm, n, inverse):
+    """Lengths that are not powers of two (Bluestein), forward unnormalised,
+    inverse 1/n, as the reference backend."""
+    rng = np.random.default_rng(n + 7)
+    batch = max(1, 2048 // n)
+    x = (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))).astype(np.complex64)
+    y = Fm.fft(torch.from_numpy(x).cuda(), inverse=inverse).cpu().numpy()
+    ref = np.fft.ifft(x.astype(np.complex128), axis=1) if inverse else np.fft.fft(x.astype(np.complex128), axis=1)
+    assert orc.rel_l2(y, ref) < 1e-5
+
+
+@pytest.mark.parametrize("n", [8, 64, 4096, 6, 9, 100, 1001])
+def test_rfft_and_irfft_reference_semantics(Fm, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((3, n)).astype(np.float32)
+    X = Fm.rfft(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert X.shape == (3, n // 2 + 1)
+    assert orc.rel_l2(X, np.fft.rfft(x.astype(np.float64), axis=1)) < 1e-5
+    # C2R: bins 0 .. n/2 given, bins 1 .. n/2+1-2 mirrored, the rest zero,
+    # inverse scaled by 1/n (and once more by 1/n for non-power-of-two n)
+    half = n // 2 + 1
+    Xin = (rng.standard_normal((2, half)) + 1j * rng.standard_normal((2, half))).astype(np.complex64)
+    full = np.zeros((2, n), np.complex128)
+    full[:, :half] = Xin
+    for m in range(1, half - 1):
+        full[:, n - m] = np.conj(Xin[:, m])
+    expect = np.fft.ifft(full, axis=1).real  # unnormalised inverse, then 1/n
+    if n & (n - 1):
+        expect = expect / n
+    got = Fm.irfft(torch.from_numpy(Xin).cuda(), n).cpu().numpy()
+    assert orc.rel_l2(got, expect) < 1e-5
+
+
+@pytest.mark.parametrize("ny,nx", [(64, 128), (48, 100), (7, 9), (1, 256), (256, 3)])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fft2_vs_numpy(Fm, ny, nx, inverse):
+    rng = np.random.default_rng(ny * 1000 + nx)
+    x = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(np.complex64)
+    y = Fm.fft2(torch.from_numpy(x).cuda(), inverse=inverse).cpu().numpy()
+    ref = np.fft.ifft2(x.astype(np.complex128)) if inverse else np.fft.fft2(x.astype(np.complex128))
+    assert orc.rel_l2(y, ref) < 1e-5
 
 
 @pytest.mark.parametrize("n", [7, 8, 101])

@@ -13,9 +13,11 @@ import torch
 from ._lib import lib
 
 FFT_FORWARD, FFT_INVERSE = 1, -1
-FFT_TYPE_C2C = 0
+FFT_TYPE_C2C, FFT_TYPE_R2C, FFT_TYPE_C2R = 0, 1, 2
 _FFT_SIGS = {
     "fft_plan_create_1d": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int]),
+    "fft_plan_create_2d": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int]),
     "fft_execute": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fft_execute_on_stream": (ctypes.c_int, [ctypes.c_void_p] * 4),
@@ -57,18 +59,37 @@ def _check(code):
 
 
 class Plan:
-    """fft_plan_create_1d / fft_execute / fft_plan_destroy."""
+    """fft_plan_create_1d / fft_plan_create_2d, fft_execute, fft_plan_destroy.
 
-    def __init__(self, nx, batch=1, direction=FFT_FORWARD):
+    1-D: `nx` points, `batch` rows, type C2C (complex64 rows), R2C (float32
+    rows in, nx/2+1 complex64 bins out) or C2R (nx/2+1 bins in, float32 rows
+    out).  2-D (ny given): a complex64 [ny, nx] array, as the reference's
+    2-D plans transform complex data whatever their type."""
+
+    def __init__(self, nx, batch=1, direction=FFT_FORWARD, type=FFT_TYPE_C2C, ny=None):
         h = ctypes.c_void_p()
-        _check(_L().fft_plan_create_1d(ctypes.byref(h), int(nx), int(batch), FFT_TYPE_C2C, int(direction)))
-        self.handle, self.nx, self.batch = h, int(nx), int(batch)
+        if ny is None:
+            _check(_L().fft_plan_create_1d(ctypes.byref(h), int(nx), int(batch), int(type), int(direction)))
+        else:
+            _check(_L().fft_plan_create_2d(ctypes.byref(h), int(nx), int(ny), int(type), int(direction)))
+        self.handle, self.nx, self.batch, self.type, self.ny = h, int(nx), int(batch), int(type), ny
+
+    def _shapes(self):
+        if self.ny is not None:
+            return (torch.complex64, self.nx * self.ny), (torch.complex64, self.nx * self.ny)
+        half = self.nx // 2 + 1
+        if self.type == FFT_TYPE_R2C:
+            return (torch.float32, self.nx * self.batch), (torch.complex64, half * self.batch)
+        if self.type == FFT_TYPE_C2R:
+            return (torch.complex64, half * self.batch), (torch.float32, self.nx * self.batch)
+        return (torch.complex64, self.nx * self.batch), (torch.complex64, self.nx * self.batch)
 
     def execute(self, x, out=None, stream=None):
-        if x.dtype != torch.complex64 or not x.is_cuda or x.numel() != self.nx * self.batch:
-            raise ValueError("input must be complex64 [batch, nx] on the device")
+        (din, nin), (dout, nout) = self._shapes()
+        if x.dtype != din or not x.is_cuda or x.numel() != nin:
+            raise ValueError(f"input must be {din} with {nin} elements on the device")
         x = x.contiguous()
-        out = torch.empty_like(x) if out is None else out
+        out = torch.empty(nout, dtype=dout, device=x.device) if out is None else out
         s = stream if stream is not None else torch.cuda.current_stream()
         _check(_L().fft_execute_on_stream(self.handle, out.data_ptr(), x.data_ptr(), s.cuda_stream))
         return out
@@ -86,11 +107,41 @@ class Plan:
 
 
 def fft(x, inverse=False):
-    """Batched C2C transform along the last axis (power-of-two length)."""
+    """Batched C2C transform along the last axis (any length the backend takes)."""
     n = x.shape[-1]
     p = Plan(n, x.numel() // n, FFT_INVERSE if inverse else FFT_FORWARD)
     try:
         return p.execute(x).reshape(x.shape)
+    finally:
+        p.destroy()
+
+
+def rfft(x):
+    """R2C along the last axis of float32 rows: n/2+1 bins."""
+    n = x.shape[-1]
+    p = Plan(n, x.numel() // n, FFT_FORWARD, FFT_TYPE_R2C)
+    try:
+        return p.execute(x).reshape(tuple(x.shape[:-1]) + (n // 2 + 1,))
+    finally:
+        p.destroy()
+
+
+def irfft(X, n):
+    """C2R of n/2+1 bins per row into n real samples (reference semantics)."""
+    rows = X.numel() // (n // 2 + 1)
+    p = Plan(n, rows, FFT_INVERSE, FFT_TYPE_C2R)
+    try:
+        return p.execute(X).reshape(tuple(X.shape[:-1]) + (n,))
+    finally:
+        p.destroy()
+
+
+def fft2(x, inverse=False):
+    """2-D C2C transform of a complex64 [ny, nx] array."""
+    ny, nx = x.shape
+    p = Plan(nx, 1, FFT_INVERSE if inverse else FFT_FORWARD, ny=ny)
+    try:
+        return p.execute(x).reshape(ny, nx)
     finally:
         p.destroy()
 
