@@ -162,3 +162,50 @@ def test_fftconvolve(F, n1, n2, batch, mode):
                            for i in range(batch)])
         if mode == "full" or n1 >= n2:  # numpy's same/valid follow the longer input
             assert orc.rel_l2(y, direct) < TOL
+
+
+@pytest.mark.parametrize("kind,K,down,shape,cplx", [
+    ("upfirdn", 255, 4, (4, 60000), True), ("upfirdn", 200, 3, (2, 30001), True),
+    ("upfirdn", 100, 1, (3, 20000), False), ("upfirdn", 255, 4, (2, 50000), False),
+    ("firfilter", 255, 1, (2, 100000), False), ("firfilter", 1000, 1, (1, 70000), False)])
+def test_frequency_domain_fir_matches_direct_and_oracle(F, kind, K, down, shape, cplx):
+    """Long up = 1 filters on contiguous lines run as frequency-domain
+    overlap-save; the direct polyphase kernels (variant off) and the oracle
+    agree with it."""
+    from zkx_dev_amd._lib import lib
+    rng = np.random.default_rng(K + down)
+    h = (rng.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+    x = rng.uniform(-1, 1, shape).astype(np.float32)
+    if cplx:
+        x = (x + 1j * rng.uniform(-1, 1, shape)).astype(np.complex64)
+
+    def run():
+        if kind == "firfilter":
+            return F.firfilter(dev(h), dev(x), 1).cpu().numpy()
+        return F.upfirdn(dev(h), dev(x), 1, down, 1).cpu().numpy()
+    assert lib().zkx_get_kernel_variant(b"fir_fft") == 1
+    y_fft = run()
+    assert lib().zkx_set_kernel_variant(b"fir_fft", 0) == 0
+    try:
+        y_dir = run()
+    finally:
+        lib().zkx_set_kernel_variant(b"fir_fft", 1)
+    assert orc.rel_l2(y_fft, y_dir) < 2e-6
+    if kind == "firfilter":
+        ref, _ = orc.firfilter(h, x, 1)
+    else:
+        f = lambda v: orc.upfirdn(h, v, 1, down, 1, fp64_acc=True)  # noqa: E731
+        ref = per_component(f, x) if cplx else f(x)
+    assert orc.rel_l2(y_fft, ref) < TOL
+
+
+def test_frequency_domain_firfilter_zi_zf(F):
+    rng = np.random.default_rng(5)
+    K, n = 255, 40000
+    b = (rng.standard_normal(K) / 16).astype(np.float32)
+    x = rng.uniform(-1, 1, (2, n)).astype(np.float32)
+    zi = rng.standard_normal((2, K - 1)).astype(np.float32)
+    y, zf = F.firfilter(dev(b), dev(x), 1, zi=dev(zi))
+    ref_y, ref_zf = orc.firfilter(b, x, 1, zi=zi)
+    assert orc.rel_l2(y.cpu().numpy(), ref_y) < TOL
+    assert orc.rel_l2(zf.cpu().numpy(), ref_zf) < TOL

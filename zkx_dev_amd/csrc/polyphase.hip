@@ -28,6 +28,15 @@
 #include <numeric>
 
 namespace zkx {
+
+template <class V>
+int fir_fft_run(const float* h, int K, const V* x, int lines, long long n, int D, long long out_n, V* y,
+                hipStream_t st, bool* done);
+
+// 1: long up = 1 filters on contiguous lines take the frequency-domain path
+// (fir_fft.hip); 0: always the direct polyphase kernels.
+int g_fir_fft = 1;
+
 namespace {
 
 template <class V>
@@ -327,6 +336,11 @@ int upfirdn_impl(const float* h, int taps, const V* x, int outer, int n, int inn
     if (out_n < 0 || out_n > std::numeric_limits<int>::max() ||
         static_cast<long long>(outer) * inner * out_n > std::numeric_limits<int>::max())
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "upfirdn output sample count");
+    if (g_fir_fft && inner == 1 && up == 1 && n > 0) {
+        bool done = false;
+        rc = fir_fft_run<V>(h, taps, x, outer, n, down, out_n, out, st, &done);
+        if (rc || done) return rc;
+    }
     return run_polyphase<V>(h, x, out, make_plan(up, down, 0, taps, n, out_n), outer, inner, st);
 }
 
@@ -370,8 +384,14 @@ int firfilter_impl(const float* b, int taps, const float* x, int outer, int n, i
     if (rc) return rc;
     if (!x || !y) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "firfilter: null pointer");
     const int state = taps - 1;
+    const bool fft_ok = g_fir_fft && inner == 1;
     if (!zi && !zf) {
         // y is the first n outputs of the full convolution
+        if (fft_ok) {
+            bool done = false;
+            rc = fir_fft_run<float>(b, taps, x, outer, n, 1, n, y, st, &done);
+            if (rc || done) return rc;
+        }
         return run_polyphase<float>(b, x, y, make_plan(1, 1, 0, taps, n, n), outer, inner, st);
     }
     // full convolution into scratch, then zi / zf split
@@ -380,8 +400,10 @@ int firfilter_impl(const float* b, int taps, const float* x, int outer, int n, i
     rc = t_fir_ws.reserve(sizeof(float) * static_cast<size_t>(outer) * inner * full_n, &ws);
     if (rc) return rc;
     float* full = static_cast<float*>(ws);
-    rc = run_polyphase<float>(b, x, full, make_plan(1, 1, 0, taps, n, full_n), outer, inner, st);
-    if (rc) return rc;
+    bool done = false;
+    if (fft_ok && (rc = fir_fft_run<float>(b, taps, x, outer, n, 1, full_n, full, st, &done))) return rc;
+    if (!done && (rc = run_polyphase<float>(b, x, full, make_plan(1, 1, 0, taps, n, full_n), outer, inner, st)))
+        return rc;
     const long long total = static_cast<long long>(outer) * inner * full_n;
     hipLaunchKernelGGL((firfilter_state_kernel<float>), dim3(grid_for(total)), dim3(256), 0, st, full, outer,
                        static_cast<long long>(n), inner, state, zi, y, zf);

@@ -1031,6 +1031,7 @@ int validate_cfar2d(int rows, int cols, int g0, int g1, int r0, int r1, double p
 // exported for other translation units of the library
 int doppler_dft(const float2* x, int channels, int P, int R, int nfft, int window, const float* ew, int terms,
                 float2* out, float* power, hipStream_t st);
+extern int g_fir_fft;
 
 int fft_rows_dispatch(int n, const float2* x, long long rows, bool inverse, float scale, float2* y,
                       hipStream_t st)
@@ -1066,6 +1067,11 @@ int zkx_set_kernel_variant(const char* op, int variant)
         g_doppler_variant = variant;
         return ZKX_SUCCESS;
     }
+    if (name == "fir_fft") {
+        if (variant < 0 || variant > 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "fir_fft variant out of range");
+        g_fir_fft = variant;
+        return ZKX_SUCCESS;
+    }
 
     ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unknown kernel variant op");
 }
@@ -1076,6 +1082,7 @@ int zkx_get_kernel_variant(const char* op)
     const std::string name(op);
     if (name == "pulse_compression") return g_pc_variant;
     if (name == "pulse_doppler") return g_doppler_variant;
+    if (name == "fir_fft") return g_fir_fft;
     return -1;
 }
 
