@@ -383,7 +383,7 @@ def main():
     if args.cfar_mode == 1:
         t_dc = timed(lambda: radar.doppler_cfar(pc_out, GUARD, REF, PFA, window="hann", threshold=thr,
                                                 detections=det))
-        stages["doppler_cfar_kernel<128>"] = (t_dc, cells * (8 + 4 + 1))
+        stages["doppler_cfar_kernel<128, 2, 2, 8, 8>"] = (t_dc, cells * (8 + 4 + 1))
     # reference-identical CFAR pipeline beside it
     pipe0 = radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
                                 guard=GUARD, reference=REF, pfa=PFA, cfar_mode=0, max_batch=B)

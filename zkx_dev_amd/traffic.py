@@ -26,7 +26,7 @@ from collections import defaultdict
 BATCH, P, R = 64, 128, 4096
 COPY_BYTES = 256 * 1024 * 1024
 KERNELS = {"pc_fused_kernel<4096>": "pc_fused_kernel<4096>",
-           "doppler_cfar_kernel<128>": "doppler_cfar_kernel<128>"}
+           "doppler_cfar_kernel<128, 2, 2, 8, 8>": "doppler_cfar_kernel<128, 2, 2, 8, 8>"}
 
 
 def probe():
