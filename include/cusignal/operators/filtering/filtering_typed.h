@@ -1,12 +1,20 @@
 // The reference's polyphase filtering operator API
 // (ZKX/cusignal_cpp/include/cusignal/operators/filtering/filtering_typed.h)
-// on libzkx_radar.so for float inputs: upfirdn_device, resample_poly_device,
-// firfilter_device and firfilter_fp32_resident_device with the reference's
-// option/result structs, shape/axis rules, output dtypes and exceptions.
+// on libzkx_radar.so: upfirdn_device, resample_poly_device, firfilter_device
+// and firfilter_fp32_resident_device with the reference's option/result
+// structs, shape/axis rules, output dtypes and exceptions, for the element
+// types it instantiates (float, __half, int32, int16, int8;
+// filtering_typed.cu:683+).  Non-float inputs are loaded to float on the
+// device as the reference does (convert_signal_kernel,
+// filtering_kernels.cuh:1277-1286) and filtered by the same FP32 kernels.
+// Extensions: TypedComplex<float> signals (real taps act on each component)
+// for the three filters, and fftconvolve_device.
 // Filtering runs in the gfx950 kernels behind the C ABI; the host only plans
 // lines, designs the default resample filter (FP64, as the reference does on
 // its host) and broadcasts zi.
 #pragma once
+
+#include <hip/hip_fp16.h>
 
 #include <cmath>
 #include <cstdint>
@@ -52,12 +60,14 @@ struct ResamplePolyOptions {
     bool gpupath = true;
 };
 
-// The FP16 / integer members of the reference's result exist for its other
-// input dtypes; the float path fills fp32 or fp64 only.
 struct ResamplePolyDeviceResult {
     ResamplePolyOutputDtype dtype{ResamplePolyOutputDtype::fp64};
+    DeviceArray<__half> fp16;
     DeviceArray<float> fp32;
     DeviceArray<double> fp64;
+    DeviceArray<std::int32_t> int32;
+    DeviceArray<std::int16_t> int16;
+    DeviceArray<std::int8_t> int8;
     std::vector<int> shape;
 };
 
@@ -75,7 +85,93 @@ struct FirfilterDeviceResult {
     bool has_zf = false;
 };
 
+// Results of the complex extension: the same fields with complex storage.
+struct UpfirdnComplexDeviceResult {
+    UpfirdnOutputDtype dtype{UpfirdnOutputDtype::fp32};
+    DeviceArray<ComplexFloat> fp32;
+    DeviceArray<ComplexDouble> fp64;
+    std::vector<int> shape;
+};
+
+struct ResamplePolyComplexDeviceResult {
+    ResamplePolyOutputDtype dtype{ResamplePolyOutputDtype::fp64};
+    DeviceArray<ComplexFloat> fp32;
+    DeviceArray<ComplexDouble> fp64;
+    std::vector<int> shape;
+};
+
+struct FirfilterComplexDeviceResult {
+    DeviceArray<ComplexFloat> y;
+    DeviceArray<ComplexFloat> zf;
+    std::vector<int> y_shape;
+    std::vector<int> zf_shape;
+    bool has_zf = false;
+};
+
+// fftconvolve (extension; cuSignal convolve.py:159-320): every row of in1
+// (the last axis of `shape`, or a 1-D signal when shape is empty) convolved
+// with the 1-D in2; modes as cuSignal/scipy.
+enum class FftconvolveMode { full, same, valid };
+
+struct FftconvolveOptions {
+    std::vector<int> shape;
+    FftconvolveMode mode = FftconvolveMode::full;
+};
+
+struct FftconvolveDeviceResult {
+    DeviceArray<ComplexFloat> out;
+    std::vector<int> shape;
+};
+
 namespace detail {
+
+// The element types the reference instantiates, as the C ABI's codes.
+template <class T>
+constexpr int dtype_code()
+{
+    if constexpr (std::is_same_v<T, float>) return ZKX_DTYPE_F32;
+    else if constexpr (std::is_same_v<T, __half>) return ZKX_DTYPE_F16;
+    else if constexpr (std::is_same_v<T, std::int32_t>) return ZKX_DTYPE_I32;
+    else if constexpr (std::is_same_v<T, std::int16_t>) return ZKX_DTYPE_I16;
+    else {
+        static_assert(std::is_same_v<T, std::int8_t>, "element type must be float, __half, int32, int16 or int8");
+        return ZKX_DTYPE_I8;
+    }
+}
+
+// Float samples of `scalars` device values of type T: the values themselves
+// for float, a converted device copy otherwise.
+class FloatSamples {
+public:
+    template <class T>
+    FloatSamples(const T* data, std::size_t scalars)
+    {
+        if constexpr (std::is_same_v<T, float>) {
+            ptr_ = data;
+        } else {
+            (void)dtype_code<T>();
+            owned_ = DeviceArray<float>(scalars);
+            if (scalars) check_abi(zkx_convert_to_f32(data, dtype_code<T>(), scalars, owned_.data(), nullptr));
+            ptr_ = owned_.data();
+        }
+    }
+    const float* data() const { return ptr_; }
+
+private:
+    const float* ptr_ = nullptr;
+    DeviceArray<float> owned_;
+};
+
+template <class T>
+FloatSamples as_float(const DeviceArray<T>& a)
+{
+    return FloatSamples(a.data(), a.size());
+}
+template <class T>
+FloatSamples as_float(const DeviceArray<TypedComplex<T>>& a)
+{
+    return FloatSamples(reinterpret_cast<const T*>(a.data()), 2 * a.size());
+}
 
 struct Lines {
     int outer = 1, n = 1, inner = 1, axis = 0;
@@ -109,6 +205,22 @@ inline DeviceArray<double> widen(const DeviceArray<float>& v)
     DeviceArray<double> out(v.size());
     if (!v.empty()) check_abi(zkx_widen_f32_to_f64(v.data(), out.data(), v.size(), nullptr));
     return out;
+}
+
+inline DeviceArray<ComplexDouble> widen(const DeviceArray<ComplexFloat>& v)
+{
+    DeviceArray<ComplexDouble> out(v.size());
+    if (!v.empty())
+        check_abi(zkx_widen_c64_to_c128(reinterpret_cast<const zkx_complex64*>(v.data()),
+                                        reinterpret_cast<zkx_complex128*>(out.data()), v.size(), nullptr));
+    return out;
+}
+
+template <class E>
+void copy_device(E* dst, const E* src, std::size_t count)
+{
+    if (count)
+        cuda_utils::check_hip(hipMemcpy(dst, src, sizeof(E) * count, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
 }
 
 // Abramowitz & Stegun 9.8.1 / 9.8.2 polynomial I0, the approximation of the
@@ -160,139 +272,261 @@ inline std::vector<double> design_resample_poly(int up, int down, const Resample
     return h;
 }
 
+// resample_poly_output_dtype (filtering_typed.cpp:62-101)
+template <typename T>
+ResamplePolyOutputDtype resample_poly_dtype(bool identity, bool custom, int reduced_up)
+{
+    if (identity) {
+        if constexpr (std::is_same_v<T, float>) return ResamplePolyOutputDtype::fp32;
+        else if constexpr (std::is_same_v<T, __half>) return ResamplePolyOutputDtype::fp16;
+        else if constexpr (std::is_same_v<T, std::int32_t>) return ResamplePolyOutputDtype::int32;
+        else if constexpr (std::is_same_v<T, std::int16_t>) return ResamplePolyOutputDtype::int16;
+        else return ResamplePolyOutputDtype::int8;
+    }
+    if (!custom) return ResamplePolyOutputDtype::fp64;
+    if constexpr (std::is_same_v<T, std::int32_t>) return ResamplePolyOutputDtype::fp64;
+    if constexpr (std::is_same_v<T, std::int16_t> || std::is_same_v<T, std::int8_t>)
+        return reduced_up < 32768 ? ResamplePolyOutputDtype::fp32 : ResamplePolyOutputDtype::fp64;
+    return reduced_up < 65536 ? ResamplePolyOutputDtype::fp32 : ResamplePolyOutputDtype::fp64;
+}
+
+// Taps of one resample_poly call as float on the device: custom coefficients
+// scaled by the reduced up in their own type, or the FP64 design rounded.
+template <class T>
+DeviceArray<float> resample_poly_taps(int u, int d, const ResamplePolyOptions& options, const DeviceArray<T>* coefficients)
+{
+    if (coefficients) {
+        if (coefficients->empty()) throw std::invalid_argument("resample_poly coefficients must be non-empty");
+        DeviceArray<float> taps(coefficients->size());
+        check_abi(zkx_resample_poly_scale_taps(coefficients->data(), dtype_code<T>(),
+                                               static_cast<int>(coefficients->size()), u, taps.data(), nullptr));
+        return taps;
+    }
+    const std::vector<double> h = design_resample_poly(u, d, options);
+    return DeviceArray<float>::from_host(std::vector<float>(h.begin(), h.end()));
+}
+
+// zi broadcast to the state shape (shape with axis length K-1); the given
+// zi_shape may hold 1 on any non-axis dimension.  E: the element type.
+template <class E>
+DeviceArray<E> broadcast_zi(const DeviceArray<E>& zi, const std::vector<int>& zi_shape, const std::vector<int>& state,
+                            int axis)
+{
+    const std::vector<int> zs = zi_shape.empty() ? std::vector<int>{static_cast<int>(zi.size())} : zi_shape;
+    if (zs.size() != state.size() && !(zs.size() == 1 && state.size() == 1))
+        throw std::invalid_argument("firfilter zi rank must match x");
+    long long zcount = 1;
+    std::size_t state_count = 1;
+    for (std::size_t i = 0; i < state.size(); ++i) state_count *= static_cast<std::size_t>(state[i]);
+    for (std::size_t i = 0; i < zs.size(); ++i) {
+        if (zs[i] != state[i] && !(zs[i] == 1 && static_cast<int>(i) != axis))
+            throw std::invalid_argument("firfilter zi shape cannot broadcast to the state shape");
+        zcount *= zs[i];
+    }
+    if (static_cast<std::size_t>(zcount) != zi.size()) throw std::invalid_argument("firfilter zi size mismatch");
+    DeviceArray<E> full(state_count);
+    if (zi.size() == state_count) {
+        copy_device(full.data(), zi.data(), state_count);
+        return full;
+    }
+    const std::vector<E> zh = zi.to_host();
+    std::vector<E> host(state_count);
+    std::vector<int> idx(state.size(), 0);
+    for (std::size_t e = 0; e < state_count; ++e) {
+        std::size_t src = 0;
+        for (std::size_t dd = 0; dd < state.size(); ++dd) src = src * zs[dd] + (zs[dd] == 1 ? 0 : idx[dd]);
+        host[e] = zh[src];
+        for (int dd = static_cast<int>(state.size()) - 1; dd >= 0; --dd) {
+            if (++idx[dd] < state[dd]) break;
+            idx[dd] = 0;
+        }
+    }
+    return DeviceArray<E>::from_host(host);
+}
+
 }  // namespace detail
 
-// filtering_typed.h:744-748 (float input: FP32 result)
+// filtering_typed.h:744-748: FP32 result, FP64 storage for int32 input
+// (filtering_typed.cpp:77-83, filtering_typed.cu:596-645).
 template <typename T>
 void upfirdn_device(const DeviceArray<T>& h, const DeviceArray<T>& x, int up, int down, UpfirdnDeviceResult& out,
                     const UpfirdnOptions& options = {})
 {
-    static_assert(std::is_same_v<T, float>, "libzkx_radar implements the float upfirdn path");
+    if (h.empty()) throw std::invalid_argument("upfirdn coefficients must be non-empty");
+    if (up < 1 || down < 1) throw std::invalid_argument("upfirdn factors must be positive");
+    if (x.empty()) throw std::invalid_argument("upfirdn input must be non-empty");
+    const detail::Lines l = detail::plan_lines(options.shape, options.axis, x.size(), "upfirdn");
+    if (l.shape.size() > 2) throw std::invalid_argument("upfirdn supports rank 1 or 2");
+    const detail::FloatSamples hf = detail::as_float(h), xf = detail::as_float(x);
+    const long long out_n = zkx_upfirdn_out_len(l.n, static_cast<long long>(h.size()), up, down);
+    out = UpfirdnDeviceResult{};
+    out.shape = l.shape;
+    out.shape[l.axis] = static_cast<int>(out_n);
+    DeviceArray<float> y(static_cast<std::size_t>(l.outer) * out_n * l.inner);
+    detail::check_abi(zkx_upfirdn_f32(hf.data(), static_cast<int>(h.size()), xf.data(), l.outer, l.n, l.inner, up,
+                                      down, y.data(), nullptr));
+    if constexpr (std::is_same_v<T, std::int32_t>) {
+        out.dtype = UpfirdnOutputDtype::fp64;
+        out.fp64 = detail::widen(y);
+    } else {
+        out.dtype = UpfirdnOutputDtype::fp32;
+        out.fp32 = std::move(y);
+    }
+}
+
+// Complex extension: real float taps on a TypedComplex<float> signal.
+template <typename T>
+void upfirdn_device(const DeviceArray<T>& h, const DeviceArray<TypedComplex<T>>& x, int up, int down,
+                    UpfirdnComplexDeviceResult& out, const UpfirdnOptions& options = {})
+{
+    static_assert(std::is_same_v<T, float>, "complex upfirdn takes float taps and TypedComplex<float> samples");
     if (h.empty()) throw std::invalid_argument("upfirdn coefficients must be non-empty");
     if (up < 1 || down < 1) throw std::invalid_argument("upfirdn factors must be positive");
     if (x.empty()) throw std::invalid_argument("upfirdn input must be non-empty");
     const detail::Lines l = detail::plan_lines(options.shape, options.axis, x.size(), "upfirdn");
     if (l.shape.size() > 2) throw std::invalid_argument("upfirdn supports rank 1 or 2");
     const long long out_n = zkx_upfirdn_out_len(l.n, static_cast<long long>(h.size()), up, down);
+    out = UpfirdnComplexDeviceResult{};
     out.shape = l.shape;
     out.shape[l.axis] = static_cast<int>(out_n);
     out.dtype = UpfirdnOutputDtype::fp32;
-    out.fp64.reset();
-    out.fp32 = DeviceArray<float>(static_cast<std::size_t>(l.outer) * out_n * l.inner);
-    detail::check_abi(zkx_upfirdn_f32(h.data(), static_cast<int>(h.size()), x.data(), l.outer, l.n, l.inner, up, down,
-                                      out.fp32.data(), nullptr));
+    out.fp32 = DeviceArray<ComplexFloat>(static_cast<std::size_t>(l.outer) * out_n * l.inner);
+    detail::check_abi(zkx_upfirdn_c64(h.data(), static_cast<int>(h.size()),
+                                      reinterpret_cast<const zkx_complex64*>(x.data()), l.outer, l.n, l.inner, up,
+                                      down, reinterpret_cast<zkx_complex64*>(out.fp32.data()), nullptr));
 }
 
-// filtering_typed.h:765-770.  Default design: FP64 result.  Float custom
-// coefficients: scaled by the reduced up in float, FP32 result.
+// filtering_typed.h:765-770 (filtering_typed.cpp:2717-2779): identity copies
+// in the input's own type; otherwise FP32 filtering with the dtype of
+// resample_poly_output_dtype (default design: FP64 storage).
 template <typename T>
 void resample_poly_device(const DeviceArray<T>& x, int up, int down, ResamplePolyDeviceResult& out,
                           const ResamplePolyOptions& options = {}, const DeviceArray<T>* coefficients = nullptr)
 {
-    static_assert(std::is_same_v<T, float>, "libzkx_radar implements the float resample_poly path");
+    (void)detail::dtype_code<T>();
     if (up < 1 || down < 1) throw std::invalid_argument("resample_poly factors must be positive");
     const int g = std::gcd(up, down);
     const int u = up / g, d = down / g;
-    out.fp32.reset();
-    out.fp64.reset();
+    out = ResamplePolyDeviceResult{};
+    out.dtype = detail::resample_poly_dtype<T>(u == 1 && d == 1, coefficients != nullptr, u);
     if (u == 1 && d == 1) {
         out.shape = options.shape.empty() ? std::vector<int>{static_cast<int>(x.size())} : options.shape;
-        out.dtype = ResamplePolyOutputDtype::fp32;
-        out.fp32 = DeviceArray<float>(x.size());
-        if (!x.empty())
-            cuda_utils::check_hip(hipMemcpy(out.fp32.data(), x.data(), sizeof(float) * x.size(), hipMemcpyDeviceToDevice),
-                                  "hipMemcpy D2D");
+        auto copy_into = [&](auto& member) {
+            member.reset(x.size());
+            detail::copy_device(member.data(), x.data(), x.size());
+        };
+        if constexpr (std::is_same_v<T, float>) copy_into(out.fp32);
+        else if constexpr (std::is_same_v<T, __half>) copy_into(out.fp16);
+        else if constexpr (std::is_same_v<T, std::int32_t>) copy_into(out.int32);
+        else if constexpr (std::is_same_v<T, std::int16_t>) copy_into(out.int16);
+        else copy_into(out.int8);
         return;
     }
     const detail::Lines l = detail::plan_lines(options.shape, options.axis, x.size(), "resample_poly");
     if (l.shape.size() > 2) throw std::invalid_argument("resample_poly supports rank 1 or 2");
-    std::vector<float> taps;
-    bool fp64_result = true;
-    if (coefficients) {
-        if (coefficients->empty()) throw std::invalid_argument("resample_poly coefficients must be non-empty");
-        const std::vector<float> c = coefficients->to_host();
-        taps.resize(c.size());
-        for (std::size_t i = 0; i < c.size(); ++i) taps[i] = c[i] * static_cast<float>(u);
-        fp64_result = false;
-    } else {
-        const std::vector<double> h = detail::design_resample_poly(u, d, options);
-        taps.assign(h.begin(), h.end());
-    }
-    const DeviceArray<float> taps_d = DeviceArray<float>::from_host(taps);
+    const DeviceArray<float> taps = detail::resample_poly_taps(u, d, options, coefficients);
+    const detail::FloatSamples xf = detail::as_float(x);
     const long long out_n = (static_cast<long long>(l.n) * u + d - 1) / d;
     out.shape = l.shape;
     out.shape[l.axis] = static_cast<int>(out_n);
     DeviceArray<float> y(static_cast<std::size_t>(l.outer) * out_n * l.inner);
-    detail::check_abi(zkx_resample_poly_f32(taps_d.data(), static_cast<int>(taps.size()), x.data(), l.outer, l.n,
+    detail::check_abi(zkx_resample_poly_f32(taps.data(), static_cast<int>(taps.size()), xf.data(), l.outer, l.n,
                                             l.inner, up, down, y.data(), nullptr));
-    if (fp64_result) {
-        out.dtype = ResamplePolyOutputDtype::fp64;
-        out.fp64 = detail::widen(y);
-    } else {
-        out.dtype = ResamplePolyOutputDtype::fp32;
-        out.fp32 = std::move(y);
-    }
+    if (out.dtype == ResamplePolyOutputDtype::fp32) out.fp32 = std::move(y);
+    else out.fp64 = detail::widen(y);
 }
 
-// filtering_typed.h:552-556
+// Complex extension: float taps (custom or designed) on TypedComplex<float>.
+template <typename T>
+void resample_poly_device(const DeviceArray<TypedComplex<T>>& x, int up, int down,
+                          ResamplePolyComplexDeviceResult& out, const ResamplePolyOptions& options = {},
+                          const DeviceArray<T>* coefficients = nullptr)
+{
+    static_assert(std::is_same_v<T, float>, "complex resample_poly takes TypedComplex<float> samples");
+    if (up < 1 || down < 1) throw std::invalid_argument("resample_poly factors must be positive");
+    const int g = std::gcd(up, down);
+    const int u = up / g, d = down / g;
+    out = ResamplePolyComplexDeviceResult{};
+    out.dtype = detail::resample_poly_dtype<T>(u == 1 && d == 1, coefficients != nullptr, u);
+    if (u == 1 && d == 1) {
+        out.shape = options.shape.empty() ? std::vector<int>{static_cast<int>(x.size())} : options.shape;
+        out.fp32.reset(x.size());
+        detail::copy_device(reinterpret_cast<TypedComplex<T>*>(out.fp32.data()), x.data(), x.size());
+        return;
+    }
+    const detail::Lines l = detail::plan_lines(options.shape, options.axis, x.size(), "resample_poly");
+    if (l.shape.size() > 2) throw std::invalid_argument("resample_poly supports rank 1 or 2");
+    const DeviceArray<float> taps = detail::resample_poly_taps(u, d, options, coefficients);
+    const long long out_n = (static_cast<long long>(l.n) * u + d - 1) / d;
+    out.shape = l.shape;
+    out.shape[l.axis] = static_cast<int>(out_n);
+    DeviceArray<ComplexFloat> y(static_cast<std::size_t>(l.outer) * out_n * l.inner);
+    detail::check_abi(zkx_resample_poly_c64(taps.data(), static_cast<int>(taps.size()),
+                                            reinterpret_cast<const zkx_complex64*>(x.data()), l.outer, l.n, l.inner,
+                                            up, down, reinterpret_cast<zkx_complex64*>(y.data()), nullptr));
+    if (out.dtype == ResamplePolyOutputDtype::fp32) out.fp32 = std::move(y);
+    else out.fp64 = detail::widen(y);
+}
+
+// filtering_typed.h:552-556 (filtering_typed.cu:117-164): b, x and zi loaded
+// to float; y and zf are float.
 template <typename T>
 void firfilter_device(const DeviceArray<T>& b, const DeviceArray<T>& x, FirfilterDeviceResult& out,
                       const FirfilterOptions& options = {}, const DeviceArray<T>* zi = nullptr)
 {
-    static_assert(std::is_same_v<T, float>, "libzkx_radar implements the float firfilter path");
     if (b.empty()) throw std::invalid_argument("firfilter coefficients must be non-empty");
     if (x.empty()) throw std::invalid_argument("firfilter input must be non-empty");
     const detail::Lines l = detail::plan_lines(options.shape, options.axis, x.size(), "firfilter");
     const int k1 = static_cast<int>(b.size()) - 1;
+    out = FirfilterDeviceResult{};
     out.y_shape = l.shape;
     out.y = DeviceArray<float>(x.size());
     out.has_zf = zi != nullptr;
-    out.zf.reset();
-    out.zf_shape.clear();
-    DeviceArray<float> zi_full;
+    DeviceArray<T> zi_full;
     if (zi) {
-        // zi broadcasts to the state shape (shape with axis length K-1); the
-        // given zi_shape may hold 1 on any non-axis dimension.
         std::vector<int> state = l.shape;
         state[l.axis] = k1;
-        const std::vector<int> zs = options.zi_shape.empty() ? std::vector<int>{static_cast<int>(zi->size())}
-                                                             : options.zi_shape;
-        if (zs.size() != state.size() && !(zs.size() == 1 && state.size() == 1))
-            throw std::invalid_argument("firfilter zi rank must match x");
-        long long zcount = 1;
-        for (std::size_t i = 0; i < zs.size(); ++i) {
-            if (zs[i] != state[i] && !(zs[i] == 1 && static_cast<int>(i) != l.axis))
-                throw std::invalid_argument("firfilter zi shape cannot broadcast to the state shape");
-            zcount *= zs[i];
-        }
-        if (static_cast<std::size_t>(zcount) != zi->size()) throw std::invalid_argument("firfilter zi size mismatch");
-        const std::size_t state_count = static_cast<std::size_t>(l.outer) * k1 * l.inner;
-        if (zi->size() == state_count) {
-            zi_full = DeviceArray<float>(state_count);
-            if (state_count)
-                cuda_utils::check_hip(hipMemcpy(zi_full.data(), zi->data(), sizeof(float) * state_count,
-                                                hipMemcpyDeviceToDevice), "hipMemcpy D2D");
-        } else {
-            const std::vector<float> zh = zi->to_host();
-            std::vector<float> full(state_count);
-            std::vector<int> idx(state.size(), 0);
-            for (std::size_t e = 0; e < state_count; ++e) {
-                std::size_t src = 0;
-                for (std::size_t dd = 0; dd < state.size(); ++dd) src = src * zs[dd] + (zs[dd] == 1 ? 0 : idx[dd]);
-                full[e] = zh[src];
-                for (int dd = static_cast<int>(state.size()) - 1; dd >= 0; --dd) {
-                    if (++idx[dd] < state[dd]) break;
-                    idx[dd] = 0;
-                }
-            }
-            zi_full = DeviceArray<float>::from_host(full);
-        }
+        zi_full = detail::broadcast_zi(*zi, options.zi_shape, state, l.axis);
         out.zf_shape = state;
-        out.zf = DeviceArray<float>(state_count);
+        out.zf = DeviceArray<float>(zi_full.size());
     }
-    detail::check_abi(zkx_firfilter_f32(b.data(), static_cast<int>(b.size()), x.data(), l.outer, l.n, l.inner,
-                                        zi ? zi_full.data() : nullptr, out.y.data(), zi ? out.zf.data() : nullptr,
+    const detail::FloatSamples bf = detail::as_float(b), xf = detail::as_float(x);
+    const detail::FloatSamples zf_in = detail::as_float(zi_full);
+    detail::check_abi(zkx_firfilter_f32(bf.data(), static_cast<int>(b.size()), xf.data(), l.outer, l.n, l.inner,
+                                        zi ? zf_in.data() : nullptr, out.y.data(), zi ? out.zf.data() : nullptr,
                                         nullptr));
+}
+
+// Complex extension: real float taps on TypedComplex<float> samples (and
+// complex zi); each component is filtered as a float line.
+template <typename T>
+void firfilter_device(const DeviceArray<T>& b, const DeviceArray<TypedComplex<T>>& x,
+                      FirfilterComplexDeviceResult& out, const FirfilterOptions& options = {},
+                      const DeviceArray<TypedComplex<T>>* zi = nullptr)
+{
+    static_assert(std::is_same_v<T, float>, "complex firfilter takes float taps and TypedComplex<float> samples");
+    if (b.empty()) throw std::invalid_argument("firfilter coefficients must be non-empty");
+    if (x.empty()) throw std::invalid_argument("firfilter input must be non-empty");
+    const detail::Lines l = detail::plan_lines(options.shape, options.axis, x.size(), "firfilter");
+    const int k1 = static_cast<int>(b.size()) - 1;
+    out = FirfilterComplexDeviceResult{};
+    out.y_shape = l.shape;
+    out.y = DeviceArray<ComplexFloat>(x.size());
+    out.has_zf = zi != nullptr;
+    DeviceArray<TypedComplex<T>> zi_full;
+    if (zi) {
+        std::vector<int> state = l.shape;
+        state[l.axis] = k1;
+        zi_full = detail::broadcast_zi(*zi, options.zi_shape, state, l.axis);
+        out.zf_shape = state;
+        out.zf = DeviceArray<ComplexFloat>(zi_full.size());
+    }
+    // the interleaved (re, im) pairs are float lines with twice the inner count
+    detail::check_abi(zkx_firfilter_f32(b.data(), static_cast<int>(b.size()), reinterpret_cast<const float*>(x.data()),
+                                        l.outer, l.n, 2 * l.inner,
+                                        zi ? reinterpret_cast<const float*>(zi_full.data()) : nullptr,
+                                        reinterpret_cast<float*>(out.y.data()),
+                                        zi ? reinterpret_cast<float*>(out.zf.data()) : nullptr, nullptr));
 }
 
 // filtering_typed.h:563-565: 1-D, no state, caller-owned output, no allocation.
@@ -303,6 +537,29 @@ inline void firfilter_fp32_resident_device(const DeviceArray<float>& b, const De
     if (y.size() != x.size()) throw std::invalid_argument("firfilter resident output must match input size");
     detail::check_abi(zkx_firfilter_f32(b.data(), static_cast<int>(b.size()), x.data(), 1, static_cast<int>(x.size()),
                                         1, nullptr, y.data(), nullptr, nullptr));
+}
+
+// fftconvolve extension on complex signals of any instantiated component
+// type (loaded to float): out has the row count of in1 and the mode's length.
+template <typename T>
+void fftconvolve_device(const DeviceArray<TypedComplex<T>>& in1, const DeviceArray<TypedComplex<T>>& in2,
+                        FftconvolveDeviceResult& out, const FftconvolveOptions& options = {})
+{
+    if (in1.empty() || in2.empty()) throw std::invalid_argument("fftconvolve inputs must be non-empty");
+    const detail::Lines l = detail::plan_lines(options.shape, -1, in1.size(), "fftconvolve");
+    const int mode = static_cast<int>(options.mode);
+    const long long out_n = zkx_fftconvolve_out_len(l.n, static_cast<long long>(in2.size()), mode);
+    if (out_n < 0) throw std::invalid_argument("fftconvolve mode is invalid for these lengths");
+    const detail::FloatSamples a = detail::as_float(in1), b = detail::as_float(in2);
+    out = FftconvolveDeviceResult{};
+    out.shape = l.shape;
+    out.shape.back() = static_cast<int>(out_n);
+    out.out = DeviceArray<ComplexFloat>(static_cast<std::size_t>(l.outer) * out_n);
+    if (out_n == 0) return;
+    detail::check_abi(zkx_fftconvolve_c64(reinterpret_cast<const zkx_complex64*>(a.data()), l.outer, l.n,
+                                          reinterpret_cast<const zkx_complex64*>(b.data()),
+                                          static_cast<int>(in2.size()), mode,
+                                          reinterpret_cast<zkx_complex64*>(out.out.data()), nullptr));
 }
 
 }  // namespace cusignal

@@ -1,13 +1,18 @@
 // The reference's radar operator API (ZKX/cusignal_cpp/include/cusignal/
 // operators/radartools/radartools_typed.h) on libzkx_radar.so: same option and
 // result structs, same function names and overloads, same dtype rules and
-// exceptions, for float element types (TypedComplex<float> / float), which is
-// the radar hot path.  The computation is the gfx950 HIP library behind the C
-// ABI; nothing here computes on the host except cfar_alpha's scalar.
-// The reference's *_typed_cpu functions are its CPU reference path and are not
-// part of this library.
+// exceptions, for the element types the reference instantiates (float,
+// __half, int32, int16, int8; radartools_typed.cpp:810-811).  Inputs are
+// loaded to float on the device as the reference does
+// (operator_compute_policy.h:42-45,64-67) and run through the same FP32
+// kernels; integer CA-CFAR detections use the integer comparison of
+// radartools_kernels.cuh:23-37.  The computation is the gfx950 HIP library
+// behind the C ABI; nothing here computes on the host except cfar_alpha's
+// scalar.  The reference's *_typed_cpu functions are its CPU reference path;
+// they are not part of this library.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <type_traits>
@@ -56,22 +61,23 @@ struct RadarComplexDeviceResult {
 };
 
 namespace detail {
+// radar_complex_result_is_fp64 (radartools_typed.cpp:271-278)
 template <class T>
-constexpr void require_float()
+constexpr bool radar_result_is_fp64(RadarWindowKind w)
 {
-    static_assert(std::is_same_v<T, float>, "libzkx_radar implements the float (complex64) operator path");
+    if (w == RadarWindowKind::none) return std::is_integral_v<T>;
+    if (w == RadarWindowKind::explicit_fp32) return std::is_same_v<T, std::int32_t>;
+    return true;
 }
-inline bool radar_result_is_fp64(RadarWindowKind w)
+// one host value loaded to float (InputTraits<T>::load_continuous)
+template <class T>
+float host_load(T v)
 {
-    // radartools_typed.cpp:271-278 for float inputs
-    return w != RadarWindowKind::none && w != RadarWindowKind::explicit_fp32;
-}
-inline DeviceArray<ComplexDouble> widen(const DeviceArray<ComplexFloat>& v)
-{
-    DeviceArray<ComplexDouble> out(v.size());
-    check_abi(zkx_widen_c64_to_c128(reinterpret_cast<const zkx_complex64*>(v.data()),
-                                    reinterpret_cast<zkx_complex128*>(out.data()), v.size(), nullptr));
-    return out;
+    (void)dtype_code<T>();
+    if constexpr (std::is_same_v<T, __half>)
+        return __half2float(v);
+    else
+        return static_cast<float>(v);
 }
 }  // namespace detail
 
@@ -82,7 +88,6 @@ RadarComplexDeviceResult pulse_compression_device(const DeviceArray<TypedComplex
                                                   const PulseCompressionOptions& options,
                                                   const DeviceArray<float>* explicit_window = nullptr)
 {
-    detail::require_float<T>();
     const int nfft = options.nfft == -1 ? options.samples_per_pulse : options.nfft;
     if (options.num_pulses < 1 || options.samples_per_pulse < 1 || nfft < 1 ||
         x.size() != static_cast<std::size_t>(options.num_pulses) * options.samples_per_pulse || pulse_template.empty())
@@ -92,15 +97,16 @@ RadarComplexDeviceResult pulse_compression_device(const DeviceArray<TypedComplex
     if (needs_window != (explicit_window != nullptr) ||
         (explicit_window && explicit_window->size() != pulse_template.size()))
         throw std::invalid_argument("pulse_compression window is invalid");
+    const detail::FloatSamples xf = detail::as_float(x), tf = detail::as_float(pulse_template);
     DeviceArray<ComplexFloat> computed(static_cast<std::size_t>(options.num_pulses) * nfft);
     detail::check_abi(zkx_pulse_compression_c64(
-        reinterpret_cast<const zkx_complex64*>(x.data()), options.num_pulses, options.samples_per_pulse,
-        reinterpret_cast<const zkx_complex64*>(pulse_template.data()), static_cast<int>(pulse_template.size()), nfft,
+        reinterpret_cast<const zkx_complex64*>(xf.data()), options.num_pulses, options.samples_per_pulse,
+        reinterpret_cast<const zkx_complex64*>(tf.data()), static_cast<int>(pulse_template.size()), nfft,
         options.normalize ? 1 : 0, static_cast<int>(options.window), explicit_window ? explicit_window->data() : nullptr,
         reinterpret_cast<zkx_complex64*>(computed.data()), nullptr));
     RadarComplexDeviceResult result;
     result.shape = {options.num_pulses, nfft};
-    result.is_fp64 = detail::radar_result_is_fp64(options.window);
+    result.is_fp64 = detail::radar_result_is_fp64<T>(options.window);
     if (result.is_fp64) result.fp64 = detail::widen(computed);
     else result.fp32 = std::move(computed);
     return result;
@@ -119,7 +125,6 @@ void pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x, const PulseDopp
                           DeviceArray<ComplexFloat>& output, DeviceArray<float>& power,
                           const DeviceArray<float>* explicit_window = nullptr)
 {
-    detail::require_float<T>();
     const int nfft = options.nfft == -1 ? options.num_pulses : options.nfft;
     if (options.num_pulses < 1 || options.samples_per_pulse < 1 || nfft < 1 ||
         x.size() != static_cast<std::size_t>(options.num_pulses) * options.samples_per_pulse)
@@ -132,7 +137,8 @@ void pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x, const PulseDopp
     const std::size_t count = static_cast<std::size_t>(nfft) * options.samples_per_pulse;
     if (output.size() != count || power.size() != count)
         throw std::invalid_argument("pulse_doppler preallocated output shape mismatch");
-    detail::check_abi(zkx_pulse_doppler_c64(reinterpret_cast<const zkx_complex64*>(x.data()), options.num_pulses,
+    const detail::FloatSamples xf = detail::as_float(x);
+    detail::check_abi(zkx_pulse_doppler_c64(reinterpret_cast<const zkx_complex64*>(xf.data()), options.num_pulses,
                                             options.samples_per_pulse, nfft, static_cast<int>(options.window),
                                             explicit_window ? explicit_window->data() : nullptr,
                                             reinterpret_cast<zkx_complex64*>(output.data()), power.data(), nullptr));
@@ -142,7 +148,6 @@ template <class T>
 void pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x, const PulseDopplerOptions& options,
                           DeviceArray<ComplexFloat>& output, const DeviceArray<float>* explicit_window)
 {
-    detail::require_float<T>();
     const int nfft = options.nfft == -1 ? options.num_pulses : options.nfft;
     if (options.num_pulses < 1 || options.samples_per_pulse < 1 || nfft < 1 ||
         x.size() != static_cast<std::size_t>(options.num_pulses) * options.samples_per_pulse)
@@ -154,7 +159,8 @@ void pulse_doppler_device(const DeviceArray<TypedComplex<T>>& x, const PulseDopp
         throw std::invalid_argument("pulse_doppler window is invalid");
     if (output.size() != static_cast<std::size_t>(nfft) * options.samples_per_pulse)
         throw std::invalid_argument("pulse_doppler preallocated output shape mismatch");
-    detail::check_abi(zkx_pulse_doppler_c64(reinterpret_cast<const zkx_complex64*>(x.data()), options.num_pulses,
+    const detail::FloatSamples xf = detail::as_float(x);
+    detail::check_abi(zkx_pulse_doppler_c64(reinterpret_cast<const zkx_complex64*>(xf.data()), options.num_pulses,
                                             options.samples_per_pulse, nfft, static_cast<int>(options.window),
                                             explicit_window ? explicit_window->data() : nullptr,
                                             reinterpret_cast<zkx_complex64*>(output.data()), nullptr, nullptr));
@@ -172,26 +178,33 @@ RadarComplexDeviceResult pulse_doppler_device(const DeviceArray<TypedComplex<T>>
     pulse_doppler_device(x, options, computed, explicit_window);
     RadarComplexDeviceResult result;
     result.shape = {nfft, options.samples_per_pulse};
-    result.is_fp64 = detail::radar_result_is_fp64(options.window);
+    result.is_fp64 = detail::radar_result_is_fp64<T>(options.window);
     if (result.is_fp64) result.fp64 = detail::widen(computed);
     else result.fp32 = std::move(computed);
     return result;
 }
 
-// radartools_typed.h:207-222: alpha = N (pfa^(-1/N) - 1) in FP64
+// radartools_typed.h:207-222 (radartools_typed.cpp:592-617): the CPU form
+// in FP64 from the pfa loaded to float; the device form in FP32 on the
+// device, stored as double.
 template <class T>
 double cfar_alpha_typed_cpu(T pfa, int reference)
 {
-    detail::require_float<T>();
     double alpha = 0.0;
-    detail::check_abi(zkx_cfar_alpha(static_cast<double>(pfa), reference, &alpha));
+    detail::check_abi(zkx_cfar_alpha(static_cast<double>(detail::host_load(pfa)), reference, &alpha));
     return alpha;
 }
 
 template <class T>
 DeviceArray<double> cfar_alpha_device(T pfa, int reference)
 {
-    return DeviceArray<double>::from_host({cfar_alpha_typed_cpu(pfa, reference)});
+    const float p = detail::host_load(pfa);
+    if (reference < 1 || !std::isfinite(p) || p <= 0.0f || p > 1.0f)
+        throw std::invalid_argument("cfar_alpha requires reference >= 1 and pfa in (0,1]");
+    DeviceArray<double> out(1);
+    detail::check_abi(zkx_cfar_alpha_f32_device(p, reference, out.data(), nullptr));
+    cuda_utils::synchronize_stream();
+    return out;
 }
 
 // radartools_typed.h:182-195 (preallocated outputs)
@@ -199,23 +212,27 @@ template <class T>
 void ca_cfar_device(const DeviceArray<T>& x, const std::vector<int>& shape, const CaCfarOptions& options,
                     DeviceArray<float>& threshold, DeviceArray<CfarDetection>& detections)
 {
-    detail::require_float<T>();
+    (void)detail::dtype_code<T>();
     if (shape.size() > 2) throw std::invalid_argument("ca_cfar supports only scalar, 1D, or 2D input");
     if (threshold.size() != x.size() || detections.size() != x.size())
         throw std::invalid_argument("ca_cfar preallocated output shape must match input");
     if (shape.empty()) {
         if (x.size() != 1) throw std::invalid_argument("ca_cfar scalar input must contain one element");
-        const float v = x.to_host()[0];
+        const float v = detail::host_load(x.to_host()[0]);
         threshold = DeviceArray<float>::from_host({0.0f});
         detections = DeviceArray<CfarDetection>::from_host({v > 0.0f ? CfarDetection::yes : CfarDetection::no});
         return;
     }
     if (options.guard_cells.size() != shape.size() || options.reference_cells.size() != shape.size())
         throw std::invalid_argument("ca_cfar cell parameter rank mismatch");
-    detail::check_abi(zkx_ca_cfar_f32(x.data(), static_cast<int>(shape.size()), shape.data(),
+    const detail::FloatSamples xf = detail::as_float(x);
+    detail::check_abi(zkx_ca_cfar_f32(xf.data(), static_cast<int>(shape.size()), shape.data(),
                                       options.guard_cells.data(), options.reference_cells.data(), options.pfa,
                                       threshold.data(), reinterpret_cast<uint8_t*>(detections.data()), nullptr,
                                       nullptr));
+    if constexpr (std::is_integral_v<T>)
+        detail::check_abi(zkx_cfar_detect_typed(x.data(), detail::dtype_code<T>(), threshold.data(), x.size(),
+                                                reinterpret_cast<uint8_t*>(detections.data()), nullptr));
 }
 
 // radartools_typed.h:154-170

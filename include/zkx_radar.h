@@ -165,6 +165,43 @@ int zkx_widen_c64_to_c128(const zkx_complex64* in, zkx_complex128* out, size_t c
 int zkx_widen_f32_to_f64(const float* in, double* out, size_t count, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* element types of the reference's other instantiations                   */
+/* ---------------------------------------------------------------------- */
+/* The operators are instantiated for float, __half, int32, int16 and int8
+ * (radartools_typed.cpp:810-811, filtering_typed.cpp:2781+); every input
+ * sample is loaded to float (operator_compute_policy.h:42-45,64-67) and the
+ * FP32 kernels run on it.  Complex inputs are 2*count scalars. */
+#define ZKX_DTYPE_F32 0
+#define ZKX_DTYPE_F16 1
+#define ZKX_DTYPE_I32 2
+#define ZKX_DTYPE_I16 3
+#define ZKX_DTYPE_I8 4
+
+/* out[i] = (float)in[i] for `count` scalars of element type `dtype`
+ * (the reference's convert_signal_kernel, filtering_kernels.cuh:1277-1286). */
+int zkx_convert_to_f32(const void* in, int dtype, size_t count, float* out, void* stream);
+
+/* CA-CFAR detections of an integer input from its float thresholds:
+ * det = x > floor(T) in the input type; NaN T -> 0, -inf -> 1, +inf -> 0,
+ * T below / at or above the type's range -> 1 / 0
+ * (ca_cfar_detect, radartools_kernels.cuh:23-37).  dtype: an integer code. */
+int zkx_cfar_detect_typed(const void* x, int dtype, const float* threshold, size_t count,
+                          uint8_t* detections, void* stream);
+
+/* resample_poly custom coefficients scaled by the reduced up in their own
+ * element type, as float taps (resample_poly_scaled_custom_value_device,
+ * filtering_kernels.cuh:460-507): float c*up; half rounded to half when
+ * up <= 255; integers multiplied with two's-complement wrap-around in int8
+ * (up <= 127, int8 only), int16 (up < 32768) or int32. */
+int zkx_resample_poly_scale_taps(const void* coefficients, int dtype, int count, int up, float* taps,
+                                 void* stream);
+
+/* cfar_alpha_device: alpha = N * (powf(pfa, -1/N) - 1) computed in FP32 on
+ * the device and stored as one double at `alpha` (device memory)
+ * (cfar_alpha_kernel, radartools_kernels.cuh:136-148; radartools_typed.cpp:605-617). */
+int zkx_cfar_alpha_f32_device(float pfa, int reference, double* alpha, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* fused dwell pipeline (Task1 steps 2-4 in one pass per dwell)             */
 /* ---------------------------------------------------------------------- */
 

@@ -337,6 +337,37 @@ def _scaled_custom(value, up, dtype):
     return float(wrap(v * up, 16 if up < 32768 else 32))
 
 
+def scaled_custom_device(value, up, dtype):
+    """resample_poly_scaled_custom_value_device (filtering_kernels.cuh:473-497),
+    the device form the GPU path uses: as _scaled_custom, except that float
+    and half products stay in FP32 at every up, and the result is a float."""
+    if dtype == "FP32":
+        return np.float32(np.float32(value) * np.float32(up))
+    if dtype == "FP16":
+        v = np.float32(np.float16(value)) * np.float32(up)
+        return np.float32(np.float16(v)) if up <= 255 else np.float32(v)
+    return np.float32(_scaled_custom(value, up, dtype))
+
+
+def cfar_alpha_f32(pfa, reference):
+    """cfar_alpha_kernel (radartools_kernels.cuh:136-148): alpha in FP32 from
+    the pfa loaded to float, n * (powf(p, -1/n) - 1)."""
+    n = np.float32(reference)
+    p = np.float32(pfa)
+    return np.float32(n * (np.power(p, np.float32(-1.0) / n) - np.float32(1.0)))
+
+
+def cfar_alpha_f32_tolerance(pfa, reference):
+    """How far two FP32 evaluations of cfar_alpha_f32 may differ when their
+    powf results differ by up to 2 ulp (powf is not correctly rounded on every
+    platform): n * 2 ulp(p^(-1/n)) carried through the subtraction, plus 2 ulp
+    of the result."""
+    n = np.float32(reference)
+    pw = np.power(np.float32(pfa), np.float32(-1.0) / n)
+    a = cfar_alpha_f32(pfa, reference)
+    return float(n) * 2 * float(np.spacing(pw)) + 2 * float(np.spacing(np.float32(max(abs(a), 1e-30))))
+
+
 def resample_poly_typed(x, dtype, up, down, axis=0, taps=None, window_mode="kaiser", beta=5.0):
     """resample_poly_typed_cpu (filtering_typed.cpp:2617-2715) for any of the
     five input dtypes.  Returns (values as float64, output dtype name)."""

@@ -71,8 +71,11 @@ def test_task1_chain(tmp_path):
     assert np.allclose(pw, np.abs(rd.astype(np.complex128)) ** 2, rtol=1e-6, atol=0)
     # step 4: alpha and CFAR bit-identical to the reference restatement
     count = orc.ca_cfar_count(guard, ref)
-    # cfar_alpha_device<float> takes pfa as float (radartools_typed.cpp:593-617)
-    assert alpha == orc.cfar_alpha(float(np.float32(pfa)), count)
+    # cfar_alpha_device<float> computes alpha in FP32 on the device from the
+    # pfa loaded to float (radartools_typed.cpp:605-617, radartools_kernels.cuh:136-148)
+    a32 = orc.cfar_alpha_f32(pfa, count)
+    assert alpha == float(np.float32(alpha))
+    assert abs(alpha - float(a32)) <= orc.cfar_alpha_f32_tolerance(pfa, count)
     othr, odet = orc.ca_cfar(pw, guard, ref, pfa)
     assert np.array_equal(thr.view(np.uint32), othr.view(np.uint32))
     assert np.array_equal(det, odet)

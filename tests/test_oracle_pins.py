@@ -182,3 +182,65 @@ def test_firfilter_vs_scipy_lfilter(K, n):
     y, zf = orc.firfilter(b, x, zi=zi)
     assert np.allclose(y, ref_y, rtol=1e-5, atol=1e-5)
     assert np.allclose(zf, ref_zf, rtol=1e-5, atol=1e-5)
+
+
+def test_cfar_alpha_f32_device_form():
+    # FP32 alpha: a float32 value within FP32 rounding of the FP64 closed form;
+    # powf's result p^(-1/n) ~ 1 is rounded before 1 is subtracted, so the
+    # bound scales with n * p^(-1/n)
+    for pfa, n in ((1e-6, 416), (1e-3, 24), (0.5, 1), (1.0, 128)):
+        a = orc.cfar_alpha_f32(pfa, n)
+        assert isinstance(a, np.float32)
+        exact = orc.cfar_alpha(float(np.float32(pfa)), n)
+        bound = 4 * np.finfo(np.float32).eps * n * float(np.float32(pfa)) ** (-1.0 / n)
+        assert abs(float(a) - exact) <= bound
+    assert orc.cfar_alpha_f32(1.0, 24) == 0.0
+
+
+@pytest.mark.parametrize("dtype", ["FP32", "FP16", "INT32", "INT16", "INT8"])
+def test_scaled_custom_device_matches_cpu_form_below_65536(dtype):
+    # below up = 65536 the device form is the CPU form (pinned by the golden
+    # resample_poly digests) rounded to float
+    vals = {"FP32": [0.1, -3.5, 1e-3], "FP16": [0.1, -3.5, 1e-3], "INT32": [7, -100000, 2 ** 30],
+            "INT16": [7, -300, 30000], "INT8": [7, -100, 127]}[dtype]
+    for up in (1, 3, 127, 128, 255, 256, 32767, 32768):
+        for v in vals:
+            assert orc.scaled_custom_device(v, up, dtype) == np.float32(orc._scaled_custom(v, up, dtype))
+
+
+def test_scaled_custom_device_closed_forms():
+    # integer products wrap in the narrowest type the rule picks
+    assert orc.scaled_custom_device(100, 3, "INT8") == 44.0          # 300 mod 256
+    assert orc.scaled_custom_device(100, 200, "INT8") == 20000.0     # int16, no wrap
+    assert orc.scaled_custom_device(300, 200, "INT16") == -5536.0    # 60000 - 65536
+    assert orc.scaled_custom_device(300, 40000, "INT16") == 12000000.0  # int32 above 32767
+    assert orc.scaled_custom_device(2 ** 30, 4, "INT32") == 0.0      # 2^32 mod 2^32
+    # half: the product is rounded to half for up <= 255 only
+    assert orc.scaled_custom_device(0.1, 3, "FP16") == np.float32(np.float16(np.float32(np.float16(0.1)) * 3))
+    assert orc.scaled_custom_device(0.1, 300, "FP16") == np.float32(np.float16(0.1)) * np.float32(300)
+    # float: FP32 product at every up (the CPU form switches to FP64 at 65536)
+    assert orc.scaled_custom_device(0.1, 70000, "FP32") == np.float32(0.1) * np.float32(70000)
+
+
+def test_ca_cfar_integer_detection_rule():
+    # the integer rule compares with floor(T) in the input type: with
+    # all-zero rows around one cell the threshold there is 0 and x = 0 is not
+    # detected; negative inputs against threshold 0 never are
+    x = np.zeros((15, 15), np.int16)
+    x[7, 7] = 1
+    thr, det = orc.ca_cfar(x, (1, 1), (2, 2), 1e-3)
+    assert thr[7, 7] == 0 and det[7, 7] == 1
+    assert det[6, 7] == 0  # x = 0, T > 0 from the neighbour
+    xf = x.astype(np.float32)
+    thr_f, _ = orc.ca_cfar(xf, (1, 1), (2, 2), 1e-3)
+    assert np.array_equal(thr, thr_f)  # thresholds come from the float samples
+    # fractional thresholds: x > floor(T), e.g. T = 0.3 -> x = 1 > 0 detected,
+    # while the float rule would also detect 1 > 0.3; T = 1.7 -> x = 1 not, x = 2 yes
+    y = np.full((15, 15), 1, np.int8)
+    thr_y, det_y = orc.ca_cfar(y, (1, 1), (2, 2), 0.5)
+    t = float(thr_y[7, 7])
+    assert 0 < t < 1 and det_y[7, 7] == 1
+    z = np.full((15, 15), 2, np.int8)
+    z[7, 7] = 1
+    thr_z, det_z = orc.ca_cfar(z, (1, 1), (2, 2), 1e-3)
+    assert thr_z[7, 7] > 1 and det_z[7, 7] == (1 > np.floor(thr_z[7, 7]))

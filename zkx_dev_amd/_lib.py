@@ -37,6 +37,10 @@ SIGNATURES = {
     "zkx_ca_cfar_window_f32": (I, [P, I, I, I, I, I, I, D, P, P, P]),
     "zkx_widen_c64_to_c128": (I, [P, P, SZ, P]),
     "zkx_widen_f32_to_f64": (I, [P, P, SZ, P]),
+    "zkx_convert_to_f32": (I, [P, I, SZ, P, P]),
+    "zkx_cfar_detect_typed": (I, [P, I, P, SZ, P, P]),
+    "zkx_resample_poly_scale_taps": (I, [P, I, I, I, P, P]),
+    "zkx_cfar_alpha_f32_device": (I, [ctypes.c_float, I, P, P]),
     "zkx_dwell_workspace_bytes": (SZ, [I, I, I]),
     "zkx_doppler_cfar_c64": (I, [P, I, I, I, I, I, I, I, I, D, P, P, P, P, P]),
     "zkx_dwell_batch_c64": (I, [P, I, I, I, P, I, I, I, I, I, D, I, P, P, P, P, P]),
