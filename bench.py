@@ -69,7 +69,7 @@ def bench_filters(args):
     stream = torch.cuda.current_stream()
     out = {}
     cases = {
-        "resample_poly_3_2_121": (lambda: filtering.resample_poly(x, 3, 2, axis=1, coefficients=kaiser),
+        "resample_poly_3_2_121": (lambda: filtering.resample_poly(x, 3, 2, axis=1, coefficients=h_k),
                                   16 * (1 << 20) * 8 + 16 * 1572864 * 8 + 121 * 4,
                                   16 * 1572864 * (121 / 3) * 4),
         "upfirdn_1_4_255": (lambda: filtering.upfirdn(h_lp, x, 1, 4, axis=1),

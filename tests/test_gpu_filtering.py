@@ -76,6 +76,9 @@ def test_resample_poly_custom(F, up, down, K, shape, axis, cplx):
     ref = per_component(f, x) if cplx else f(x)
     assert y.shape == ref.shape
     assert orc.rel_l2(y, ref) < TOL
+    # device-resident taps are scaled on the device to the same float32 taps
+    y_dev = F.resample_poly(dev(x), up, down, axis, coefficients=dev(taps)).cpu().numpy()
+    assert np.array_equal(y_dev, y)
 
 
 @pytest.mark.parametrize("up,down", [(2, 1), (3, 2), (1, 3), (4, 6)])

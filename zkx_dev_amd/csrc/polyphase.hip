@@ -81,10 +81,62 @@ __global__ void taps_table_kernel(const float* __restrict__ h, PolyPlan p, int M
     }
 }
 
-// Shared layout: x tile [XL], out tile [TT*Pn].  The taps of a wavefront's
-// residue are uniform across its lanes and come through scalar loads, so the
-// only LDS read per shift is the one x sample that feeds M FMAs.
+// Shared layout: the x tile [XL] only.  The taps of a wavefront's residue are
+// uniform across its lanes and come through scalar loads, so the only LDS
+// read per shift is the one x sample that feeds M FMAs.
+//
+// acc[i] = output t0 + i of residue r, t0 = tc*64*M + lane*M; the lane's tile
+// elements are t0*D + dq + s = (tc*64 + lane)*S + (dq + s), s < SL.
 template <class V, int M>
+__device__ __forceinline__ void residue_outputs(const V* xs, int Lc, int S, int SL, int dq, const float* tr, int tc,
+                                                int lane, V (&acc)[M])
+{
+    const V* xb = xs + tc * 64 + lane;
+#pragma unroll
+    for (int i = 0; i < M; ++i) acc[i] = VOps<V>::zero();
+    int k = dq + SL - 1;  // dq + s, walked down with its quotient / remainder by S
+    int kq = k / S, km = k % S;
+    int s = SL - 1;
+    // four shifts per step: their four x reads and 4*M taps (one scalar
+    // load) are all in flight before the first FMA, in the same order of
+    // accumulation as one shift at a time
+    for (; s >= 3; s -= 4) {
+        int off[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            off[u] = km * Lc + kq;
+            if (--km < 0) {
+                km = S - 1;
+                --kq;
+            }
+        }
+        V xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = xb[off[u]];
+        const float* trs = tr + (s - 3) * M;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[(3 - u) * M + i], xv[u], acc[i]);
+    }
+    for (; s >= 0; --s) {
+        const V xv = xb[km * Lc + kq];
+        const float* trs = tr + s * M;
+#pragma unroll
+        for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[i], xv, acc[i]);
+        if (--km < 0) {
+            km = S - 1;
+            --kq;
+        }
+    }
+}
+
+// PN > 0: the residue count Pn = PN is compiled in and each wave computes all
+// PN residues of its M outputs per lane, so a lane holds M*PN consecutive
+// outputs of y and stores them as one contiguous run (the wave writes whole
+// cache lines).  PN == 0: any Pn, one residue per task, each output stored
+// where it lands (every Pn-th element).
+template <class V, int M, int PN>
 __global__ void __launch_bounds__(64 * kWaves)
 polyphase_kernel(const float* __restrict__ tab, const V* __restrict__ x, V* __restrict__ y, PolyPlan p, int TT, int XL)
 {
@@ -96,7 +148,6 @@ polyphase_kernel(const float* __restrict__ tab, const V* __restrict__ x, V* __re
     const int S = M * p.D;
     const int Lc = (XL + S - 1) / S;
     V* xs = reinterpret_cast<V*>(smraw);
-    V* ys = xs + static_cast<long long>(S) * Lc;
     const int tid = threadIdx.x;
     const long long line = blockIdx.y;
     const V* xl = x + line * p.n_in;
@@ -113,47 +164,58 @@ polyphase_kernel(const float* __restrict__ tab, const V* __restrict__ x, V* __re
             const long long g = xstart + e;
             tmp[u] = (e < XL && g >= 0 && g < p.n_in) ? xl[g] : VOps<V>::zero();
         }
+        if ((S & (S - 1)) == 0) {  // S a power of two: shift and mask
+            const int sh = __builtin_ctz(S);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int e = base + u * static_cast<int>(blockDim.x);
-            if (e < S * Lc) xs[(e % S) * Lc + e / S] = tmp[u];
+            for (int u = 0; u < 16; ++u) {
+                const int e = base + u * static_cast<int>(blockDim.x);
+                if (e < S * Lc) xs[(e & (S - 1)) * Lc + (e >> sh)] = tmp[u];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = base + u * static_cast<int>(blockDim.x);
+                if (e < S * Lc) xs[(e % S) * Lc + e / S] = tmp[u];
+            }
         }
     }
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;
     const int NC = TT / (64 * M);
-    for (int task = wave; task < p.Pn * NC; task += kWaves) {
-        const int r = task / NC, tc = task % NC;
-        const long long mr = static_cast<long long>(r) * p.down + p.m_off;
-        const int dq = static_cast<int>(mr / p.up - q0);
-        const float* tr = tab + static_cast<long long>(r) * SL * M;
-        // lane's first output t0 = tc*64*M + lane*M, tile element t0*D + dq + s
-        //   = (tc*64 + lane)*S + (dq + s)
-        const V* xb = xs + tc * 64 + lane;
-        V acc[M];
+    if constexpr (PN > 0) {
+        for (int tc = wave; tc < NC; tc += kWaves) {
+            V acc[PN][M];
 #pragma unroll
-        for (int i = 0; i < M; ++i) acc[i] = VOps<V>::zero();
-        int k = dq + SL - 1;  // dq + s, walked down with its quotient / remainder by S
-        int kq = k / S, km = k % S;
-        for (int s = SL - 1; s >= 0; --s) {
-            const V xv = xb[km * Lc + kq];
-            const float* trs = tr + s * M;
+            for (int r = 0; r < PN; ++r) {
+                const long long mr = static_cast<long long>(r) * p.down + p.m_off;
+                const int dq = static_cast<int>(mr / p.up - q0);
+                residue_outputs<V, M>(xs, Lc, S, SL, dq, tab + static_cast<long long>(r) * SL * M, tc, lane, acc[r]);
+            }
+            const long long n0 = (T0 + tc * 64 * M + lane * M) * PN;
+            V* yo = yl + n0;
+            if (n0 + M * PN <= p.n_out) {
 #pragma unroll
-            for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[i], xv, acc[i]);
-            if (--km < 0) {
-                km = S - 1;
-                --kq;
+                for (int e = 0; e < M * PN; ++e) yo[e] = acc[e % PN][e / PN];
+            } else {
+#pragma unroll
+                for (int e = 0; e < M * PN; ++e)
+                    if (n0 + e < p.n_out) yo[e] = acc[e % PN][e / PN];
             }
         }
-        const int t0 = tc * 64 * M + lane * M;
+    } else {
+        for (int task = wave; task < p.Pn * NC; task += kWaves) {
+            const int r = task / NC, tc = task % NC;
+            const long long mr = static_cast<long long>(r) * p.down + p.m_off;
+            const int dq = static_cast<int>(mr / p.up - q0);
+            V acc[M];
+            residue_outputs<V, M>(xs, Lc, S, SL, dq, tab + static_cast<long long>(r) * SL * M, tc, lane, acc);
+            const long long n0 = (T0 + tc * 64 * M + lane * M) * p.Pn + r;
 #pragma unroll
-        for (int i = 0; i < M; ++i) ys[(t0 + i) * p.Pn + r] = acc[i];
-    }
-    __syncthreads();
-    const long long nbase = T0 * p.Pn;
-    for (int e = tid; e < TT * p.Pn; e += blockDim.x) {
-        const long long n = nbase + e;
-        if (n < p.n_out) yl[n] = ys[e];
+            for (int i = 0; i < M; ++i) {
+                const long long n = n0 + static_cast<long long>(i) * p.Pn;
+                if (n < p.n_out) yl[n] = acc[i];
+            }
+        }
     }
 }
 
@@ -214,18 +276,18 @@ unsigned grid_for(long long n)
 
 thread_local ThreadWorkspace t_taps_ws;
 
-template <class V, int M>
+template <class V, int M, int PN>
 int launch_tiled(const float* h, const V* x, V* y, const PolyPlan& p, long long lines, hipStream_t st, bool* done)
 {
     *done = false;
-    const int NC = kWaves / std::gcd(p.Pn, kWaves);
+    const int NC = PN > 0 ? kWaves : kWaves / std::gcd(p.Pn, kWaves);
     const int TT = NC * 64 * M;
     const long long SL = p.Jmax + static_cast<long long>(M - 1) * p.D;
     const long long q_last = ((static_cast<long long>(p.Pn) - 1) * p.down + p.m_off) / p.up - p.m_off / p.up;
     const long long XL = static_cast<long long>(TT - 1) * p.D + q_last + p.Jmax + (M - 1) * p.D + 1;
     const long long S = static_cast<long long>(M) * p.D;
     const long long XLp = (XL + S - 1) / S * S;  // transposed tile rounds up to whole columns
-    const size_t lds = sizeof(V) * (static_cast<size_t>(XLp) + static_cast<size_t>(TT) * p.Pn);
+    const size_t lds = sizeof(V) * static_cast<size_t>(XLp);
     const long long tab_count = static_cast<long long>(p.Pn) * SL * M;
     // very long phase tables or tiles go to the generic kernel
     if (lds > 150 * 1024 || XL > std::numeric_limits<int>::max() || tab_count > (1 << 20)) return ZKX_SUCCESS;
@@ -240,9 +302,9 @@ int launch_tiled(const float* h, const V* x, V* y, const PolyPlan& p, long long 
                        dim3(256), 0, st, h, p, M, tab);
     ZKX_LAUNCH_CHECK();
     if (lds > 64 * 1024)
-        ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(polyphase_kernel<V, M>),
+        ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(polyphase_kernel<V, M, PN>),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL((polyphase_kernel<V, M>), dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(lines)),
+    hipLaunchKernelGGL((polyphase_kernel<V, M, PN>), dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(lines)),
                        dim3(64 * kWaves), lds, st, tab, x, y, p, TT, static_cast<int>(XL));
     ZKX_LAUNCH_CHECK();
     *done = true;
@@ -260,17 +322,17 @@ size_t tiled_lds_bytes(const PolyPlan& p, int M)
     const long long XL = (TT - 1) * p.D + q_last + p.Jmax + (M - 1) * p.D + 1;
     (void)SL;
     const long long S = static_cast<long long>(M) * p.D;
-    return sizeof(V) * static_cast<size_t>((XL + S - 1) / S * S + TT * p.Pn);
+    return sizeof(V) * static_cast<size_t>((XL + S - 1) / S * S);
 }
 
 // Register blocking depth: M = 8 gives more FMAs per LDS read of x, M = 4
-// less zero-tap padding ((M-1)*D per output group) and a smaller tile; take
-// 8 only when the taps per phase dwarf the padding and the tile still leaves
-// room for three blocks per CU.
+// less zero-tap padding ((M-1)*D shifts per output group); take 8 only when
+// the taps per phase dwarf the padding (Jmax >= 64 D; at 121 taps, 3/2, M = 8
+// measured 12% slower) and the tile stays small.
 template <class V>
 int pick_m(const PolyPlan& p)
 {
-    return (p.Jmax >= 16 * p.D && tiled_lds_bytes<V>(p, 8) <= 52 * 1024) ? 8 : 4;
+    return (p.Jmax >= 64 * p.D && tiled_lds_bytes<V>(p, 8) <= 52 * 1024) ? 8 : 4;
 }
 
 template <class V>
@@ -279,8 +341,18 @@ int run_polyphase(const float* h, const V* x, V* y, const PolyPlan& p, int outer
     if (p.n_out <= 0 || outer <= 0 || inner <= 0) return ZKX_SUCCESS;
     if (inner == 1) {
         bool done = false;
-        int rc = pick_m<V>(p) == 8 ? launch_tiled<V, 8>(h, x, y, p, outer, st, &done)
-                                : launch_tiled<V, 4>(h, x, y, p, outer, st, &done);
+        int rc;
+        if (pick_m<V>(p) == 8) {
+            rc = launch_tiled<V, 8, 0>(h, x, y, p, outer, st, &done);
+        } else {
+            switch (p.Pn) {  // residue counts compiled in for whole-line stores
+            case 1: rc = launch_tiled<V, 4, 1>(h, x, y, p, outer, st, &done); break;
+            case 2: rc = launch_tiled<V, 4, 2>(h, x, y, p, outer, st, &done); break;
+            case 3: rc = launch_tiled<V, 4, 3>(h, x, y, p, outer, st, &done); break;
+            case 4: rc = launch_tiled<V, 4, 4>(h, x, y, p, outer, st, &done); break;
+            default: rc = launch_tiled<V, 4, 0>(h, x, y, p, outer, st, &done); break;
+            }
+        }
         if (rc) return rc;
         if (done) return ZKX_SUCCESS;
     }

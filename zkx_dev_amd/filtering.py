@@ -125,15 +125,20 @@ def resample_poly(x, up, down, axis=0, *, coefficients=None, window="kaiser", be
         return x.clone()
     if x.dim() > 2:
         raise ValueError("resample_poly nonidentity rank")
-    if coefficients is not None:
-        t = np.asarray(coefficients.cpu() if isinstance(coefficients, torch.Tensor) else coefficients,
-                       np.float32)
-        taps = (t * np.float32(u)).astype(np.float32) if u < 65536 else (t.astype(np.float64) * u)
+    if coefficients is not None and isinstance(coefficients, torch.Tensor) and coefficients.is_cuda and u < 65536:
+        # device-resident taps are scaled on the device (the same float32 product)
+        taps_d = (coefficients.to(torch.float32) * float(np.float32(u))).contiguous()
         fp64_out = False
     else:
-        taps = design_resample_poly_coefficients(up, down, window, beta, design_window)
-        fp64_out = True
-    taps_d = torch.as_tensor(np.ascontiguousarray(taps, np.float32), device=x.device)
+        if coefficients is not None:
+            t = np.asarray(coefficients.cpu() if isinstance(coefficients, torch.Tensor) else coefficients,
+                           np.float32)
+            taps = (t * np.float32(u)).astype(np.float32) if u < 65536 else (t.astype(np.float64) * u)
+            fp64_out = False
+        else:
+            taps = design_resample_poly_coefficients(up, down, window, beta, design_window)
+            fp64_out = True
+        taps_d = torch.as_tensor(np.ascontiguousarray(taps, np.float32), device=x.device)
     out_n = (n * u + d - 1) // d
     shape = list(x.shape)
     shape[ax] = out_n
