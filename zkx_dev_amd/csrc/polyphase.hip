@@ -94,41 +94,51 @@ __device__ __forceinline__ void residue_outputs(const V* xs, int Lc, int S, int 
     const V* xb = xs + tc * 64 + lane;
 #pragma unroll
     for (int i = 0; i < M; ++i) acc[i] = VOps<V>::zero();
-    int k = dq + SL - 1;  // dq + s, walked down with its quotient / remainder by S
-    int kq = k / S, km = k % S;
+    // element dq + s sits at (k mod S)*Lc + k/S, k = dq + s.  Walking s down
+    // the offset drops by Lc, or rises by (S-1)*Lc - 1 when k mod S wraps.
+    // S = M*D is a multiple of 4, so once k mod S + 1 is one too, four shifts
+    // never wrap inside a step: their offsets are off - u*Lc, and the wrap is
+    // one scalar test per step.
+    static_assert(M % 4 == 0, "steps of four shifts need S = M*D divisible by 4");
+    const int k = dq + SL - 1;
+    int km = __builtin_amdgcn_readfirstlane(k % S);
+    int off = __builtin_amdgcn_readfirstlane(km * Lc + k / S);
+    const int wrap = (S - 1) * Lc - 1;
     int s = SL - 1;
+    auto single = [&]() {
+        const V xv = xb[off];
+        const float* trs = tr + s * M;
+#pragma unroll
+        for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[i], xv, acc[i]);
+        if (km == 0) {
+            km = S - 1;
+            off += wrap;
+        } else {
+            --km;
+            off -= Lc;
+        }
+    };
+    for (; s >= 0 && ((km + 1) & 3) != 0; --s) single();
     // four shifts per step: their four x reads and 4*M taps (one scalar
     // load) are all in flight before the first FMA, in the same order of
     // accumulation as one shift at a time
     for (; s >= 3; s -= 4) {
-        int off[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            off[u] = km * Lc + kq;
-            if (--km < 0) {
-                km = S - 1;
-                --kq;
-            }
-        }
         V xv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xv[u] = xb[off[u]];
+        for (int u = 0; u < 4; ++u) xv[u] = xb[off - u * Lc];
         const float* trs = tr + (s - 3) * M;
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
             for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[(3 - u) * M + i], xv[u], acc[i]);
-    }
-    for (; s >= 0; --s) {
-        const V xv = xb[km * Lc + kq];
-        const float* trs = tr + s * M;
-#pragma unroll
-        for (int i = 0; i < M; ++i) acc[i] = VOps<V>::fma(trs[i], xv, acc[i]);
-        if (--km < 0) {
-            km = S - 1;
-            --kq;
+        off -= 4 * Lc;
+        km -= 4;
+        if (km < 0) {
+            km += S;
+            off += S * Lc - 1;
         }
     }
+    for (; s >= 0; --s) single();
 }
 
 // PN > 0: the residue count Pn = PN is compiled in and each wave computes all
