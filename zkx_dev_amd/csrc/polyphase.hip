@@ -166,10 +166,11 @@ polyphase_kernel(const float* __restrict__ tab, const V* __restrict__ x, V* __re
     // m of residue r at T: T*up*D + m_r, m_r = r*down + m_off
     const long long q0 = p.m_off / p.up;  // base of residue 0 at T = 0
     const long long xstart = T0 * p.D + q0 - (p.Jmax - 1);
-    for (int base = tid; base < S * Lc; base += 16 * static_cast<int>(blockDim.x)) {
-        V tmp[16];
+    constexpr int LB = 8;  // tile samples in flight per thread
+    for (int base = tid; base < S * Lc; base += LB * static_cast<int>(blockDim.x)) {
+        V tmp[LB];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
+        for (int u = 0; u < LB; ++u) {
             const int e = base + u * static_cast<int>(blockDim.x);
             const long long g = xstart + e;
             tmp[u] = (e < XL && g >= 0 && g < p.n_in) ? xl[g] : VOps<V>::zero();
@@ -177,13 +178,13 @@ polyphase_kernel(const float* __restrict__ tab, const V* __restrict__ x, V* __re
         if ((S & (S - 1)) == 0) {  // S a power of two: shift and mask
             const int sh = __builtin_ctz(S);
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < LB; ++u) {
                 const int e = base + u * static_cast<int>(blockDim.x);
                 if (e < S * Lc) xs[(e & (S - 1)) * Lc + (e >> sh)] = tmp[u];
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < LB; ++u) {
                 const int e = base + u * static_cast<int>(blockDim.x);
                 if (e < S * Lc) xs[(e % S) * Lc + e / S] = tmp[u];
             }
