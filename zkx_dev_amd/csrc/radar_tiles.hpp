@@ -138,7 +138,9 @@ __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile,
         }
     }
     __syncthreads();
-    constexpr int RUN = 8;
+    // outputs per horizontal task: with the cells compiled in, a quarter of
+    // the tile's outputs, so the N * 4 tasks are one per thread (THREADS = 4N)
+    constexpr int RUN = (CG1 >= 0 && CR1 >= 0) ? (C::NCM - 2 * (CG1 + CR1) + 3) / 4 : 8;
     const int runs = (TCO + RUN - 1) / RUN;
     for (int task = tid; task < N * runs; task += nthr) {
         const int f = task % N, o0 = (task / N) * RUN;
