@@ -19,6 +19,7 @@ host.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -40,7 +41,8 @@ def parse():
     ap.add_argument("--cfar-mode", type=int, default=1)
     ap.add_argument("--plain-dwells", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dwell", "filters", "batched", "serve", "split"], default="dwell")
+    ap.add_argument("--workload", choices=["dwell", "filters", "batched", "serve", "split", "task1"],
+                    default="dwell")
     ap.add_argument("--dwells", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--lanes", type=int, default=2,
@@ -310,6 +312,90 @@ def bench_split(args):
         "cube_bytes": C * Pp * Rr * 8}))
 
 
+def bench_task1(args):
+    """The reference's Task1 chain around the radar path: the echo source
+    (step 1) at the dwell shape 128 x 4096 with a 256-sample LFM, the whole
+    five-step Task1 dwell at its largest profile (128 x 512, 128-sample LFM),
+    and the ambiguity function (step 5) at that profile and for a 4096-sample
+    waveform (the largest 2-D map, 8191 x 8192).  Per-call device time with
+    events on the launch stream; algorithmic bytes are the outputs written
+    (the inputs are a few KB and stay in L2)."""
+    import torch
+    from zkx_dev_amd import radar, task1
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps):
+        for _ in range(max(3, args.warmup // 10)):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    reps = max(10, args.steps // 2)
+    s11 = dict(num_pulses=128, samples_per_pulse=512, pulse_samples=128, sample_rate_hz=2.56e6, prf_hz=8000,
+               bandwidth_hz=6.4e5, carrier_frequency_hz=1e10, target_delay_samples=171, doppler_bin=16,
+               target_amplitude=1.5, noise_std=0.035, noise_seed=20260720, pfa=1e-3, cfar_guard_doppler=1,
+               cfar_guard_range=2, cfar_reference_doppler=2, cfar_reference_range=6, ambiguity_nfreq=256)
+    big = dict(s11, samples_per_pulse=R, pulse_samples=TAPS, target_delay_samples=1000, ambiguity_nfreq=512)
+    cfg_big = task1.Task1Config.from_mapping(big)
+    cfg_s11 = task1.Task1Config.from_mapping(s11)
+    from zkx_dev_amd._lib import check, lib
+    res = {}
+    # the kernels through the C ABI into preallocated outputs (device time)
+    n = cfg_big.num_pulses * cfg_big.samples_per_pulse
+    bufs = [torch.empty(cfg_big.pulse_samples, dtype=torch.complex64, device=dev)] + \
+        [torch.empty(n, dtype=torch.complex64, device=dev) for _ in range(3)]
+    params = task1._EchoParams(cfg_big.num_pulses, cfg_big.samples_per_pulse, cfg_big.pulse_samples,
+                               cfg_big.target_delay_samples, cfg_big.sample_rate_hz, cfg_big.prf_hz,
+                               cfg_big.bandwidth_hz, cfg_big.doppler_hz(), cfg_big.target_amplitude,
+                               cfg_big.noise_std, cfg_big.noise_seed)
+    ptrs = [radar._ptr(b) for b in bufs]
+    st_ptr = radar._stream()
+    t = timed(lambda: check(lib().zkx_echo_source_c64(ctypes.byref(params), *ptrs, st_ptr)), reps)
+    nbytes = 24 * n + 8 * cfg_big.pulse_samples
+    res["echo_source_128x4096_all_outputs"] = {"us": t * 1e6, "alg_bytes": nbytes, "hbm_frac": nbytes / t / HBM_PEAK,
+                                               "gsamples_per_s": n / t / 1e9}
+    t = timed(lambda: check(lib().zkx_echo_source_c64(ctypes.byref(params), None, None, None, ptrs[3], st_ptr)),
+              reps)
+    res["echo_source_128x4096_echo_only"] = {"us": t * 1e6, "alg_bytes": 8 * n, "hbm_frac": 8 * n / t / HBM_PEAK}
+    # the whole Task1 dwell through the Python host layer (launch-bound at this size)
+    t = timed(lambda: task1.run(cfg_s11), max(5, reps // 4))
+    res["task1_dwell_128x512_steps1to5_python_calls"] = {"us": t * 1e6, "dwells_per_s": 1.0 / t}
+    w = task1.echo_source(cfg_s11)["waveform"]
+    small = torch.empty((255, 256), dtype=torch.float32, device=dev)
+    t = timed(lambda: check(lib().zkx_ambgfun_c64(radar._ptr(w), 128, 128, None, 0, 0, 0, 2.56e6, 0.0, 0,
+                                                  radar._ptr(small), st_ptr)), reps)
+    cells = 255 * 256
+    res["ambgfun_2d_T128"] = {"us": t * 1e6, "cells": cells, "alg_bytes": 4 * cells,
+                              "hbm_frac": 4 * cells / t / HBM_PEAK}
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy((rng.standard_normal(4096) + 1j * rng.standard_normal(4096)).astype(np.complex64)).to(dev)
+    out = torch.empty((8191, 8192), dtype=torch.float32, device=dev)
+
+    def amb_big():
+        check(lib().zkx_ambgfun_c64(radar._ptr(x), 4096, 4096, None, 0, 0, 0, 1.0, 0.0, 0, radar._ptr(out), st_ptr))
+    t = timed(amb_big, reps)
+    cells = 8191 * 8192
+    flops = 8191 * 5.0 * 8192 * 13
+    res["ambgfun_2d_T4096"] = {"us": t * 1e6, "cells": cells, "alg_bytes": 4 * cells,
+                               "hbm_frac": 4 * cells / t / HBM_PEAK, "fft_tflops": flops / t / 1e12,
+                               "fp32_valu_frac": flops / t / 78.6e12}
+    cut_out = torch.empty(8192, dtype=torch.float64, device=dev)
+    for name, code in (("delay", 1), ("doppler", 2)):
+        t = timed(lambda: check(lib().zkx_ambgfun_c64(radar._ptr(x), 4096, 4096, None, 0, 0, code, 1.0, 0.0, 1,
+                                                      radar._ptr(cut_out), st_ptr)), reps)
+        res[f"ambgfun_{name}_cut_T4096"] = {"us": t * 1e6}
+    print(json.dumps({"metric": "Task1 chain around the radar path: per-call device time", "workload": "task1",
+                      "results": res}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -318,6 +404,9 @@ def main():
         sys.exit(2)
     if args.workload == "filters":
         bench_filters(args)
+        return
+    if args.workload == "task1":
+        bench_task1(args)
         return
     if args.workload == "batched":
         bench_batched(args)

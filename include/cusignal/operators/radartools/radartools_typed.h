@@ -14,13 +14,37 @@
 
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <stdexcept>
+#include <string>
 #include <type_traits>
 #include <vector>
 
 #include "../filtering/filtering_typed.h"
 
 namespace cusignal {
+
+enum class AmbgfunCut { two_dimensional, delay, doppler };
+enum class AmbgfunOutputDtype { fp32, fp64 };
+
+// radartools_typed.h:13-23: fs, prf (unused by the result, as in the
+// reference), cut, cut_value; the shapes describe higher-rank inputs of the
+// 2-D cut, whose first extent selects the samples that take part.
+struct AmbgfunOptions {
+    double fs = 1.0;
+    double prf = 1.0;
+    AmbgfunCut cut = AmbgfunCut::two_dimensional;
+    double cut_value = 0.0;
+    std::vector<int> input_shape;
+    std::vector<int> reference_shape;
+};
+
+struct AmbgfunDeviceResult {
+    AmbgfunOutputDtype dtype{AmbgfunOutputDtype::fp32};
+    DeviceArray<float> fp32;
+    DeviceArray<double> fp64;
+    std::vector<int> shape;
+};
 
 enum class CfarDetection : std::uint8_t { no = 0, yes = 1 };
 
@@ -79,7 +103,77 @@ float host_load(T v)
     else
         return static_cast<float>(v);
 }
+// ambgfun_first_extent (radartools_typed.cpp:87-106)
+inline int ambgfun_first_extent(std::size_t count, const std::vector<int>& shape, const char* label)
+{
+    if (count > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+        throw std::invalid_argument(std::string("ambgfun ") + label + " length exceeds int capacity");
+    if (shape.empty()) return static_cast<int>(count);
+    std::size_t product = 1;
+    for (const int extent : shape) {
+        if (extent < 1) throw std::invalid_argument(std::string("ambgfun ") + label + " shape");
+        if (product > count / static_cast<std::size_t>(extent))
+            throw std::invalid_argument(std::string("ambgfun ") + label + " shape product");
+        product *= static_cast<std::size_t>(extent);
+    }
+    if (product != count) throw std::invalid_argument(std::string("ambgfun ") + label + " data/shape mismatch");
+    return shape.front();
+}
 }  // namespace detail
+
+// radartools_typed.h:134-139 (radartools_typed.cpp:411-462).  All arithmetic
+// is FP32 on the device; the 2-D cut of float/half inputs is FP32, every
+// other result is FP64 storage of the FP32 values (ambgfun_output_dtype,
+// radartools_typed.cpp:119-130).
+template <class T>
+void ambgfun_device(const DeviceArray<TypedComplex<T>>& x, AmbgfunDeviceResult& output,
+                    const AmbgfunOptions& options = {}, const DeviceArray<TypedComplex<T>>* y = nullptr)
+{
+    (void)detail::dtype_code<T>();
+    if (options.cut != AmbgfunCut::two_dimensional && options.cut != AmbgfunCut::delay &&
+        options.cut != AmbgfunCut::doppler)
+        throw std::invalid_argument("ambgfun cut is invalid");
+    if (options.cut != AmbgfunCut::two_dimensional && options.fs == 0.0)
+        throw std::invalid_argument("ambgfun fs must be nonzero for one-dimensional cuts");
+    if (y == nullptr && !options.reference_shape.empty())
+        throw std::invalid_argument("ambgfun reference_shape requires y");
+    const bool high_rank = options.input_shape.size() > 1 || (y != nullptr && options.reference_shape.size() > 1);
+    if (high_rank && options.cut != AmbgfunCut::two_dimensional)
+        throw std::invalid_argument("ambgfun high-rank input is only defined for cut='2d'");
+    const DeviceArray<TypedComplex<T>>& reference = y == nullptr ? x : *y;
+    if (x.empty() || reference.empty()) throw std::invalid_argument("ambgfun inputs must be nonempty");
+    const int nx = detail::ambgfun_first_extent(x.size(), options.input_shape, "input");
+    const int ny = y == nullptr ? nx : detail::ambgfun_first_extent(reference.size(), options.reference_shape, "reference");
+    if (nx > std::numeric_limits<int>::max() - ny + 1) throw std::invalid_argument("ambgfun input length exceeds int capacity");
+    const int cut = static_cast<int>(options.cut);
+    const long long count = zkx_ambgfun_out_len(nx, ny, cut);
+    if (count < 0) throw std::invalid_argument("ambgfun FFT length exceeds int capacity");
+    const bool fp32 = options.cut == AmbgfunCut::two_dimensional &&
+                      (std::is_same_v<T, float> || std::is_same_v<T, __half>);
+    const detail::FloatSamples xf = detail::as_float(x);
+    const detail::FloatSamples yf = detail::as_float(reference);
+    AmbgfunDeviceResult result;
+    result.dtype = fp32 ? AmbgfunOutputDtype::fp32 : AmbgfunOutputDtype::fp64;
+    const long long rows = static_cast<long long>(nx) + ny - 1;
+    if (options.cut == AmbgfunCut::two_dimensional)
+        result.shape = {static_cast<int>(rows), static_cast<int>(count / rows)};
+    else
+        result.shape = {static_cast<int>(count)};
+    void* dst = nullptr;
+    if (fp32) {
+        result.fp32 = DeviceArray<float>(static_cast<std::size_t>(count));
+        dst = result.fp32.data();
+    } else {
+        result.fp64 = DeviceArray<double>(static_cast<std::size_t>(count));
+        dst = result.fp64.data();
+    }
+    detail::check_abi(zkx_ambgfun_c64(reinterpret_cast<const zkx_complex64*>(xf.data()), x.size(), nx,
+                                      y == nullptr ? nullptr : reinterpret_cast<const zkx_complex64*>(yf.data()),
+                                      reference.size(), ny, cut, options.fs, options.cut_value, fp32 ? 0 : 1, dst,
+                                      nullptr));
+    cuda_utils::synchronize_stream();
+    output = std::move(result);
+}
 
 // radartools_typed.h:237-262
 template <class T>

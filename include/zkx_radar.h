@@ -165,6 +165,66 @@ int zkx_widen_c64_to_c128(const zkx_complex64* in, zkx_complex128* out, size_t c
 int zkx_widen_f32_to_f64(const float* in, double* out, size_t count, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* ambiguity function                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Number of output values of zkx_ambgfun_c64 for first extents nx, ny and a
+ * cut (0 = 2-D, 1 = delay, 2 = Doppler), or -1 for invalid arguments. */
+long long zkx_ambgfun_out_len(int nx, int ny, int cut);
+
+/*
+ * Normalized ambiguity function, complex64 in, FP32 compute.
+ * Replaces ambgfun_device (radartools_typed.h:134-139, radartools_typed.cpp:411-462,
+ * radartools_typed.cu:30-126).
+ *   x [x_count] (device); the first nx samples take part, the norm covers all
+ *   x_count (a higher-rank input flattened: nx = shape[0]).  y likewise, or
+ *   NULL for y = x (then y_count and ny are ignored).  rows = nx + ny - 1,
+ *   nfreq = the next power of two >= rows.
+ *   cut 0 (2-D): out [rows, nfreq], float, or double when out_fp64 != 0 (the
+ *     reference's FP64 storage for integer inputs; values computed in FP32);
+ *   cut 1 (delay, at delay cut_value seconds): out double [nfreq];
+ *   cut 2 (Doppler, at cut_value Hz): out double [rows].
+ * fs must be nonzero for the cuts; cuts need x_count == nx and y_count == ny.
+ * The 2-D and delay cuts need nfreq <= 8192 (UNSUPPORTED_OPERATION beyond).
+ * Zero-norm inputs give NaN, as in the reference.
+ */
+int zkx_ambgfun_c64(const zkx_complex64* x, size_t x_count, int nx, const zkx_complex64* y, size_t y_count,
+                    int ny, int cut, double fs, double cut_value, int out_fp64, void* out, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* synthetic echo source (the step before the radar path)                  */
+/* ---------------------------------------------------------------------- */
+
+/* Task1 configuration fields that the echo step reads
+ * (Task1/pipeline/include/task1/task1_config.h:16-34); doppler_hz is
+ * doppler_bin * prf_hz / num_pulses in FP32 (task1_config.h:100-103). */
+typedef struct zkx_echo_params {
+    int num_pulses;
+    int samples_per_pulse;
+    int pulse_samples;
+    int target_delay_samples;
+    float sample_rate_hz;
+    float prf_hz;
+    float bandwidth_hz;
+    float doppler_hz;
+    float target_amplitude;
+    float noise_std;
+    uint32_t noise_seed;
+} zkx_echo_params;
+
+/*
+ * LFM waveform, delayed Doppler-shifted echo of one point target, and uniform
+ * hash noise, generated on the device.  Replaces Task1 step 1
+ * (Task1/pipeline/src/steps/step1/step1.cu:11-71: generate_lfm_kernel and
+ * simulate_echo_kernel).
+ *   waveform [pulse_samples], noiseless / noise / echo [num_pulses, samples_per_pulse];
+ *   waveform, noiseless and noise may be NULL.  The noise is bit-identical to
+ *   the reference's; the rest is FP32 with its operation order.
+ */
+int zkx_echo_source_c64(const zkx_echo_params* params, zkx_complex64* waveform, zkx_complex64* noiseless,
+                        zkx_complex64* noise, zkx_complex64* echo, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* element types of the reference's other instantiations                   */
 /* ---------------------------------------------------------------------- */
 /* The operators are instantiated for float, __half, int32, int16 and int8

@@ -54,6 +54,7 @@ def lib():
             "orc_window": (None, [i, i, i, P]),
             "orc_pulse_compression": (i, [P, i, i, P, i, i, i, P, i, i, P]),
             "orc_pulse_doppler": (i, [P, i, i, i, i, P, i, P]),
+            "orc_ambgfun": (i, [P, sz, i, P, sz, i, i, d, d, P]),
             "orc_cfar_alpha": (d, [d, i]),
             "orc_ca_cfar_count": (i, [i, i, i, i, i]),
             "orc_ca_cfar_alpha_f": (ctypes.c_float, [i, i, i, i, i, d]),
@@ -173,6 +174,141 @@ def pulse_doppler(x, nfft=-1, window="none", exact=False, explicit_window=None):
     if rc:
         raise ValueError(f"pulse_doppler rc={rc}")
     return out.view(np.complex128).reshape(n, R)
+
+
+AMBGFUN_CUTS = {"2d": 0, "delay": 1, "doppler": 2}
+
+
+def ambgfun(x, fs=1.0, prf=1.0, y=None, cut="2d", cut_value=0.0, nx=None, ny=None,
+            output_fp32=True):
+    """radartools_typed.cpp:329-409 ambgfun_typed_cpu (FP64 direct sums).
+    x, y: complex 1-D arrays of loaded samples (y=None means y = x); nx/ny the
+    first extents when a higher-rank input is flattened (the norm covers all
+    samples).  The 2-D cut of a float/half input is returned as FP32 (the
+    reference narrows it); the cuts and integer inputs stay FP64.  prf does not
+    enter the result (as in the reference)."""
+    del prf
+    xf = _as_c64_f32(np.asarray(x).reshape(-1))
+    xc = xf.size // 2
+    nx = xc if nx is None else int(nx)
+    if y is None:
+        yf, yc, ny_ = None, 0, 0
+    else:
+        yf = _as_c64_f32(np.asarray(y).reshape(-1))
+        yc = yf.size // 2
+        ny_ = yc if ny is None else int(ny)
+    n_y = nx if y is None else ny_
+    rows = nx + n_y - 1
+    nfreq = 1 << max(0, (rows - 1).bit_length())
+    code = AMBGFUN_CUTS[cut]
+    size = rows * nfreq if code == 0 else (nfreq if code == 1 else rows)
+    out = np.empty(size, np.float64)
+    rc = lib().orc_ambgfun(_p(xf), xc, nx, _p(yf), yc, ny_, code, float(fs), float(cut_value), _p(out))
+    if rc:
+        raise ValueError(f"ambgfun rc={rc}")
+    if code == 0:
+        out = out.reshape(rows, nfreq)
+        if output_fp32:
+            out = out.astype(np.float32)
+    return out
+
+
+def ambgfun_fft64_row(xn, yn, row, nfreq):
+    """One delay row of cuSignal's 2-D cut (radartools.py:127-143,207-213) for
+    normalised waveforms: nfreq * |fftshift(ifft(yn[c] conj(xn[c-(nx-1)+row]), nfreq))|."""
+    nx, ny = xn.size, yn.size
+    c = np.arange(ny)
+    xc = c - (nx - 1) + row
+    live = (xc >= 0) & (xc < nx)
+    d = np.zeros(ny, np.complex128)
+    d[live] = yn[live] * np.conj(xn[xc[live]])
+    return nfreq * np.abs(np.fft.fftshift(np.fft.ifft(d, nfreq)))
+
+
+def ambgfun_fft64(x, fs=1.0, y=None, cut="2d", cut_value=0.0):
+    """The cuSignal formulation (cusignal/radartools/radartools.py:185-234)
+    with numpy's FP64 FFTs: an independent computation of the same function,
+    used to pin the direct-sum restatement above."""
+    x = np.asarray(x, np.complex128)
+    xn = x / np.linalg.norm(x)
+    yn = xn if y is None else np.asarray(y, np.complex128) / np.linalg.norm(y)
+    nx, ny = xn.size, yn.size
+    rows = nx + ny - 1
+    nfreq = 1 << max(0, (rows - 1).bit_length())
+    if cut == "2d":
+        return np.stack([ambgfun_fft64_row(xn, yn, row, nfreq) for row in range(rows)])
+    if cut == "delay":
+        fd = -fs / 2 + fs * np.arange(nfreq) / nfreq
+        xs = np.fft.ifft(np.fft.fft(xn, nfreq) * np.exp(1j * 2 * np.pi * fd * cut_value))
+        ypad = np.zeros(nfreq, np.complex128)
+        ypad[:ny] = yn
+        return nfreq * np.abs(np.fft.ifftshift(np.fft.ifft(ypad * np.conj(xs), nfreq)))
+    t = np.arange(nx) / fs
+    fy = np.fft.fft(yn, rows)
+    fx = np.fft.fft(xn * np.exp(1j * 2 * np.pi * cut_value * t), rows)
+    return np.abs(np.fft.fftshift(np.fft.ifft(fy * np.conj(fx))))
+
+
+# ---------------------------------------------------------------------------
+# echo source (the step before the radar path)
+# ---------------------------------------------------------------------------
+
+def echo_noise_bits(index, seed):
+    """Task1 step1 noise hash (ZKX/Task1/accuracy/validation/step1_validation.cu:8-16):
+    a 32-bit integer mix of (index, seed); index is a uint64 array."""
+    m = np.uint64(0xFFFFFFFF)
+    i = np.asarray(index, np.uint64)
+    v = (np.uint64(seed) ^ ((i * np.uint64(747796405) + np.uint64(2891336453)) & m)) & m
+    v ^= v >> np.uint64(16)
+    v = (v * np.uint64(2246822519)) & m
+    v ^= v >> np.uint64(13)
+    v = (v * np.uint64(3266489917)) & m
+    return v ^ (v >> np.uint64(16))
+
+
+def echo_uniform_noise(index, seed, noise_std):
+    """step1_validation.cu:18-22: ((bits & 0xffff) / 32767.5 - 1) * std, in FP32."""
+    b = (echo_noise_bits(index, seed) & np.uint64(0xFFFF)).astype(np.float32)
+    return (b / np.float32(32767.5) - np.float32(1.0)) * np.float32(noise_std)
+
+
+def echo_source(num_pulses, samples_per_pulse, pulse_samples, sample_rate_hz, prf_hz,
+                bandwidth_hz, target_delay_samples, doppler_hz, target_amplitude, noise_std,
+                noise_seed):
+    """Task1 step1 CPU reference (step1_validation.cu:24-80): LFM waveform with
+    FP64 phase, one delayed Doppler-shifted echo (FP64 phase), FP32 hash noise.
+    Float parameters are taken as FP32 numbers, as the reference's config holds
+    them.  Returns (waveform [T], noiseless [P,S], noise [P,S], echo [P,S]),
+    all complex64."""
+    f32 = lambda v: float(np.float32(v))
+    fs, prf, bw = f32(sample_rate_hz), f32(prf_hz), f32(bandwidth_hz)
+    amp, fd = f32(target_amplitude), f32(doppler_hz)
+    P, S, T, d = num_pulses, samples_per_pulse, pulse_samples, target_delay_samples
+    pi = 3.14159265358979323846
+    width = T / fs
+    slope = bw / width
+    n = np.arange(T, dtype=np.float64)
+    t = n / fs - 0.5 * width
+    ph = pi * slope * t * t
+    w = (np.cos(ph).astype(np.float32) + 1j * np.sin(ph).astype(np.float32)).astype(np.complex64)
+    idx = np.arange(P * S, dtype=np.int64)
+    pulse, sample = idx // S, idx % S
+    src = sample - d
+    live = (src >= 0) & (src < T)
+    time = pulse.astype(np.float64) / prf + sample.astype(np.float64) / fs
+    phase = 2.0 * pi * fd * time
+    c, s_ = np.cos(phase), np.sin(phase)
+    wr = np.where(live, w.real[np.clip(src, 0, T - 1)].astype(np.float64), 0.0)
+    wi = np.where(live, w.imag[np.clip(src, 0, T - 1)].astype(np.float64), 0.0)
+    nl_re = np.where(live, (amp * (wr * c - wi * s_)), 0.0).astype(np.float32)
+    nl_im = np.where(live, (amp * (wr * s_ + wi * c)), 0.0).astype(np.float32)
+    base = 2 * idx.astype(np.uint64)
+    nz_re = echo_uniform_noise(base, noise_seed, noise_std)
+    nz_im = echo_uniform_noise(base + np.uint64(1), noise_seed, noise_std)
+    noiseless = (nl_re + 1j * nl_im).astype(np.complex64).reshape(P, S)
+    noise = (nz_re + 1j * nz_im).astype(np.complex64).reshape(P, S)
+    echo = ((nl_re + nz_re) + 1j * (nl_im + nz_im)).astype(np.complex64).reshape(P, S)
+    return w, noiseless, noise, echo
 
 
 def cfar_alpha(pfa, reference):

@@ -22,7 +22,7 @@
  *     :238-251 radar_window_host, :592-603 cfar_alpha_typed_cpu,
  *     :161-220 validate_ca_cfar, :222-236 ca_cfar_detect_host,
  *     :465-555 ca_cfar_typed_cpu, :618-675 pulse_compression_typed_cpu,
- *     :714-759 pulse_doppler_typed_cpu
+ *     :714-759 pulse_doppler_typed_cpu, :16-72 + :329-409 ambgfun_typed_cpu
  *   ZKX/cusignal_cpp/src/operators/filtering/filtering_typed.cpp
  *     :104-157 resample_poly_scaled_custom_value, :159-174 resample_poly_bessel_i0,
  *     :176-218 design_resample_poly_coefficients,
@@ -223,6 +223,159 @@ ORC_API int orc_pulse_doppler(const float *x, int P, int R, int nfft, int window
         }
     }
     free(w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* ambiguity function                                                  */
+/* ------------------------------------------------------------------ */
+
+/* radartools_typed.cpp:31-53 normalized_reference: FP64 energy of the whole
+ * (loaded) input, then each sample divided by its square root; a zero norm
+ * gives NaN samples.  x is interleaved (re, im) floats, count samples. */
+static void orc_amb_normalize(const float *x, size_t count, double *out)
+{
+    double sum = 0.0;
+    for (size_t i = 0; i < count; ++i) {
+        const double re = (double)x[2 * i], im = (double)x[2 * i + 1];
+        sum += re * re + im * im;
+    }
+    const double norm = sqrt(sum);
+    for (size_t i = 0; i < count; ++i) {
+        if (norm == 0.0) {
+            out[2 * i] = NAN;
+            out[2 * i + 1] = NAN;
+        } else {
+            out[2 * i] = (double)x[2 * i] / norm;
+            out[2 * i + 1] = (double)x[2 * i + 1] / norm;
+        }
+    }
+}
+
+/* complex products in the C99 form (ac - bd, ad + bc) */
+static void orc_cmul_d(double ar, double ai, double br, double bi, double *cr, double *ci)
+{
+    const double ac = ar * br, bd = ai * bi, ad = ar * bi, bc = ai * br;
+    *cr = ac - bd;
+    *ci = ad + bc;
+}
+
+/* radartools_typed.cpp:55-72 direct_transform: sums over the first
+ * min(n_in, length) samples; the inverse carries +i and 1/length. */
+static void orc_amb_dft(const double *in, int n_in, int length, int inverse, double *out)
+{
+    const double sign = inverse ? 1.0 : -1.0;
+    const double scale = inverse ? 1.0 / (double)length : 1.0;
+    for (int f = 0; f < length; ++f) {
+        double sr = 0.0, si = 0.0;
+        for (int k = 0; k < n_in && k < length; ++k) {
+            const double angle = sign * 2.0 * orc_pi_d * (double)f * (double)k / (double)length;
+            double tr, ti;
+            orc_cmul_d(in[2 * k], in[2 * k + 1], cos(angle), sin(angle), &tr, &ti);
+            sr += tr;
+            si += ti;
+        }
+        out[2 * f] = sr * scale;
+        out[2 * f + 1] = si * scale;
+    }
+}
+
+/* radartools_typed.cpp:329-409 ambgfun_typed_cpu.  x: x_count loaded samples
+ * of which the first nx take part (the norm covers all of them); y likewise,
+ * or NULL for y = x.  cut: 0 = 2-D [rows, nfreq], 1 = delay [nfreq],
+ * 2 = Doppler [rows], rows = nx + ny - 1, nfreq = next power of two >= rows.
+ * Writes FP64 magnitudes to out.  Returns 0 or a reference error number. */
+ORC_API int orc_ambgfun(const float *x, size_t x_count, int nx, const float *y, size_t y_count,
+                        int ny, int cut, double fs, double cut_value, double *out)
+{
+    if (!y) {
+        y = x;
+        y_count = x_count;
+        ny = nx;
+    }
+    if (nx < 1 || ny < 1 || (size_t)nx > x_count || (size_t)ny > y_count) return 1002;
+    if (cut < 0 || cut > 2) return 1002;
+    if (cut != 0 && fs == 0.0) return 1002;
+    const int rows = nx + ny - 1;
+    int nfreq = 1;
+    while (nfreq < rows) nfreq <<= 1;
+    double *xn = (double *)malloc(sizeof(double) * 2 * x_count);
+    double *yn = (double *)malloc(sizeof(double) * 2 * y_count);
+    double *a = (double *)calloc((size_t)2 * (size_t)nfreq, sizeof(double));
+    double *b = (double *)calloc((size_t)2 * (size_t)nfreq, sizeof(double));
+    double *c = (double *)calloc((size_t)2 * (size_t)nfreq, sizeof(double));
+    if (!xn || !yn || !a || !b || !c) {
+        free(xn); free(yn); free(a); free(b); free(c);
+        return 2003;
+    }
+    orc_amb_normalize(x, x_count, xn);
+    orc_amb_normalize(y, y_count, yn);
+    if (cut == 0) {
+        for (int row = 0; row < rows; ++row) {
+            for (int shifted = 0; shifted < nfreq; ++shifted) {
+                const int f = (shifted - nfreq / 2 + nfreq) % nfreq;
+                double sr = 0.0, si = 0.0;
+                for (int col = 0; col < ny; ++col) {
+                    const int xc = col - (nx - 1) + row;
+                    if (xc < 0 || xc >= nx) continue;
+                    const double angle = 2.0 * orc_pi_d * (double)f * (double)col / (double)nfreq;
+                    double pr, pi, tr, ti;
+                    orc_cmul_d(yn[2 * col], yn[2 * col + 1], xn[2 * xc], -xn[2 * xc + 1], &pr, &pi);
+                    orc_cmul_d(pr, pi, cos(angle), sin(angle), &tr, &ti);
+                    sr += tr;
+                    si += ti;
+                }
+                out[(size_t)row * (size_t)nfreq + (size_t)shifted] = hypot(sr, si);
+            }
+        }
+    } else if (cut == 1) {
+        orc_amb_dft(xn, (int)x_count, nfreq, 0, a);
+        for (int k = 0; k < nfreq; ++k) {
+            const double freq = -fs / 2.0 + fs * (double)k / (double)nfreq;
+            const double angle = 2.0 * orc_pi_d * freq * cut_value;
+            double tr, ti;
+            orc_cmul_d(a[2 * k], a[2 * k + 1], cos(angle), sin(angle), &tr, &ti);
+            a[2 * k] = tr;
+            a[2 * k + 1] = ti;
+        }
+        orc_amb_dft(a, nfreq, nfreq, 1, b);           /* x shifted */
+        memset(a, 0, sizeof(double) * 2 * (size_t)nfreq);
+        for (int k = 0; k < ny && k < nfreq; ++k)
+            orc_cmul_d(yn[2 * k], yn[2 * k + 1], b[2 * k], -b[2 * k + 1], &a[2 * k], &a[2 * k + 1]);
+        orc_amb_dft(a, nfreq, nfreq, 1, c);
+        for (int i = 0; i < nfreq; ++i) {
+            const int src = (i + nfreq / 2) % nfreq;
+            out[i] = (double)nfreq * hypot(c[2 * src], c[2 * src + 1]);
+        }
+    } else {
+        double *m = (double *)malloc(sizeof(double) * 2 * x_count);
+        double *yf = (double *)malloc(sizeof(double) * 2 * (size_t)rows);
+        if (!m || !yf) {
+            free(m); free(yf); free(xn); free(yn); free(a); free(b); free(c);
+            return 2003;
+        }
+        memcpy(m, xn, sizeof(double) * 2 * x_count);
+        for (int k = 0; k < nx; ++k) {
+            const double t = (double)k / fs;
+            const double angle = 2.0 * orc_pi_d * cut_value * t;
+            double tr, ti;
+            orc_cmul_d(m[2 * k], m[2 * k + 1], cos(angle), sin(angle), &tr, &ti);
+            m[2 * k] = tr;
+            m[2 * k + 1] = ti;
+        }
+        orc_amb_dft(yn, (int)y_count, rows, 0, yf);
+        orc_amb_dft(m, (int)x_count, rows, 0, a);
+        for (int k = 0; k < rows; ++k)
+            orc_cmul_d(yf[2 * k], yf[2 * k + 1], a[2 * k], -a[2 * k + 1], &b[2 * k], &b[2 * k + 1]);
+        orc_amb_dft(b, rows, rows, 1, c);
+        for (int i = 0; i < rows; ++i) {
+            const int src = (i - rows / 2 + rows) % rows;
+            out[i] = hypot(c[2 * src], c[2 * src + 1]);
+        }
+        free(m);
+        free(yf);
+    }
+    free(xn); free(yn); free(a); free(b); free(c);
     return 0;
 }
 

@@ -8,7 +8,10 @@
 //   IN : int32 P, R, T, guard_d, guard_r, ref_d, ref_r, doppler_window; f64 pfa;
 //        complex64 echo[P*R]; complex64 waveform[T]
 //   OUT: complex64 compressed[P*R]; complex64 range_doppler[P*R]; f32 power[P*R];
-//        f64 alpha; f32 threshold[P*R]; u8 detections[P*R]
+//        f64 alpha; f32 threshold[P*R]; u8 detections[P*R];
+//        f32 ambiguity[(2T-1)*nfreq]; f64 cut over delay[2T-1]; f64 cut over Doppler[nfreq]
+// The ambiguity step follows Task1/pipeline/src/steps/step5/step5.cu:22-57
+// (fs = 2.56 MHz, prf = 8 kHz as in the Task1 profiles).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -92,6 +95,22 @@ int main(int argc, char** argv)
         cusignal::DeviceArray<cusignal::CfarDetection> detections(static_cast<std::size_t>(P) * R);
         cusignal::ca_cfar_device<float>(power, {P, R}, cf, threshold, detections);
 
+        // step 5: ambiguity function of the waveform, 2-D and the two cuts through 0
+        cusignal::AmbgfunOptions ao;
+        ao.fs = 2.56e6;
+        ao.prf = 8000.0;
+        ao.cut = cusignal::AmbgfunCut::two_dimensional;
+        cusignal::AmbgfunDeviceResult amb_2d, amb_delay, amb_doppler;
+        cusignal::ambgfun_device<float>(d_waveform, amb_2d, ao);
+        ao.cut = cusignal::AmbgfunCut::doppler;
+        ao.cut_value = 0.0;
+        cusignal::ambgfun_device<float>(d_waveform, amb_delay, ao);
+        ao.cut = cusignal::AmbgfunCut::delay;
+        cusignal::ambgfun_device<float>(d_waveform, amb_doppler, ao);
+        if (amb_2d.dtype != cusignal::AmbgfunOutputDtype::fp32 ||
+            amb_delay.dtype != cusignal::AmbgfunOutputDtype::fp64)
+            throw std::runtime_error("ambgfun output dtype");
+
         std::ofstream out(argv[2], std::ios::binary);
         write_vec(out, compressed_h);
         write_vec(out, range_doppler.to_host());
@@ -99,6 +118,9 @@ int main(int argc, char** argv)
         write_vec(out, alpha.to_host());
         write_vec(out, threshold.to_host());
         write_vec(out, detections.to_host());
+        write_vec(out, amb_2d.fp32.to_host());
+        write_vec(out, amb_delay.fp64.to_host());
+        write_vec(out, amb_doppler.fp64.to_host());
         if (!out) throw std::runtime_error("write failed");
     } catch (const std::exception& e) {
         std::cerr << "task1_chain: " << e.what() << "\n";

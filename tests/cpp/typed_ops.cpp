@@ -1,7 +1,7 @@
 // The radar and filtering operators called through the cusignal operator API
 // for one element type of the reference's instantiations (float, __half,
 // int32, int16, int8; radartools_typed.cpp:810-811, filtering_typed.cu:683+),
-// plus the complex filtering extensions and fftconvolve.  Built with g++ and
+// plus the complex filtering extensions, fftconvolve and ambgfun.  Built with g++ and
 // linked to libzkx_radar.so.
 //
 // usage: typed_ops IN OUT
@@ -147,8 +147,26 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     cusignal::FftconvolveDeviceResult cv;
     cusignal::fftconvolve_device<T>(d_x, d_t, cv, {{P, R}, cusignal::FftconvolveMode::same});
     w.vec(0, cv.out.to_host());
+    // 14-16: ambiguity function of the template: 2-D (FP32 for float and half,
+    // FP64 storage for integers), delay cut and Doppler cut (FP64 storage)
+    cusignal::AmbgfunOptions ao;
+    ao.fs = 1000.0;
+    ao.prf = 100.0;
+    cusignal::AmbgfunDeviceResult ar;
+    cusignal::ambgfun_device<T>(d_t, ar, ao);
+    w.flag(ar.dtype == cusignal::AmbgfunOutputDtype::fp64 ? 1 : 0);
+    if (ar.dtype == cusignal::AmbgfunOutputDtype::fp64) w.vec(1, ar.fp64.to_host());
+    else w.vec(0, ar.fp32.to_host());
+    ao.cut = cusignal::AmbgfunCut::delay;
+    ao.cut_value = 0.002;
+    cusignal::ambgfun_device<T>(d_t, ar, ao);
+    w.vec(1, ar.fp64.to_host());
+    ao.cut = cusignal::AmbgfunCut::doppler;
+    ao.cut_value = 37.0;
+    cusignal::ambgfun_device<T>(d_t, ar, ao);
+    w.vec(1, ar.fp64.to_host());
     if constexpr (std::is_same_v<T, float>) {
-        // 14-16: complex filtering extensions on the pulses (float taps)
+        // 17-19: complex filtering extensions on the pulses (float taps)
         cusignal::UpfirdnComplexDeviceResult uc;
         cusignal::upfirdn_device<float>(d_h, d_x, up, down, uc, {{P, R}, 1});
         w.vec(0, uc.fp32.to_host());

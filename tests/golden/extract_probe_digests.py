@@ -22,7 +22,7 @@ import shlex
 import sys
 
 REF = "/root/reference"
-OPS = ["pulse_compression", "pulse_doppler", "ca_cfar", "cfar_alpha",
+OPS = ["pulse_compression", "pulse_doppler", "ca_cfar", "cfar_alpha", "ambgfun",
        "upfirdn", "resample_poly", "firfilter"]
 KEEP = ["requested_dtype", "output_dtype", "typed_input_manifest",
         "actual_input_digest", "cpu_output_digest", "gpu_output_digest",

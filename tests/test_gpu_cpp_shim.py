@@ -59,6 +59,10 @@ def test_task1_chain(tmp_path):
     alpha = take(np.float64, 1)[0]
     thr = take(np.float32, n).reshape(P, R)
     det = take(np.uint8, n).reshape(P, R)
+    rows, nfreq = 2 * T - 1, 256
+    amb = take(np.float32, rows * nfreq).reshape(rows, nfreq)
+    amb_delay = take(np.float64, rows)
+    amb_doppler = take(np.float64, nfreq)
     assert off == len(out)
 
     # step 2 against the FP64 oracle and the Python binding
@@ -79,6 +83,14 @@ def test_task1_chain(tmp_path):
     othr, odet = orc.ca_cfar(pw, guard, ref, pfa)
     assert np.array_equal(thr.view(np.uint32), othr.view(np.uint32))
     assert np.array_equal(det, odet)
+    # step 5: the same kernels as the Python binding, and the FP64 oracle
+    w = torch.from_numpy(d.template.astype(np.complex64)).cuda()
+    assert np.array_equal(amb, radar.ambgfun(w, 2.56e6, 8000.0).cpu().numpy())
+    assert np.array_equal(amb_delay, radar.ambgfun(w, 2.56e6, 8000.0, cut="doppler").cpu().numpy())
+    assert np.array_equal(amb_doppler, radar.ambgfun(w, 2.56e6, 8000.0, cut="delay").cpu().numpy())
+    assert orc.rel_l2(amb, orc.ambgfun(d.template, 2.56e6, output_fp32=False)) < 1e-5
+    assert orc.rel_l2(amb_delay, orc.ambgfun(d.template, 2.56e6, cut="doppler")) < 1e-5
+    assert orc.rel_l2(amb_doppler, orc.ambgfun(d.template, 2.56e6, cut="delay")) < 1e-5
     hits = {(int(a), int(b)) for a, b in zip(*np.nonzero(det))}
     assert any((db, rb) in hits for rb, db in zip(d.range_bins, d.doppler_bins)), "no target detected"
 

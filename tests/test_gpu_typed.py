@@ -161,6 +161,18 @@ def test_typed_operators(name, tmp_path):
     cv = _c(next(b))
     ref = filtering.fftconvolve(x, tm, mode="same").cpu().numpy()
     assert np.array_equal(cv.reshape(ref.shape), ref)
+    # ambgfun of the template: 2-D FP32 for float/half, FP64 storage for
+    # integers (radartools_typed.cpp:119-130); the cuts are FP64 storage
+    flag, val = next(b)[0], next(b)
+    assert flag == (1 if integral else 0)
+    ref = radar.ambgfun(tm, 1000.0, 100.0).cpu().numpy()
+    assert np.array_equal(val.reshape(ref.shape), ref.astype(val.dtype), equal_nan=True)
+    val = next(b)
+    ref = radar.ambgfun(tm, 1000.0, 100.0, cut="delay", cutValue=0.002).cpu().numpy()
+    assert val.dtype == np.float64 and np.array_equal(val, ref, equal_nan=True)
+    val = next(b)
+    ref = radar.ambgfun(tm, 1000.0, 100.0, cut="doppler", cutValue=37.0).cpu().numpy()
+    assert val.dtype == np.float64 and np.array_equal(val, ref, equal_nan=True)
     if name == "FP32":
         # complex extensions: real taps act on each component
         uc = _c(next(b))

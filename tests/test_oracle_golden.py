@@ -64,6 +64,12 @@ def run_case(case):
         manifest = f"power:{dt}:bytes={x.nbytes}:sha256={_digest(x)}"
         thr, _ = orc.ca_cfar(x, (1, 1), (4, 4), float(a["pfa"]))
         return manifest, _digest(thr.astype(np.float64))
+    if op == "ambgfun":
+        n = int(a["n0"])
+        x = orc.probe_complex_input(n, 19, dt)
+        manifest = f"waveform:{dt}:bytes={x.nbytes}:sha256={_digest(x)}"
+        y = orc.ambgfun(_complex_from_typed(x), float(a["fs"]), float(a["prf"]), output_fp32=not integral)
+        return manifest, _digest(y.astype(np.float64))
     if op == "cfar_alpha":
         pfa = np.array([float(a["pfa"])], np.float32).astype(orc.NP_DTYPE[dt])
         ref = int(a["reference"])
@@ -121,8 +127,13 @@ def _libm_bound(case):
     unrounded, so its digest depends on the last bit of the host libm's cos/sin
     for large arguments.  Those cases differ here (glibc 2.35) while the FP32 and
     FP16 cases at the same shapes -- same code path, narrowed to float -- match
-    bit for bit, as does n = 16.  They stay as expected failures, not skips."""
+    bit for bit, as does n = 16.  The integer-input 2-D ambiguity function is the
+    same situation (unrounded FP64 direct sums; its FP32 and FP16 cases match
+    bit for bit, and FMA contraction or another hypot does not reproduce the
+    integer digests).  They stay as expected failures, not skips."""
     a = case["args"]
+    if case["op"] == "ambgfun":
+        return bool(orc.INTEGRAL[a["dtype"]])
     return (case["op"] == "pulse_doppler" and orc.INTEGRAL[a["dtype"]]
             and int(a["n0"]) >= 32)
 

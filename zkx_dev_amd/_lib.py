@@ -41,6 +41,9 @@ SIGNATURES = {
     "zkx_cfar_detect_typed": (I, [P, I, P, SZ, P, P]),
     "zkx_resample_poly_scale_taps": (I, [P, I, I, I, P, P]),
     "zkx_cfar_alpha_f32_device": (I, [ctypes.c_float, I, P, P]),
+    "zkx_ambgfun_out_len": (LL, [I, I, I]),
+    "zkx_ambgfun_c64": (I, [P, SZ, I, P, SZ, I, I, D, D, I, P, P]),
+    "zkx_echo_source_c64": (I, [P, P, P, P, P, P]),
     "zkx_dwell_workspace_bytes": (SZ, [I, I, I]),
     "zkx_doppler_cfar_c64": (I, [P, I, I, I, I, I, I, I, I, D, P, P, P, P, P]),
     "zkx_dwell_batch_c64": (I, [P, I, I, I, P, I, I, I, I, I, D, I, P, P, P, P, P]),

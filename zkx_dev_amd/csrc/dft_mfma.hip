@@ -222,7 +222,7 @@ doppler_dft_kernel(const float2* __restrict__ x, int P, int used, int nfft, int 
                 const float2 val = make_float2(acc[a][b][v], acc[a][b][v + 1]);
                 const long long o = static_cast<long long>(k) * R + n;
                 if (yc) yc[o] = val;
-                if (pc) pc[o] = val.x * val.x + val.y * val.y;
+                if (pc) pc[o] = zkx::power_of(val);
             }
         }
 }

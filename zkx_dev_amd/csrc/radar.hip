@@ -261,7 +261,7 @@ doppler_kernel(const float2* __restrict__ x, int P, int R, int window,
     float pw[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
-        pw[k] = v[k].x * v[k].x + v[k].y * v[k].y;
+        pw[k] = zkx::power_of(v[k]);
         if (active) {
             const long long o = obase + static_cast<long long>(t + k * C::T) * R + col;
             if (y) y[o] = v[k];

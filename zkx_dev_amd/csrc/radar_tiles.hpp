@@ -129,7 +129,7 @@ __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile,
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
         const int f = t + k * C::T;
-        const float pv = v[k].x * v[k].x + v[k].y * v[k].y;
+        const float pv = power_of(v[k]);
         pw[f * PS + s] = pv;
         if (inb && s >= cm && s < cm + TCO) {
             const int o = f * R + col;

@@ -24,6 +24,10 @@ __host__ __device__ __forceinline__ float2 cmulc(float2 a, float2 b)
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 __host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// |a|^2 as the reference's power kernel evaluates re*re + im*im under FMA
+// contraction: fma(re, re, im*im) (Task1/pipeline/src/steps/step4/step4.cu:18;
+// its archived Task1 power map equals this form in every cell).
+__device__ __forceinline__ float power_of(float2 a) { return fmaf(a.x, a.x, a.y * a.y); }
 
 // Map a HIP runtime status onto the public error numbers.
 inline int hip_status_to_code(hipError_t e)

@@ -181,6 +181,62 @@ def cfar_alpha(pfa, reference):
     return out.value
 
 
+def cfar_alpha_device(pfa, reference, *, stream=None):
+    """cfar_alpha_device<float> (radartools_typed.cpp:605-617): alpha computed in
+    FP32 on the device (powf) from the pfa loaded to float, returned as a
+    one-element float64 device tensor."""
+    p = float(torch.tensor(float(pfa), dtype=torch.float32))
+    if int(reference) < 1 or not (p > 0.0 and p <= 1.0):
+        raise ValueError("cfar_alpha requires reference >= 1 and pfa in (0,1]")
+    out = torch.empty(1, dtype=torch.float64, device=torch.device("cuda", torch.cuda.current_device()))
+    check(lib().zkx_cfar_alpha_f32_device(ctypes.c_float(p), int(reference), _ptr(out), _stream(stream)))
+    return out
+
+
+AMBGFUN_CUTS = {"2d": 0, "delay": 1, "doppler": 2}
+
+
+def ambgfun(x, fs, prf, y=None, cut="2d", cutValue=0.0, *, out_fp64=False, stream=None):
+    """ambgfun_device (radartools_typed.h:134-139) with cusignal.ambgfun's
+    call form (cusignal/radartools/radartools.py:146).
+
+    x, y: complex64 device tensors; for a higher-rank x (2-D cut only) the
+    first extent selects the samples and the norm covers the whole array.
+    Returns the 2-D cut as float32 [Nx+Ny-1, Nfreq] (float64 when out_fp64,
+    the storage the reference uses for integer inputs) and the delay /
+    Doppler cuts as float64 [Nfreq] / [Nx+Ny-1].  prf does not enter the
+    result, as in the reference.
+    """
+    del prf
+    cut = str(cut).lower()
+    if cut not in AMBGFUN_CUTS:
+        raise ValueError("2d, delay, and doppler are the only cut values allowed")
+    code = AMBGFUN_CUTS[cut]
+    x = _device_tensor(x, torch.complex64, "x")
+    if y is not None:
+        y = _device_tensor(y, torch.complex64, "y")
+    if x.numel() == 0 or (y is not None and y.numel() == 0):
+        raise ValueError("ambgfun inputs must be nonempty")
+    if code != 0 and (x.dim() > 1 or (y is not None and y.dim() > 1)):
+        raise ValueError("ambgfun high-rank input is only defined for cut='2d'")
+    if code != 0 and float(fs) == 0.0:
+        raise ValueError("ambgfun fs must be nonzero for one-dimensional cuts")
+    nx = x.shape[0] if x.dim() > 0 else 1
+    ny = nx if y is None else (y.shape[0] if y.dim() > 0 else 1)
+    n = lib().zkx_ambgfun_out_len(nx, ny, code)
+    if n < 0:
+        raise ValueError("ambgfun FFT length exceeds int capacity")
+    rows = nx + ny - 1
+    fp64 = code != 0 or out_fp64
+    out = torch.empty(n, dtype=torch.float64 if fp64 else torch.float32, device=x.device)
+    rc = lib().zkx_ambgfun_c64(_ptr(x), x.numel(), nx, _ptr(y), 0 if y is None else y.numel(), ny, code,
+                               float(fs), float(cutValue), int(fp64), _ptr(out), _stream(stream))
+    if rc in (1001, 1002):
+        raise ValueError(lib().zkx_last_error_message().decode())
+    check(rc)
+    return out.view(rows, n // rows) if code == 0 else out
+
+
 def ca_cfar(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=None, detections=None,
             stream=None):
     """ca_cfar_device (radartools_typed.h:154-195): float32 map of rank 1 or 2.
