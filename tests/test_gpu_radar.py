@@ -154,6 +154,10 @@ def _cfar_inputs(kind, shape, seed):
     ("exp", (64, 1000), (1, 2), (2, 6), 1e-3),
     ("exp", (33, 70), (0, 3), (5, 1), 0.5),
     ("ones", (40, 40), (2, 2), (3, 3), 1e-3),
+    ("exp", (130, 516), (1, 2), (3, 5), 1e-4),
+    ("exp", (7, 256), (1, 1), (1, 2), 1e-2),
+    ("probe", (256, 192), (2, 1), (4, 3), 1e-3),
+    ("exp", (65, 68), (1, 1), (2, 2), 1e-3),
 ])
 def test_ca_cfar_bit_identical(R, kind, shape, guard, ref, pfa):
     x = _cfar_inputs(kind, shape, sum(shape))

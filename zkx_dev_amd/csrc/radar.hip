@@ -353,6 +353,135 @@ __global__ void __launch_bounds__(kScanRows) cfar_rowscan_kernel(float* __restri
     }
 }
 
+// Latency-bound scans for the reference's sequential FP32 order.  The serial
+// chain is one add per element per row (or column); what costs is waiting on
+// memory, so loads run SD elements ahead of the adds in straight-line code
+// (unconditional, clamped addresses; the compiler then waits for exactly the
+// load it needs instead of all of them).  Lanes past the edge work on the last
+// column and store the same values the lane that owns it stores.
+constexpr int SD = 8;
+
+// Column prefix (axis 0) per thread; x and I must not overlap.
+__global__ void __launch_bounds__(256)
+cfar_colscan_stream_kernel(const float* __restrict__ x, int rows, int cols, float* __restrict__ I)
+{
+    const int c = min(static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), cols - 1);
+    const long long base = static_cast<long long>(blockIdx.y) * rows * cols + c;
+    float buf[SD];
+#pragma unroll
+    for (int d = 0; d < SD; ++d) buf[d] = x[base + static_cast<long long>(min(d, rows - 1)) * cols];
+    const int rmain = rows - rows % SD;
+    float run = 0.0f;
+    for (int r0 = 0; r0 < rmain; r0 += SD) {
+#pragma unroll
+        for (int d = 0; d < SD; ++d) {
+            run = __fadd_rn(run, buf[d]);
+            I[base + static_cast<long long>(r0 + d) * cols] = run;
+            buf[d] = x[base + static_cast<long long>(min(r0 + d + SD, rows - 1)) * cols];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < SD; ++d) {
+        if (rmain + d < rows) {
+            run = __fadd_rn(run, buf[d]);
+            I[base + static_cast<long long>(rmain + d) * cols] = run;
+        }
+    }
+}
+
+// Row prefix (axis 1) in place with coalesced traffic: a block owns 64 rows;
+// its 256 threads move 64-column chunks between HBM and an LDS tile (one
+// float4 per thread per 16 columns, RD chunks in flight in registers), and the
+// first wave scans each tile row left to right.  Rows past the end and the
+// column tail are clamped so that every thread issues the same loads and
+// stores in every iteration of the main loop (exact waits); clamped lanes
+// store the values the owning lane stores.  Requires cols % 4 == 0.
+constexpr int RT_CW = 64, RT_TH = 256, RT_RD = 4;
+__global__ void __launch_bounds__(RT_TH)
+cfar_rowscan_tile_kernel(float* I, int rows, int cols)
+{
+    __shared__ float tile[2][64][RT_CW + 1];
+    constexpr int PER = 64 * RT_CW / 4 / RT_TH;  // float4 per thread per chunk
+    const int tid = threadIdx.x;
+    float* dw = I + static_cast<long long>(blockIdx.y) * rows * cols;
+    const int r0 = blockIdx.x * 64;
+    const int chunks = cols / RT_CW;  // full chunks; the tail is handled after
+    // element e of a chunk: row e / 16, float4 column e % 16
+    long long off[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * RT_TH;
+        const int r = min(r0 + e / (RT_CW / 4), rows - 1);
+        off[i] = static_cast<long long>(r) * cols + 4 * (e % (RT_CW / 4));
+    }
+    float4 buf[RT_RD][PER];
+    if (chunks > 0) {
+#pragma unroll
+        for (int d = 0; d < RT_RD; ++d)
+#pragma unroll
+            for (int i = 0; i < PER; ++i)
+                buf[d][i] = *reinterpret_cast<const float4*>(dw + off[i] + static_cast<long long>(min(d, chunks - 1)) * RT_CW);
+    }
+    float run = 0.0f;
+    const int lane = tid;  // scanning lane (first wave): row r0 + lane
+    auto step = [&](float4 (&cur)[PER], int k) {
+        float (*t)[RT_CW + 1] = tile[k & 1];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + i * RT_TH;
+            float* q = &t[e / (RT_CW / 4)][4 * (e % (RT_CW / 4))];
+            q[0] = cur[i].x;
+            q[1] = cur[i].y;
+            q[2] = cur[i].z;
+            q[3] = cur[i].w;
+        }
+        const int kn = min(k + RT_RD, chunks - 1);
+#pragma unroll
+        for (int i = 0; i < PER; ++i)
+            cur[i] = *reinterpret_cast<const float4*>(dw + off[i] + static_cast<long long>(kn) * RT_CW);
+        __syncthreads();
+        if (tid < 64) {
+            float* row = t[lane];
+            float v[RT_CW];
+#pragma unroll
+            for (int j = 0; j < RT_CW; ++j) v[j] = row[j];
+#pragma unroll
+            for (int j = 0; j < RT_CW; ++j) {
+                run = __fadd_rn(run, v[j]);
+                v[j] = run;
+            }
+#pragma unroll
+            for (int j = 0; j < RT_CW; ++j) row[j] = v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + i * RT_TH;
+            const float* q = &t[e / (RT_CW / 4)][4 * (e % (RT_CW / 4))];
+            *reinterpret_cast<float4*>(dw + off[i] + static_cast<long long>(k) * RT_CW) =
+                make_float4(q[0], q[1], q[2], q[3]);
+        }
+    };
+    // whole groups of RT_RD chunks: straight-line bodies, so every wait is exact
+    const int kmain = chunks - chunks % RT_RD;
+    for (int k0 = 0; k0 < kmain; k0 += RT_RD) {
+#pragma unroll
+        for (int d = 0; d < RT_RD; ++d) step(buf[d], k0 + d);
+    }
+#pragma unroll
+    for (int d = 0; d < RT_RD; ++d)
+        if (kmain + d < chunks) step(buf[d], kmain + d);
+    // column tail (cols % 64 values), in place, by the lanes that own a row
+    const int c0 = chunks * RT_CW;
+    if (c0 < cols && tid < 64 && r0 + lane < rows) {
+        float* row = dw + static_cast<long long>(r0 + lane) * cols;
+        for (int c = c0; c < cols; ++c) {
+            run = __fadd_rn(run, row[c]);
+            row[c] = run;
+        }
+    }
+}
+
 // Threshold and detection per cell from the integral image, in the reference's
 // association: T = (alpha/N) * (outer - inner); borders T = 0; det = x > T.
 __global__ void cfar_threshold_kernel(const float* __restrict__ x, const float* __restrict__ I,
@@ -381,6 +510,61 @@ __global__ void cfar_threshold_kernel(const float* __restrict__ x, const float* 
     const long long i = base + static_cast<long long>(r) * cols + c;
     thr[i] = T;
     det[i] = (!isnan(T) && x[i] > T) ? 1 : 0;
+}
+
+// The same thresholds, TK rows of one column per thread: all integral-image
+// and x loads of the TK cells are issued before any is used (clamped indices;
+// border cells get T = 0 as in cfar_threshold_kernel), so each thread keeps
+// 9*TK loads in flight instead of 9.
+constexpr int TK = 4;
+__global__ void __launch_bounds__(256)
+cfar_threshold_rows_kernel(const float* __restrict__ x, const float* __restrict__ I, int rows, int cols, int g0,
+                           int g1, int r0, int r1, float alpha, int count, float* __restrict__ thr,
+                           unsigned char* __restrict__ det)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    const int rb = blockIdx.y * TK;
+    const long long base = static_cast<long long>(blockIdx.z) * rows * cols;
+    const float* A = I + base;
+    const int rm = g0 + r0, cm = g1 + r1;
+    const bool cin = c >= cm && c < cols - cm;
+    const int orc = min(c + cm, cols - 1), olc = max(c - cm - 1, 0);
+    const int irc = min(c + g1, cols - 1), ilc = max(c - g1 - 1, 0);
+    float v[TK][8], xv[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int r = min(rb + k, rows - 1);
+        const long long ot = min(r + rm, rows - 1), ol = max(r - rm - 1, 0);
+        const long long it = min(r + g0, rows - 1), il = max(r - g0 - 1, 0);
+        v[k][0] = A[ot * cols + orc];
+        v[k][1] = A[ol * cols + orc];
+        v[k][2] = A[ot * cols + olc];
+        v[k][3] = A[ol * cols + olc];
+        v[k][4] = A[it * cols + irc];
+        v[k][5] = A[il * cols + irc];
+        v[k][6] = A[it * cols + ilc];
+        v[k][7] = A[il * cols + ilc];
+        xv[k] = x[base + static_cast<long long>(r) * cols + c];
+    }
+    const float scale = alpha / static_cast<float>(count);
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int r = rb + k;
+        if (r >= rows) break;
+        float T = 0.0f;
+        if (cin && r >= rm && r < rows - rm) {
+            float outer = v[k][0];
+            if (r - rm - 1 >= 0) outer -= v[k][1];
+            if (c - cm - 1 >= 0) outer -= v[k][2];
+            if (r - rm - 1 >= 0 && c - cm - 1 >= 0) outer += v[k][3];
+            const float inner = v[k][4] - v[k][5] - v[k][6] + v[k][7];
+            T = scale * (outer - inner);
+        }
+        const long long o = base + static_cast<long long>(r) * cols + c;
+        thr[o] = T;
+        det[o] = (!isnan(T) && xv[k] > T) ? 1 : 0;
+    }
 }
 
 // Row scan + threshold + detection, fused.  A block owns an output band of
@@ -902,6 +1086,21 @@ int cfar_faithful_2d(const float* x, const float* colsum, int batch, int rows, i
 {
     const int count = 2 * r0 * (2 * r1 + 2 * g1 + 1) + 2 * (2 * g0 + 1) * r1;
     const float alpha = cfar_alpha_float(count, pfa);
+    if (cols % 4 == 0 && (colsum == nullptr || colsum == I)) {
+        // column prefix (unless the producer made it), row prefix in place,
+        // then every cell's threshold from the finished integral image
+        if (colsum == nullptr) {
+            hipLaunchKernelGGL(cfar_colscan_stream_kernel, dim3((cols + 255) / 256, batch), dim3(256), 0, st, x,
+                               rows, cols, I);
+            ZKX_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(cfar_rowscan_tile_kernel, dim3((rows + 63) / 64, batch), dim3(RT_TH), 0, st, I, rows, cols);
+        ZKX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(cfar_threshold_rows_kernel, dim3((cols + 255) / 256, (rows + TK - 1) / TK, batch),
+                           dim3(256), 0, st, x, I, rows, cols, g0, g1, r0, r1, alpha, count, thr, det);
+        ZKX_LAUNCH_CHECK();
+        return ZKX_SUCCESS;
+    }
     if (!colsum) {
         hipLaunchKernelGGL(cfar_colscan_kernel, dim3((cols + 255) / 256, batch), dim3(256), 0, st, x, rows, cols, I);
         ZKX_LAUNCH_CHECK();
