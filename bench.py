@@ -58,8 +58,10 @@ def bench_filters(args):
     """The polyphase configurations: 16 channels x 2^20 complex64 (noise plus
     3 tones), resample_poly up=3 down=2 with a 121-tap Kaiser(beta=5) FIR as
     custom taps; upfirdn up=1 down=4 with a 255-tap lowpass; real firfilter
-    on 2^24 uniform[-1,1] samples with the same 255 taps.  Reports per-call
-    time, algorithmic bytes and the HBM fraction for each."""
+    on 2^24 uniform[-1,1] samples with the same 255 taps; decimate by 4 of
+    those samples with its designed 81-tap filter (designed per call and as
+    resident custom taps).  Reports per-call time, algorithmic bytes and the
+    HBM fraction for each."""
     import torch
     from scipy import signal
     from zkx_dev_amd import filtering, synth
@@ -70,6 +72,7 @@ def bench_filters(args):
     lp255 = signal.firwin(255, 0.25).astype(np.float32)
     xr = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, 1 << 24).astype(np.float32)).to(dev)
     h_k, h_lp = torch.from_numpy(kaiser).to(dev), torch.from_numpy(lp255).to(dev)
+    h_dec = torch.from_numpy(filtering.design_decimate_filter(4).astype(np.float32)).to(dev)
     stream = torch.cuda.current_stream()
     out = {}
     cases = {
@@ -80,6 +83,12 @@ def bench_filters(args):
                             16 * (1 << 20) * 8 + 16 * 262208 * 8 + 255 * 4, 16 * 262208 * 255 * 4),
         "firfilter_255_f32": (lambda: filtering.firfilter(h_lp, xr),
                               (1 << 24) * 8 + 255 * 4, (1 << 24) * 255 * 2),
+        # decimate by 4 with its designed 81-tap filter (the design and its
+        # upload are part of the call, as in the reference), FP64 result
+        "decimate_4_f32": (lambda: filtering.decimate(xr, 4),
+                           (1 << 24) * 4 + (1 << 22) * 8 + 81 * 4, (1 << 22) * 81 * 2),
+        "decimate_4_f32_custom_taps": (lambda: filtering.decimate(xr, 4, h_dec),
+                                       (1 << 24) * 4 + (1 << 22) * 4 + 81 * 4, (1 << 22) * 81 * 2),
     }
     for name, (fn, nbytes, flops) in cases.items():
         for _ in range(args.warmup):

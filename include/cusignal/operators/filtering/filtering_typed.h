@@ -18,7 +18,9 @@
 
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <numeric>
+#include <optional>
 #include <stdexcept>
 #include <type_traits>
 #include <vector>
@@ -83,6 +85,23 @@ struct FirfilterDeviceResult {
     std::vector<int> y_shape;
     std::vector<int> zf_shape;
     bool has_zf = false;
+};
+
+// filtering_typed.h:144-165
+enum class DecimateOutputDtype { fp32, fp64 };
+
+struct DecimateOptions {
+    std::vector<int> shape;
+    int axis = -1;
+    std::optional<int> filter_order;
+    bool zero_phase = true;
+};
+
+struct DecimateDeviceResult {
+    DecimateOutputDtype dtype{DecimateOutputDtype::fp64};
+    DeviceArray<float> fp32;
+    DeviceArray<double> fp64;
+    std::vector<int> shape;
 };
 
 // Results of the complex extension: the same fields with complex storage.
@@ -527,6 +546,84 @@ void firfilter_device(const DeviceArray<T>& b, const DeviceArray<TypedComplex<T>
                                         zi ? reinterpret_cast<const float*>(zi_full.data()) : nullptr,
                                         reinterpret_cast<float*>(out.y.data()),
                                         zi ? reinterpret_cast<float*>(out.zf.data()) : nullptr, nullptr));
+}
+
+// filtering_typed.h:702-706 (filtering_typed.cpp:2319-2356): any rank, any
+// axis; the default filter is designed on the host in FP64 (firwin, Hamming,
+// order 20*q unless given, cutoff 1/q) and rounded to float taps; custom
+// coefficients are loaded to float on the device.  FP32 arithmetic; the
+// designed filter and int32 custom coefficients give FP64 storage.
+template <typename T>
+void decimate_device(const DeviceArray<T>& x, int q, DecimateDeviceResult& out, const DecimateOptions& options = {},
+                     const DeviceArray<T>* coefficients = nullptr)
+{
+    (void)detail::dtype_code<T>();
+    if (q < 1) throw std::invalid_argument("decimate q");
+    if (x.size() > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+        throw std::invalid_argument("decimate input exceeds int indexing");
+    // prepare_decimate_layout (filtering_typed.cpp:788-853)
+    std::vector<int> shape = options.shape;
+    if (shape.empty()) shape.push_back(static_cast<int>(x.size()));
+    long long product = 1;
+    for (const int extent : shape) {
+        if (extent < 0) throw std::invalid_argument("decimate input shape");
+        if (product != 0 && extent != 0 && product > std::numeric_limits<int>::max() / extent)
+            throw std::invalid_argument("decimate input shape overflow");
+        product *= extent;
+    }
+    if (product != static_cast<long long>(x.size())) throw std::invalid_argument("decimate input shape product");
+    int axis = options.axis;
+    if (axis < 0) axis += static_cast<int>(shape.size());
+    if (axis < 0 || axis >= static_cast<int>(shape.size())) throw std::invalid_argument("decimate axis");
+    long long outer = 1, inner = 1;
+    for (int d = 0; d < axis; ++d) outer *= shape[d];
+    for (std::size_t d = axis + 1; d < shape.size(); ++d) inner *= shape[d];
+    const int n = shape[axis];
+    const int out_n = n == 0 ? 0 : 1 + (n - 1) / q;
+    std::vector<int> out_shape = shape;
+    out_shape[axis] = out_n;
+    const long long out_size = outer * inner * out_n;
+    if (outer > std::numeric_limits<int>::max() || inner > std::numeric_limits<int>::max() ||
+        out_size > std::numeric_limits<int>::max())
+        throw std::invalid_argument("decimate output shape overflow");
+    const bool custom = coefficients != nullptr;
+    if (custom && options.filter_order.has_value())
+        throw std::invalid_argument("decimate filter_order and coefficients are mutually exclusive");
+    if (custom && coefficients->empty()) throw std::invalid_argument("decimate coefficients");
+    DeviceArray<float> designed;
+    int ntaps = 0;
+    if (!custom) {
+        // design_decimate_filter (filtering_typed.cpp:318-336)
+        int order = 0;
+        if (options.filter_order.has_value()) {
+            order = *options.filter_order;
+        } else {
+            if (q > std::numeric_limits<int>::max() / 20) throw std::invalid_argument("decimate default filter order overflow");
+            order = 20 * q;
+        }
+        if (order < 0 || order == std::numeric_limits<int>::max()) throw std::invalid_argument("decimate filter order");
+        std::vector<double> taps64(static_cast<std::size_t>(order) + 1);
+        detail::check_abi(zkx_firwin_lowpass_hamming(order + 1, 1.0f / static_cast<float>(q), taps64.data()));
+        std::vector<float> taps32(taps64.begin(), taps64.end());
+        designed = DeviceArray<float>::from_host(taps32);
+        ntaps = order + 1;
+    } else {
+        ntaps = static_cast<int>(coefficients->size());
+    }
+    const detail::FloatSamples hf = custom ? detail::as_float(*coefficients) : detail::as_float(designed);
+    const detail::FloatSamples xf = detail::as_float(x);
+    DeviceArray<float> y(static_cast<std::size_t>(out_size));
+    detail::check_abi(zkx_decimate_f32(hf.data(), ntaps, xf.data(), static_cast<int>(outer), n,
+                                       static_cast<int>(inner), q, options.zero_phase ? 1 : 0, y.data(), nullptr));
+    out = DecimateDeviceResult{};
+    out.shape = out_shape;
+    out.dtype = (!custom || std::is_same_v<T, std::int32_t>) ? DecimateOutputDtype::fp64 : DecimateOutputDtype::fp32;
+    if (out.dtype == DecimateOutputDtype::fp32) {
+        cuda_utils::synchronize_stream();
+        out.fp32 = std::move(y);
+    } else {
+        out.fp64 = detail::widen(y);
+    }
 }
 
 // filtering_typed.h:563-565: 1-D, no state, caller-owned output, no allocation.

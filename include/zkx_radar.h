@@ -456,6 +456,27 @@ int zkx_firfilter_f32(const float* b, int taps, const float* x, int outer, int n
                       const float* zi, float* y, float* zf, void* stream);
 
 /*
+ * decimate: FIR then every q-th sample along the middle axis of [outer, n,
+ * inner] float32 lines.  Replaces decimate_device (filtering_typed.h:702-706,
+ * filtering_typed.cpp:2319-2356, decimate_typed_cpu :2229-2317).
+ *   y[o] = sum_{t<K} h[t] x[(o + start) q - prefix - t], o < ceil(n/q);
+ *   zero_phase != 0: prefix = q - half % q, start = (half + prefix)/q,
+ *   half = (K-1)/2; otherwise prefix = start = 0.  q = 1 filters too.
+ *   out: [outer, ceil(n/q), inner].
+ */
+int zkx_decimate_f32(const float* taps, int ntaps, const float* x, int outer, int n, int inner, int q,
+                     int zero_phase, float* out, void* stream);
+
+/*
+ * Host-side FIR design for decimate's default filter: firwin with one
+ * lowpass cutoff (fs = 2, pass_zero, scale) and a Hamming window, FP64 taps
+ * (filter_design_typed.cpp:24-100, 268-296; decimate designs numtaps =
+ * order + 1 = 20q + 1 at cutoff 1/q, filtering_typed.cpp:318-336).
+ * cutoff must lie in (0, 1).  No device work.
+ */
+int zkx_firwin_lowpass_hamming(int numtaps, float cutoff, double* taps);
+
+/*
  * fftconvolve, full/same/valid (mode 0/1/2) linear convolution of two 1-D
  * complex64 signals via the LDS FFT.  No C++ counterpart exists in the
  * reference library (operator_contracts.cmake:10-63); semantics follow

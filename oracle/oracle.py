@@ -55,6 +55,8 @@ def lib():
             "orc_pulse_compression": (i, [P, i, i, P, i, i, i, P, i, i, P]),
             "orc_pulse_doppler": (i, [P, i, i, i, i, P, i, P]),
             "orc_ambgfun": (i, [P, sz, i, P, sz, i, i, d, d, P]),
+            "orc_firwin_lowpass_hamming": (i, [i, d, P]),
+            "orc_decimate": (i, [P, i, i, i, i, P, i, i, i, P]),
             "orc_cfar_alpha": (d, [d, i]),
             "orc_ca_cfar_count": (i, [i, i, i, i, i]),
             "orc_ca_cfar_alpha_f": (ctypes.c_float, [i, i, i, i, i, d]),
@@ -174,6 +176,59 @@ def pulse_doppler(x, nfft=-1, window="none", exact=False, explicit_window=None):
     if rc:
         raise ValueError(f"pulse_doppler rc={rc}")
     return out.view(np.complex128).reshape(n, R)
+
+
+def decimate_probe_input(n, seed, dtype):
+    """ZKX/test_all/operators/cases/filtering/decimate/operator_probe.cu:29:
+    r = (i*17 + seed) % 23 - 11, times 0.25 for floating types, cast to dtype."""
+    i = np.arange(n, dtype=np.uint64)
+    r = ((i * np.uint64(17) + np.uint64(seed)) % np.uint64(23)).astype(np.int64) - 11
+    if INTEGRAL[dtype]:
+        return r.astype(NP_DTYPE[dtype])
+    return (r.astype(np.float32) * np.float32(0.25)).astype(NP_DTYPE[dtype])
+
+
+def firwin_lowpass_hamming(numtaps, cutoff):
+    """filter_design_typed.cpp:268-296 firwin_typed_cpu<float>(numtaps, {cutoff},
+    Hamming, pass_zero, scale, fs = 2): FP64 taps; cutoff is loaded as float."""
+    out = np.empty(int(numtaps), np.float64)
+    rc = lib().orc_firwin_lowpass_hamming(int(numtaps), float(np.float32(cutoff)), _p(out))
+    if rc:
+        raise ValueError("firwin cutoff")
+    return out
+
+
+def decimate_design(q, filter_order=None):
+    """design_decimate_filter (filtering_typed.cpp:318-336): order 20*q unless
+    given, cutoff 1/q in float."""
+    order = 20 * q if filter_order is None else int(filter_order)
+    if order < 0:
+        raise ValueError("decimate filter order")
+    return firwin_lowpass_hamming(order + 1, np.float32(1.0) / np.float32(q))
+
+
+def decimate(x, q, axis=-1, filter_order=None, zero_phase=True, taps=None, fp32=None):
+    """decimate_typed_cpu (filtering_typed.cpp:2229-2317).  x: loaded samples
+    (any real dtype, converted to float64 exactly); taps: custom coefficients
+    (loaded), or None for the default design.  fp32 (default: custom taps)
+    selects the FP32-accumulating path; the result is float64 storage."""
+    x = np.asarray(x)
+    xs = np.ascontiguousarray(x.astype(np.float64))
+    ax = axis % xs.ndim
+    shape = xs.shape
+    outer = int(np.prod(shape[:ax], dtype=np.int64))
+    n = shape[ax]
+    inner = int(np.prod(shape[ax + 1:], dtype=np.int64))
+    h = decimate_design(q, filter_order) if taps is None else np.ascontiguousarray(taps, np.float64)
+    if fp32 is None:
+        fp32 = taps is not None
+    out_n = 0 if n == 0 else 1 + (n - 1) // q
+    out = np.empty(outer * out_n * inner, np.float64)
+    rc = lib().orc_decimate(_p(xs), outer, n, inner, int(q), _p(h), h.size, int(bool(zero_phase)), int(bool(fp32)),
+                            _p(out))
+    if rc:
+        raise ValueError(f"decimate rc={rc}")
+    return out.reshape(shape[:ax] + (out_n,) + shape[ax + 1:])
 
 
 AMBGFUN_CUTS = {"2d": 0, "delay": 1, "doppler": 2}

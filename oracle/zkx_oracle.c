@@ -28,7 +28,9 @@
  *     :176-218 design_resample_poly_coefficients,
  *     :933-1044 prepare_resample_poly_layout, :1046-1117 prepare_upfirdn_layout,
  *     :1536-1592 firfilter_typed_cpu, :2548-2616 upfirdn_typed_cpu,
- *     :2617-2715 resample_poly_typed_cpu
+ *     :2617-2715 resample_poly_typed_cpu, :318-336 + :2229-2317 decimate_typed_cpu
+ *   ZKX/cusignal_cpp/src/operators/filter_design/filter_design_typed.cpp
+ *     :14-16 sinc, :24-100 prepare_firwin, :268-296 firwin_typed_cpu
  *   Probe input generators:
  *     ZKX/test_all/operators/shared/radartools/radartools_operator_probe.cu:35-36
  *     ZKX/test_all/operators/cases/filtering/upfirdn/operator_probe.cu:23
@@ -223,6 +225,75 @@ ORC_API int orc_pulse_doppler(const float *x, int P, int R, int nfft, int window
         }
     }
     free(w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* decimate                                                            */
+/* ------------------------------------------------------------------ */
+
+static double orc_sinc(double x)
+{
+    return fabs(x) < 1.0e-15 ? 1.0 : sin(orc_pi_d * x) / (orc_pi_d * x);
+}
+
+/* filter_design_typed.cpp:24-100 + 268-296 firwin_typed_cpu for one lowpass
+ * cutoff (fs = 2, pass_zero, scale, Hamming window): taps in FP64.  cutoff
+ * is the loaded (float) cutoff.  Returns 0, or 1001 for an invalid design. */
+ORC_API int orc_firwin_lowpass_hamming(int n, double cutoff, double *out)
+{
+    if (n <= 0 || !(cutoff > 0.0 && cutoff < 1.0)) return 1001;
+    const double mid = 0.5 * (double)(n - 1);
+    for (int i = 0; i < n; ++i) {
+        const double w = n == 1 ? 1.0 : 0.54 - 0.46 * cos(2.0 * orc_pi_d * i / (n - 1));
+        const double m = i - mid;
+        out[i] = (cutoff * orc_sinc(cutoff * m) - 0.0 * orc_sinc(0.0 * m)) * w;
+    }
+    double norm = 0.0;
+    for (int i = 0; i < n; ++i) norm += out[i] * cos(orc_pi_d * (i - mid) * 0.0);
+    for (int i = 0; i < n; ++i) out[i] /= norm;
+    return 0;
+}
+
+/* filtering_typed.cpp:2229-2317 decimate_typed_cpu on lines [outer][n][inner]:
+ * y[o] = sum_{t<K} h[t] x[(o + start) q - prefix - t] (sources outside the
+ * line skipped), prefix = q - half % q and start = (half + prefix)/q for
+ * zero phase, else 0.  fp32 != 0: FP32 products and sums in tap order (the
+ * FP32-output path), else FP64.  x holds the loaded samples. */
+ORC_API int orc_decimate(const double *x, int outer, int n, int inner, int q, const double *taps, int ntaps,
+                         int zero_phase, int fp32, double *out)
+{
+    if (q < 1 || ntaps < 1 || outer < 0 || n < 0 || inner < 0) return 1001;
+    const int half = (ntaps - 1) / 2;
+    const int prefix = zero_phase ? q - half % q : 0;
+    const long long start = zero_phase ? (half + prefix) / q : 0;
+    const int out_n = n == 0 ? 0 : 1 + (n - 1) / q;
+    for (int o = 0; o < outer; ++o) {
+        for (int in = 0; in < inner; ++in) {
+            const size_t ib = (size_t)o * (size_t)n * (size_t)inner + (size_t)in;
+            const size_t ob = (size_t)o * (size_t)out_n * (size_t)inner + (size_t)in;
+            for (int k = 0; k < out_n; ++k) {
+                const long long f = ((long long)k + start) * q;
+                if (fp32) {
+                    float sum = 0.0f;
+                    for (int pt = prefix; pt < prefix + ntaps; ++pt) {
+                        const long long src = f - pt;
+                        if (src < 0 || src >= n) continue;
+                        sum += (float)taps[pt - prefix] * (float)x[ib + (size_t)src * (size_t)inner];
+                    }
+                    out[ob + (size_t)k * (size_t)inner] = (double)sum;
+                } else {
+                    double sum = 0.0;
+                    for (int pt = prefix; pt < prefix + ntaps; ++pt) {
+                        const long long src = f - pt;
+                        if (src < 0 || src >= n) continue;
+                        sum += taps[pt - prefix] * x[ib + (size_t)src * (size_t)inner];
+                    }
+                    out[ob + (size_t)k * (size_t)inner] = sum;
+                }
+            }
+        }
+    }
     return 0;
 }
 

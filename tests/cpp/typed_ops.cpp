@@ -1,7 +1,7 @@
 // The radar and filtering operators called through the cusignal operator API
 // for one element type of the reference's instantiations (float, __half,
 // int32, int16, int8; radartools_typed.cpp:810-811, filtering_typed.cu:683+),
-// plus the complex filtering extensions, fftconvolve and ambgfun.  Built with g++ and
+// plus the complex filtering extensions, fftconvolve, decimate and ambgfun.  Built with g++ and
 // linked to libzkx_radar.so.
 //
 // usage: typed_ops IN OUT
@@ -147,7 +147,24 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     cusignal::FftconvolveDeviceResult cv;
     cusignal::fftconvolve_device<T>(d_x, d_t, cv, {{P, R}, cusignal::FftconvolveMode::same});
     w.vec(0, cv.out.to_host());
-    // 14-16: ambiguity function of the template: 2-D (FP32 for float and half,
+    // 14-16: decimate along axis 1: designed filter (FP64), custom
+    // coefficients in the input type (FP32, FP64 for int32), causal form
+    cusignal::DecimateOptions dop;
+    dop.shape = {rows, n};
+    dop.axis = 1;
+    cusignal::DecimateDeviceResult dr;
+    cusignal::decimate_device<T>(d_xf, down + 1, dr, dop);
+    w.flag(dr.dtype == cusignal::DecimateOutputDtype::fp64 ? 1 : 0);
+    w.vec(1, dr.fp64.to_host());
+    cusignal::decimate_device<T>(d_xf, down + 1, dr, dop, &d_h);
+    w.flag(dr.dtype == cusignal::DecimateOutputDtype::fp64 ? 1 : 0);
+    if (dr.dtype == cusignal::DecimateOutputDtype::fp64) w.vec(1, dr.fp64.to_host());
+    else w.vec(0, dr.fp32.to_host());
+    dop.zero_phase = false;
+    dop.filter_order = 24;
+    cusignal::decimate_device<T>(d_xf, down + 1, dr, dop);
+    w.vec(1, dr.fp64.to_host());
+    // 17-19: ambiguity function of the template: 2-D (FP32 for float and half,
     // FP64 storage for integers), delay cut and Doppler cut (FP64 storage)
     cusignal::AmbgfunOptions ao;
     ao.fs = 1000.0;
@@ -166,7 +183,7 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     cusignal::ambgfun_device<T>(d_t, ar, ao);
     w.vec(1, ar.fp64.to_host());
     if constexpr (std::is_same_v<T, float>) {
-        // 17-19: complex filtering extensions on the pulses (float taps)
+        // 20-22: complex filtering extensions on the pulses (float taps)
         cusignal::UpfirdnComplexDeviceResult uc;
         cusignal::upfirdn_device<float>(d_h, d_x, up, down, uc, {{P, R}, 1});
         w.vec(0, uc.fp32.to_host());

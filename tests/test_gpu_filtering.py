@@ -212,3 +212,54 @@ def test_frequency_domain_firfilter_zi_zf(F):
     ref_y, ref_zf = orc.firfilter(b, x, 1, zi=zi)
     assert orc.rel_l2(y.cpu().numpy(), ref_y) < TOL
     assert orc.rel_l2(zf.cpu().numpy(), ref_zf) < TOL
+
+
+@pytest.mark.parametrize("shape,axis,q,order,zero_phase", [
+    ((5000,), -1, 4, None, True), ((5000,), -1, 4, None, False), ((3, 2001), 1, 3, None, True),
+    ((777, 6), 0, 5, None, True), ((4, 333, 3), 1, 2, 30, False), ((2, 3, 4, 257), -1, 8, None, True),
+    ((32, 32), 1, 4, 32, True), ((1 << 18,), 0, 13, None, True), ((10,), 0, 16, None, True)])
+def test_decimate_default_design(F, shape, axis, q, order, zero_phase):
+    """Designed filter (FP64 storage), any rank and axis, vs the FP64 oracle."""
+    x = np.random.default_rng(q + len(shape)).standard_normal(shape).astype(np.float32)
+    print(f"decimate {shape} axis={axis} q={q} order={order} zero_phase={zero_phase}", flush=True)
+    y = F.decimate(dev(x), q, order, axis=axis, zero_phase=zero_phase).cpu().numpy()
+    ref = orc.decimate(x, q, axis=axis, filter_order=order, zero_phase=zero_phase)
+    assert y.dtype == np.float64 and y.shape == ref.shape
+    assert orc.rel_l2(y, ref) < TOL
+
+
+@pytest.mark.parametrize("n,q,zero_phase", [(100, 1, True), (100, 2, False), (1000, 5, True),
+                                            (4099, 7, False), (3, 5, True), (1, 2, True)])
+def test_decimate_custom_taps(F, n, q, zero_phase):
+    """Custom coefficients: float32 result; q = 1 still filters."""
+    x = np.random.default_rng(n).standard_normal((2, n)).astype(np.float32)
+    h = np.array([1, 2, 1, -1, 1, 0.5, -0.25], np.float32)
+    y = F.decimate(dev(x), q, dev(h), axis=1, zero_phase=zero_phase).cpu().numpy()
+    ref = orc.decimate(x, q, axis=1, taps=h.astype(np.float64), zero_phase=zero_phase, fp32=False)
+    assert y.dtype == np.float32 and y.shape == ref.shape
+    assert orc.rel_l2(y, ref) < TOL
+
+
+def test_decimate_probe_cases(F):
+    """The reference probe's inputs and filters (32x32 q=4 order 32 zero phase;
+    100x100 q=5 custom; 10x10 q=2 custom causal): the device result is within
+    1e-6 of the oracle, which reproduces the reference's digests."""
+    h = np.array([1, 2, 1, -1, 1], np.float32)
+    for rows, cols, q, order, zp, custom, seed in ((32, 32, 4, 32, True, False, 83410002),
+                                                   (100, 100, 5, None, True, True, 83410003),
+                                                   (10, 10, 2, None, False, True, 83410001)):
+        x = orc.decimate_probe_input(rows * cols, seed, "FP32").reshape(rows, cols)
+        y = F.decimate(dev(x), q, dev(h) if custom else order, axis=1, zero_phase=zp).cpu().numpy()
+        ref = orc.decimate(x.astype(np.float64), q, axis=1, filter_order=order, zero_phase=zp,
+                           taps=h.astype(np.float64) if custom else None, fp32=custom)
+        assert orc.rel_l2(y, ref) < 1e-6, (rows, cols, q)
+
+
+def test_decimate_validation(F):
+    x = dev(np.ones(16, np.float32))
+    with pytest.raises(ValueError):
+        F.decimate(x, 0)
+    with pytest.raises(ValueError):
+        F.decimate(x, 1)  # the default design's cutoff 1 is the Nyquist endpoint
+    with pytest.raises(ValueError):
+        F.decimate(x, 2, -1)

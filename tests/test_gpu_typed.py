@@ -161,6 +161,20 @@ def test_typed_operators(name, tmp_path):
     cv = _c(next(b))
     ref = filtering.fftconvolve(x, tm, mode="same").cpu().numpy()
     assert np.array_equal(cv.reshape(ref.shape), ref)
+    # decimate by down+1 along axis 1: designed filter -> FP64 storage; custom
+    # coefficients in the input type -> FP32 (FP64 for int32); causal order 24
+    qd = DOWN + 1
+    flag, val = next(b)[0], next(b)
+    assert flag == 1
+    ref = filtering.decimate(xf, qd, axis=1, widen=False).cpu().numpy()
+    assert np.array_equal(val.reshape(ref.shape), ref.astype(np.float64))
+    flag, val = next(b)[0], next(b)
+    assert flag == (1 if name == "INT32" else 0)
+    ref = filtering.decimate(xf, qd, dev["h"], axis=1).cpu().numpy()
+    assert np.array_equal(val.reshape(ref.shape), ref.astype(val.dtype))
+    val = next(b)
+    ref = filtering.decimate(xf, qd, 24, axis=1, zero_phase=False, widen=False).cpu().numpy()
+    assert np.array_equal(val.reshape(ref.shape), ref.astype(np.float64))
     # ambgfun of the template: 2-D FP32 for float/half, FP64 storage for
     # integers (radartools_typed.cpp:119-130); the cuts are FP64 storage
     flag, val = next(b)[0], next(b)

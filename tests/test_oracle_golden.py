@@ -106,6 +106,22 @@ def run_case(case):
         if odt == "FP32":
             y = y.astype(np.float32).astype(np.float64)
         return manifest, _digest(y)
+    if op == "decimate":
+        rows, cols, q = int(a["rows"]), int(a["cols"]), int(a["q"])
+        order = int(a["filter-order"])
+        zero_phase, custom = a["zero-phase"] == "true", a["custom-coefficients"] == "true"
+        x = orc.decimate_probe_input(rows * cols, int(a["seed"]), dt)
+        manifest = f"x:{dt}:[{x.size}]:bytes={x.nbytes}:sha256={_digest(x)}"
+        taps = None
+        if custom:
+            h = np.array([1, 2, 1, -1, 1], np.float32).astype(orc.NP_DTYPE[dt])
+            manifest += f"|h:{dt}:[{h.size}]:bytes={h.nbytes}:sha256={_digest(h)}"
+            taps = h.astype(np.float64)
+        fp64 = not custom or dt == "INT32"
+        y = orc.decimate(x.astype(np.float64).reshape(rows, cols), q, axis=1,
+                         filter_order=None if order < 0 else order, zero_phase=zero_phase, taps=taps,
+                         fp32=not fp64)
+        return manifest, _digest(y if fp64 else y.astype(np.float32))
     if op == "firfilter":
         n, ntaps = int(a["n"]), int(a["taps"])
         x = orc.firfilter_probe_signal(n, 29, dt)

@@ -244,3 +244,74 @@ def test_ca_cfar_integer_detection_rule():
     z[7, 7] = 1
     thr_z, det_z = orc.ca_cfar(z, (1, 1), (2, 2), 1e-3)
     assert thr_z[7, 7] > 1 and det_z[7, 7] == (1 > np.floor(thr_z[7, 7]))
+
+
+# ---------------------------------------------------------------------------
+# decimate (filtering_typed.cpp:318-336, 2229-2317)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("numtaps,cutoff", [(1, 0.5), (33, 0.25), (81, 0.25), (101, 0.2), (161, 0.125), (40, 0.3)])
+def test_firwin_lowpass_hamming_vs_scipy(numtaps, cutoff):
+    """Same float cutoff into scipy.signal.firwin (Hamming, pass_zero, scale)."""
+    from scipy import signal
+    c = float(np.float32(cutoff))
+    h = orc.firwin_lowpass_hamming(numtaps, c)
+    assert np.allclose(h, signal.firwin(numtaps, c), rtol=0, atol=1e-15)
+    assert abs(h.sum() - 1.0) < 1e-14
+
+
+def test_firwin_rejects_nyquist_cutoff():
+    """decimate's default design at q = 1 asks for cutoff 1, which firwin rejects."""
+    with pytest.raises(ValueError):
+        orc.decimate_design(1)
+
+
+@pytest.mark.parametrize("shape,axis,q,zero_phase", [((1000,), -1, 4, True), ((1000,), -1, 4, False),
+                                                     ((7, 300), 1, 3, True), ((300, 5), 0, 5, True),
+                                                     ((4, 37, 3), 1, 2, False), ((50,), 0, 8, True)])
+def test_decimate_default_design_vs_scipy(shape, axis, q, zero_phase):
+    """scipy.signal.decimate(ftype='fir') designs firwin(20q+1, 1/q, hamming)
+    and runs resample_poly(x, 1, q) (zero phase) or upfirdn(...)[:ceil(n/q)]:
+    the same definition, with a double cutoff (taps differ by ~1e-8)."""
+    from scipy import signal
+    x = np.random.default_rng(q + len(shape)).standard_normal(shape)
+    y = orc.decimate(x, q, axis=axis, zero_phase=zero_phase)
+    ref = signal.decimate(x, q, ftype="fir", axis=axis, zero_phase=zero_phase)
+    assert y.shape == ref.shape
+    assert np.allclose(y, ref, rtol=0, atol=1e-6 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("q,zero_phase", [(1, True), (2, True), (3, False), (7, True), (12, False)])
+def test_decimate_custom_taps_vs_defining_filters(q, zero_phase):
+    """Custom FIR: zero phase is resample_poly(x, 1, q, window=h), the causal
+    form the first ceil(n/q) outputs of upfirdn(h, x, 1, q) (FP64 path)."""
+    from scipy import signal
+    x = np.random.default_rng(q).standard_normal(523)
+    h = np.array([1.0, 2.0, 1.0, -1.0, 1.0, 0.5, -0.25])
+    y = orc.decimate(x, q, taps=h, zero_phase=zero_phase, fp32=False)
+    if zero_phase:
+        ref = signal.resample_poly(x, 1, q, window=h) if q > 1 else np.convolve(x, h)[(h.size - 1) // 2:][:x.size]
+    else:
+        ref = signal.upfirdn(h, x, 1, q)[:-(-x.size // q)]
+    assert np.allclose(y, ref, rtol=0, atol=1e-12)
+
+
+def test_decimate_fp32_path_rounds_in_tap_order():
+    """The FP32-output path sums float products in tap order: it equals a
+    float32 loop and stays within 1e-6 of the FP64 path."""
+    x = (np.arange(64) % 11 - 5).astype(np.float64) * 0.25
+    h = np.array([1.0, 2.0, 1.0, -1.0, 1.0])
+    y32 = orc.decimate(x, 3, taps=h, fp32=True)
+    y64 = orc.decimate(x, 3, taps=h, fp32=False)
+    assert np.allclose(y32, y64, rtol=0, atol=1e-6)
+    half, pre = 2, 3 - 2 % 3
+    start = (half + pre) // 3
+    acc = np.zeros(y32.size, np.float32)
+    for o in range(y32.size):
+        s = np.float32(0)
+        for t in range(h.size):
+            src = (o + start) * 3 - pre - t
+            if 0 <= src < x.size:
+                s = np.float32(s + np.float32(h[t]) * np.float32(x[src]))
+        acc[o] = s
+    assert np.array_equal(acc.astype(np.float64), y32)

@@ -24,6 +24,7 @@
 #include "runtime.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <limits>
 #include <numeric>
 
@@ -456,6 +457,27 @@ int resample_impl(const float* taps, int ntaps, const V* x, int outer, int n, in
     return run_polyphase<V>(taps, x, out, make_plan(u, d, m_off, ntaps, n, out_n), outer, inner, st);
 }
 
+// decimate (filtering_typed.cpp:2229-2317, 2319-2356): the FIR of taps h
+// sampled every q outputs, y[o] = sum_t h[t] x[(o + start) q - prefix - t];
+// zero phase pads prefix = q - half % q zeros in front and starts at
+// (half + prefix)/q, so m(o) = o*q + half; the causal form starts at m = 0.
+// Output length ceil(n/q).  q = 1 still filters (no identity shortcut).
+template <class V>
+int decimate_impl(const float* taps, int ntaps, const V* x, int outer, int n, int inner, int q, bool zero_phase,
+                  V* out, hipStream_t st)
+{
+    if (q < 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "decimate q");
+    if (ntaps < 1 || !taps) ZKX_FAIL(ZKX_INVALID_SHAPE, "decimate coefficients");
+    int rc = check_lines(outer, n, inner);
+    if (rc) return rc;
+    const long long out_n = n == 0 ? 0 : 1 + (static_cast<long long>(n) - 1) / q;
+    if (out_n == 0 || outer == 0 || inner == 0) return ZKX_SUCCESS;
+    if (!out || !x) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "decimate: null pointer");
+    const int half = (ntaps - 1) / 2;
+    const long long m_off = zero_phase ? half : 0;
+    return run_polyphase<V>(taps, x, out, make_plan(1, q, m_off, ntaps, n, out_n), outer, inner, st);
+}
+
 thread_local ThreadWorkspace t_fir_ws;
 
 int firfilter_impl(const float* b, int taps, const float* x, int outer, int n, int inner, const float* zi, float* y,
@@ -535,6 +557,37 @@ int zkx_resample_poly_c64(const float* taps, int ntaps, const zkx_complex64* x, 
     clear_last_error();
     return resample_impl<float2>(taps, ntaps, reinterpret_cast<const float2*>(x), outer, n, inner, up, down,
                                  reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
+int zkx_decimate_f32(const float* taps, int ntaps, const float* x, int outer, int n, int inner, int q, int zero_phase,
+                     float* y, void* stream)
+{
+    clear_last_error();
+    return decimate_impl<float>(taps, ntaps, x, outer, n, inner, q, zero_phase != 0, y,
+                                static_cast<hipStream_t>(stream));
+}
+
+int zkx_firwin_lowpass_hamming(int numtaps, float cutoff, double* taps)
+{
+    clear_last_error();
+    // firwin with fs = 2, one cutoff, pass_zero, scale, Hamming window
+    // (filter_design_typed.cpp:24-100, 268-296): h[i] = c sinc(c m) w[i],
+    // m = i - (n-1)/2, then divided by its sum (unit gain at DC)
+    const double c = static_cast<double>(cutoff);
+    if (numtaps < 1 || !taps) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "firwin shape");
+    if (!(c > 0.0 && c < 1.0)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "firwin cutoff");
+    constexpr double pi = 3.141592653589793238462643383279502884;
+    const double mid = 0.5 * (numtaps - 1);
+    double sum = 0.0;
+    for (int i = 0; i < numtaps; ++i) {
+        const double w = numtaps == 1 ? 1.0 : 0.54 - 0.46 * std::cos(2.0 * pi * i / (numtaps - 1));
+        const double arg = c * (i - mid);
+        const double sinc = std::fabs(arg) < 1.0e-15 ? 1.0 : std::sin(pi * arg) / (pi * arg);
+        taps[i] = c * sinc * w;
+        sum += taps[i];
+    }
+    for (int i = 0; i < numtaps; ++i) taps[i] /= sum;
+    return ZKX_SUCCESS;
 }
 
 int zkx_firfilter_f32(const float* b, int taps, const float* x, int outer, int n, int inner, const float* zi,

@@ -3,12 +3,14 @@
 Follows ZKX/cusignal_cpp/include/cusignal/operators/filtering/filtering_typed.h:
 upfirdn (UpfirdnOptions{shape, axis=-1}), resample_poly (ResamplePolyOptions{shape,
 axis=0, window_mode, kaiser_beta, design_window}, optional custom coefficients)
-and firfilter (FirfilterOptions{shape, axis=-1}, optional zi).  float32 lines
+firfilter (FirfilterOptions{shape, axis=-1}, optional zi) and decimate
+(DecimateOptions{shape, axis=-1, filter_order, zero_phase}).  float32 lines
 are the reference's FP32 path; complex64 lines are this library's extension
 (real taps applied per component).  All compute runs in libzkx_radar.so.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -178,7 +180,65 @@ def firfilter(b, x, axis=-1, zi=None, *, stream=None):
     return (y, zf) if zi is not None else y
 
 
-__all__ = ["upfirdn", "resample_poly", "firfilter", "design_resample_poly_coefficients"]
+def design_decimate_filter(q, order=None):
+    """design_decimate_filter (filtering_typed.cpp:318-336): firwin with
+    order + 1 taps (order 20*q by default), cutoff 1/q in float, Hamming
+    window, unit DC gain; FP64 taps designed on the host by the library."""
+    q = int(q)
+    if q < 1:
+        raise ValueError("decimate q")
+    order = 20 * q if order is None else int(order)
+    if order < 0:
+        raise ValueError("decimate filter order")
+    taps = np.empty(order + 1, np.float64)
+    code = lib().zkx_firwin_lowpass_hamming(order + 1, float(np.float32(1.0) / np.float32(q)),
+                                            taps.ctypes.data_as(ctypes.c_void_p))
+    if code == 1001:
+        raise ValueError("firwin cutoff")
+    check(code)
+    return taps
+
+
+def decimate(x, q, n=None, axis=-1, zero_phase=True, *, widen=True, stream=None):
+    """decimate_device (filtering_typed.h:702-706) in cusignal.decimate's call
+    form (cusignal/filtering/resample.py:82): an FIR anti-aliasing filter, then
+    every q-th sample along `axis` of a float32 array of any rank.
+
+    n: None (filter order 20*q), an int (the order) or the FIR coefficients.
+    The designed filter gives FP64 storage (float64 when widen=True), custom
+    coefficients a float32 result, as the reference returns."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda or x.dtype != torch.float32:
+        raise ValueError("x must be a float32 device tensor")
+    q = int(q)
+    if q < 1:
+        raise ValueError("decimate q")
+    x = x.contiguous()
+    if x.dim() == 0:
+        raise ValueError("decimate input rank")
+    outer, length, inner, ax = _lines(x.shape, axis)
+    custom = n is not None and not isinstance(n, (int, np.integer))
+    if custom:
+        taps_d = _taps(n)
+        if taps_d.numel() == 0:
+            raise ValueError("decimate coefficients")
+    else:
+        taps_d = torch.as_tensor(design_decimate_filter(q, n).astype(np.float32), device=x.device)
+    out_n = 0 if length == 0 else 1 + (length - 1) // q
+    shape = list(x.shape)
+    shape[ax] = out_n
+    y = torch.empty(shape, dtype=torch.float32, device=x.device)
+    code = lib().zkx_decimate_f32(_ptr(taps_d), taps_d.numel(), _ptr(x), outer, length, inner, q,
+                                  int(bool(zero_phase)), _ptr(y), _stream(stream))
+    if code in (1001, 1002):
+        raise ValueError(lib().zkx_last_error_message().decode())
+    check(code)
+    if not custom and widen:
+        return y.to(torch.float64)
+    return y
+
+
+__all__ = ["upfirdn", "resample_poly", "firfilter", "decimate", "design_resample_poly_coefficients",
+           "design_decimate_filter"]
 
 
 def fftconvolve(in1, in2, mode="full", *, stream=None):
