@@ -73,6 +73,8 @@ def bench_filters(args):
     xr = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, 1 << 24).astype(np.float32)).to(dev)
     h_k, h_lp = torch.from_numpy(kaiser).to(dev), torch.from_numpy(lp255).to(dev)
     h_dec = torch.from_numpy(filtering.design_decimate_filter(4).astype(np.float32)).to(dev)
+    h_ch16 = torch.from_numpy(signal.firwin(512, 1.0 / 16).astype(np.float32)).to(dev)
+    h_ch64 = torch.from_numpy(signal.firwin(2048, 1.0 / 64).astype(np.float32)).to(dev)
     stream = torch.cuda.current_stream()
     out = {}
     cases = {
@@ -89,6 +91,12 @@ def bench_filters(args):
                            (1 << 24) * 4 + (1 << 22) * 8 + 81 * 4, (1 << 22) * 81 * 2),
         "decimate_4_f32_custom_taps": (lambda: filtering.decimate(xr, 4, h_dec),
                                        (1 << 24) * 4 + (1 << 22) * 4 + 81 * 4, (1 << 22) * 81 * 2),
+        # polyphase channelizer of the same 2^24 samples: 16 channels with 32
+        # taps per phase (fused form) and 64 channels (FIR + row FFT + transpose)
+        "channelize_poly_16ch_32taps": (lambda: filtering.channelize_poly(xr, h_ch16, 16),
+                                        (1 << 24) * 4 + (1 << 24) * 8 + 512 * 4, (1 << 24) * 32 * 2),
+        "channelize_poly_64ch_32taps": (lambda: filtering.channelize_poly(xr, h_ch64, 64),
+                                        (1 << 24) * 4 + (1 << 24) * 8 + 2048 * 4, (1 << 24) * 32 * 2),
     }
     for name, (fn, nbytes, flops) in cases.items():
         for _ in range(args.warmup):

@@ -104,6 +104,12 @@ struct DecimateDeviceResult {
     std::vector<int> shape;
 };
 
+// filtering_typed.h:55-58
+struct ChannelizePolyDeviceResult {
+    DeviceArray<ComplexFloat> values;
+    std::vector<int> shape;
+};
+
 // Results of the complex extension: the same fields with complex storage.
 struct UpfirdnComplexDeviceResult {
     UpfirdnOutputDtype dtype{UpfirdnOutputDtype::fp32};
@@ -624,6 +630,72 @@ void decimate_device(const DeviceArray<T>& x, int q, DecimateDeviceResult& out, 
     } else {
         out.fp64 = detail::widen(y);
     }
+}
+
+// filtering_typed.h:513-517 (plan filtering_typed.cpp:443-466): x and h
+// loaded to float on the device; complex64 [n_chans, floor(N/n_chans)].
+template <typename T>
+void channelize_poly_device(const DeviceArray<T>& x, const DeviceArray<T>& h, int n_chans,
+                            ChannelizePolyDeviceResult& out)
+{
+    (void)detail::dtype_code<T>();
+    if (n_chans <= 0) throw std::invalid_argument("channelize_poly n_chans");
+    if (x.size() > static_cast<std::size_t>(std::numeric_limits<int>::max()) ||
+        h.size() > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+        throw std::invalid_argument("channelize_poly input exceeds int indexing");
+    if (h.size() / static_cast<std::size_t>(n_chans) > 32)
+        throw std::invalid_argument("channelize_poly taps per phase exceed 32");
+    const int points = static_cast<int>(x.size() / static_cast<std::size_t>(n_chans));
+    out = ChannelizePolyDeviceResult{};
+    out.shape = {n_chans, points};
+    out.values = DeviceArray<ComplexFloat>(static_cast<std::size_t>(n_chans) * points);
+    const detail::FloatSamples xf = detail::as_float(x), hf = detail::as_float(h);
+    detail::check_abi(zkx_channelize_poly_f32(xf.data(), static_cast<long long>(x.size()), hf.data(),
+                                              static_cast<long long>(h.size()), n_chans,
+                                              reinterpret_cast<zkx_complex64*>(out.values.data()), nullptr));
+    cuda_utils::synchronize_stream();
+}
+
+// filtering_typed.h:519-523: higher-rank x/h, of which the first extents
+// select the samples (prepare_channelize_poly, filtering_typed.cpp:469-498).
+template <typename T>
+void channelize_poly_device(const DeviceArray<T>& x, const DeviceArray<T>& h, int n_chans,
+                            const std::vector<int>& input_shape, const std::vector<int>& filter_shape,
+                            ChannelizePolyDeviceResult& out)
+{
+    auto first_extent = [](const std::vector<int>& shape, std::size_t size) -> std::size_t {
+        if (shape.empty()) throw std::invalid_argument("channelize_poly shape");
+        std::size_t product = 1;
+        bool zero = false;
+        for (const int extent : shape) {
+            if (extent < 0) throw std::invalid_argument("channelize_poly shape");
+            if (extent == 0) zero = true;
+            else if (!zero) {
+                if (product > size / static_cast<std::size_t>(extent))
+                    throw std::invalid_argument("channelize_poly data/shape mismatch");
+                product *= static_cast<std::size_t>(extent);
+            }
+        }
+        if ((zero ? 0 : product) != size) throw std::invalid_argument("channelize_poly data/shape mismatch");
+        const std::size_t first = static_cast<std::size_t>(shape.front());
+        if (first > size) throw std::invalid_argument("channelize_poly first dimension exceeds storage");
+        return first;
+    };
+    const std::size_t nx = first_extent(input_shape, x.size());
+    const std::size_t nh = first_extent(filter_shape, h.size());
+    (void)detail::dtype_code<T>();
+    if (n_chans <= 0) throw std::invalid_argument("channelize_poly n_chans");
+    if (nh / static_cast<std::size_t>(n_chans) > 32)
+        throw std::invalid_argument("channelize_poly taps per phase exceed 32");
+    const int points = static_cast<int>(nx / static_cast<std::size_t>(n_chans));
+    out = ChannelizePolyDeviceResult{};
+    out.shape = {n_chans, points};
+    out.values = DeviceArray<ComplexFloat>(static_cast<std::size_t>(n_chans) * points);
+    const detail::FloatSamples xf = detail::as_float(x), hf = detail::as_float(h);
+    detail::check_abi(zkx_channelize_poly_f32(xf.data(), static_cast<long long>(nx), hf.data(),
+                                              static_cast<long long>(nh), n_chans,
+                                              reinterpret_cast<zkx_complex64*>(out.values.data()), nullptr));
+    cuda_utils::synchronize_stream();
 }
 
 // filtering_typed.h:563-565: 1-D, no state, caller-owned output, no allocation.

@@ -468,6 +468,18 @@ int zkx_decimate_f32(const float* taps, int ntaps, const float* x, int outer, in
                      int zero_phase, float* out, void* stream);
 
 /*
+ * Polyphase channelizer, float32 samples and taps, complex64 out.  Replaces
+ * channelize_poly_device (filtering_typed.h:513-523; plan
+ * filtering_typed.cpp:443-466).
+ *   M = n_chans, T = h_count / M (<= 32), P = x_count / M (short last groups
+ *   ignored); out [M][P] channel-major:
+ *   out[c][p] = sum_{ph<M} e^{+2 pi i c ph/M} sum_{t<T, p>=t} x[(p-t)M + M-1-ph] h[ph + tM].
+ *   M must be a power of two <= 8192 or any value <= 4096.
+ */
+int zkx_channelize_poly_f32(const float* x, long long x_count, const float* h, long long h_count, int n_chans,
+                            zkx_complex64* out, void* stream);
+
+/*
  * Host-side FIR design for decimate's default filter: firwin with one
  * lowpass cutoff (fs = 2, pass_zero, scale) and a Hamming window, FP64 taps
  * (filter_design_typed.cpp:24-100, 268-296; decimate designs numtaps =

@@ -57,6 +57,7 @@ def lib():
             "orc_ambgfun": (i, [P, sz, i, P, sz, i, i, d, d, P]),
             "orc_firwin_lowpass_hamming": (i, [i, d, P]),
             "orc_decimate": (i, [P, i, i, i, i, P, i, i, i, P]),
+            "orc_channelize_poly": (i, [P, ctypes.c_longlong, P, ctypes.c_longlong, i, P]),
             "orc_cfar_alpha": (d, [d, i]),
             "orc_ca_cfar_count": (i, [i, i, i, i, i]),
             "orc_ca_cfar_alpha_f": (ctypes.c_float, [i, i, i, i, i, d]),
@@ -186,6 +187,38 @@ def decimate_probe_input(n, seed, dtype):
     if INTEGRAL[dtype]:
         return r.astype(NP_DTYPE[dtype])
     return (r.astype(np.float32) * np.float32(0.25)).astype(NP_DTYPE[dtype])
+
+
+def channelize_probe_input(count, seed, taps, dtype):
+    """ZKX/test_all/operators/cases/filtering/channelize_poly/operator_probe.cu:122-135:
+    x: (i*17 + seed) % 23 - 11 (x0.25 for floating types); taps:
+    (i*7 + 3*seed) % 9 - 4 (x0.125)."""
+    i = np.arange(count, dtype=np.uint64)
+    if taps:
+        r = ((i * np.uint64(7) + np.uint64(3 * seed)) % np.uint64(9)).astype(np.int64) - 4
+        scale = np.float32(0.125)
+    else:
+        r = ((i * np.uint64(17) + np.uint64(seed)) % np.uint64(23)).astype(np.int64) - 11
+        scale = np.float32(0.25)
+    if INTEGRAL[dtype]:
+        return r.astype(NP_DTYPE[dtype])
+    return (r.astype(np.float32) * scale).astype(NP_DTYPE[dtype])
+
+
+def channelize_poly(x, h, n_chans):
+    """channelize_poly_typed_cpu (filtering_typed.cpp:443-466 and the CPU form):
+    FP64 polyphase sums and +i DFT over phases, complex64 [n_chans, N // n_chans]."""
+    xs = np.ascontiguousarray(np.asarray(x).reshape(-1).astype(np.float64))
+    hs = np.ascontiguousarray(np.asarray(h).reshape(-1).astype(np.float64))
+    M = int(n_chans)
+    if M <= 0:
+        raise ValueError("channelize_poly n_chans")
+    P = xs.size // M
+    out = np.empty(2 * M * P, np.float32)
+    rc = lib().orc_channelize_poly(_p(xs), xs.size, _p(hs), hs.size, M, _p(out))
+    if rc:
+        raise ValueError(f"channelize_poly rc={rc}")
+    return out.view(np.complex64).reshape(M, P)
 
 
 def firwin_lowpass_hamming(numtaps, cutoff):

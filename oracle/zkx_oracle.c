@@ -29,6 +29,7 @@
  *     :933-1044 prepare_resample_poly_layout, :1046-1117 prepare_upfirdn_layout,
  *     :1536-1592 firfilter_typed_cpu, :2548-2616 upfirdn_typed_cpu,
  *     :2617-2715 resample_poly_typed_cpu, :318-336 + :2229-2317 decimate_typed_cpu
+ *     :443-466 prepare_channelize_poly + channelize_poly_typed_cpu
  *   ZKX/cusignal_cpp/src/operators/filter_design/filter_design_typed.cpp
  *     :14-16 sinc, :24-100 prepare_firwin, :268-296 firwin_typed_cpu
  *   Probe input generators:
@@ -36,6 +37,7 @@
  *     ZKX/test_all/operators/cases/filtering/upfirdn/operator_probe.cu:23
  *     ZKX/test_all/operators/cases/filtering/resample_poly/operator_probe.cu:28-29
  *     ZKX/test_all/operators/shared/filtering/task2_filter_probe.cu:28
+ *     ZKX/test_all/operators/cases/filtering/channelize_poly/operator_probe.cu:122-135
  */
 #include <math.h>
 #include <stdint.h>
@@ -292,6 +294,40 @@ ORC_API int orc_decimate(const double *x, int outer, int n, int inner, int q, co
                     out[ob + (size_t)k * (size_t)inner] = sum;
                 }
             }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* channelize_poly                                                     */
+/* ------------------------------------------------------------------ */
+
+/* filtering_typed.cpp:443-466 plan + channelize_poly_typed_cpu: M channels,
+ * T = hn/M taps per phase (<= 32), P = n/M points; FP64 polyphase sums and a
+ * direct +i DFT over the phases, stored as float (re, im) channel-major
+ * [M][P].  x, h hold the loaded samples.  Returns 0 or 1001. */
+ORC_API int orc_channelize_poly(const double *x, long long n, const double *h, long long hn, int M, float *out)
+{
+    if (M <= 0 || n < 0 || hn < 0) return 1001;
+    const int T = (int)(hn / M);
+    if (T > 32) return 1001;
+    const int P = (int)(n / M);
+    for (int c = 0; c < M; ++c) {
+        for (int p = 0; p < P; ++p) {
+            double re = 0.0, im = 0.0;
+            for (int ph = 0; ph < M; ++ph) {
+                double f = 0.0;
+                for (int t = 0; t < T; ++t) {
+                    const int g = p - t;
+                    if (g >= 0) f += x[(size_t)g * (size_t)M + (size_t)(M - 1 - ph)] * h[ph + (size_t)t * (size_t)M];
+                }
+                const double angle = 6.283185307179586476925286766559 * (double)c * ph / M;
+                re += f * cos(angle);
+                im += f * sin(angle);
+            }
+            out[2 * ((size_t)c * (size_t)P + (size_t)p)] = (float)re;
+            out[2 * ((size_t)c * (size_t)P + (size_t)p) + 1] = (float)im;
         }
     }
     return 0;

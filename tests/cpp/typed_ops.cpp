@@ -1,7 +1,8 @@
 // The radar and filtering operators called through the cusignal operator API
 // for one element type of the reference's instantiations (float, __half,
 // int32, int16, int8; radartools_typed.cpp:810-811, filtering_typed.cu:683+),
-// plus the complex filtering extensions, fftconvolve, decimate and ambgfun.  Built with g++ and
+// plus the complex filtering extensions, fftconvolve, decimate,
+// channelize_poly and ambgfun.  Built with g++ and
 // linked to libzkx_radar.so.
 //
 // usage: typed_ops IN OUT
@@ -164,7 +165,11 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     dop.filter_order = 24;
     cusignal::decimate_device<T>(d_xf, down + 1, dr, dop);
     w.vec(1, dr.fp64.to_host());
-    // 17-19: ambiguity function of the template: 2-D (FP32 for float and half,
+    // 17: channelize_poly of the filtering signal with its taps, 4 channels
+    cusignal::ChannelizePolyDeviceResult cp;
+    cusignal::channelize_poly_device<T>(d_xf, d_h, 4, cp);
+    w.vec(0, cp.values.to_host());
+    // 18-20: ambiguity function of the template: 2-D (FP32 for float and half,
     // FP64 storage for integers), delay cut and Doppler cut (FP64 storage)
     cusignal::AmbgfunOptions ao;
     ao.fs = 1000.0;
@@ -183,7 +188,7 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     cusignal::ambgfun_device<T>(d_t, ar, ao);
     w.vec(1, ar.fp64.to_host());
     if constexpr (std::is_same_v<T, float>) {
-        // 20-22: complex filtering extensions on the pulses (float taps)
+        // 21-23: complex filtering extensions on the pulses (float taps)
         cusignal::UpfirdnComplexDeviceResult uc;
         cusignal::upfirdn_device<float>(d_h, d_x, up, down, uc, {{P, R}, 1});
         w.vec(0, uc.fp32.to_host());

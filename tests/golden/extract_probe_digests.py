@@ -12,6 +12,7 @@ Probe programs that produced these files:
   resample:   ZKX/test_all/operators/cases/filtering/resample_poly/operator_probe.cu
   firfilter:  ZKX/test_all/operators/shared/filtering/task2_filter_probe.cu
   decimate:   ZKX/test_all/operators/cases/filtering/decimate/operator_probe.cu
+  channelize: ZKX/test_all/operators/cases/filtering/channelize_poly/operator_probe.cu
 
 Run in a container where /root/reference exists:
     python tests/golden/extract_probe_digests.py
@@ -24,7 +25,7 @@ import sys
 
 REF = "/root/reference"
 OPS = ["pulse_compression", "pulse_doppler", "ca_cfar", "cfar_alpha", "ambgfun",
-       "upfirdn", "resample_poly", "firfilter", "decimate"]
+       "upfirdn", "resample_poly", "firfilter", "decimate", "channelize_poly"]
 KEEP = ["requested_dtype", "output_dtype", "typed_input_manifest",
         "actual_input_digest", "cpu_output_digest", "gpu_output_digest",
         "output_shape", "relative_l2", "relative_linf"]

@@ -263,3 +263,36 @@ def test_decimate_validation(F):
         F.decimate(x, 1)  # the default design's cutoff 1 is the Nyquist endpoint
     with pytest.raises(ValueError):
         F.decimate(x, 2, -1)
+
+
+@pytest.mark.parametrize("n,nh,M", [(128, 32, 8), (1024, 64, 8), (16384, 256, 16), (1000, 3, 1), (999, 20, 2),
+                                    (4096, 128, 4), (4096, 1024, 32), (3000, 60, 6), (10000, 384, 12),
+                                    (65536, 1000, 100), (1 << 20, 8192, 1024), (5000, 0, 16), (7, 16, 8)])
+def test_channelize_poly(F, n, nh, M):
+    """Fused form (M in 1..16 powers of two) and the FIR + row FFT + transpose
+    form (others) vs the FP64 oracle."""
+    rng = np.random.default_rng(n + nh + M)
+    x = rng.standard_normal(n).astype(np.float32)
+    h = rng.standard_normal(nh).astype(np.float32)
+    print(f"channelize n={n} taps={nh} M={M}", flush=True)
+    y = F.channelize_poly(dev(x), dev(h), M).cpu().numpy()
+    ref = orc.channelize_poly(x, h, M)
+    assert y.shape == ref.shape
+    if ref.size:
+        assert orc.rel_l2(y, ref) < TOL
+
+
+def test_channelize_poly_probe_cases(F):
+    for n, nh, M, seed in ((128, 32, 8, 83010001), (1024, 64, 8, 83010002), (16384, 256, 16, 83010003)):
+        x = orc.channelize_probe_input(n, seed, False, "FP32")
+        h = orc.channelize_probe_input(nh, seed + 1, True, "FP32")
+        y = F.channelize_poly(dev(x), dev(h), M).cpu().numpy()
+        assert orc.rel_l2(y, orc.channelize_poly(x, h, M)) < 1e-6
+
+
+def test_channelize_poly_validation(F):
+    x = dev(np.ones(64, np.float32))
+    with pytest.raises(ValueError):
+        F.channelize_poly(x, dev(np.ones(8, np.float32)), 0)
+    with pytest.raises(ValueError):
+        F.channelize_poly(x, dev(np.ones(66, np.float32)), 2)  # 33 taps per phase

@@ -175,6 +175,10 @@ def test_typed_operators(name, tmp_path):
     val = next(b)
     ref = filtering.decimate(xf, qd, 24, axis=1, zero_phase=False, widen=False).cpu().numpy()
     assert np.array_equal(val.reshape(ref.shape), ref.astype(np.float64))
+    # channelize_poly, 4 channels: complex64 always
+    val = _c(next(b))
+    ref = filtering.channelize_poly(dev["xf"], dev["h"], 4).cpu().numpy()
+    assert np.array_equal(val.reshape(ref.shape), ref)
     # ambgfun of the template: 2-D FP32 for float/half, FP64 storage for
     # integers (radartools_typed.cpp:119-130); the cuts are FP64 storage
     flag, val = next(b)[0], next(b)

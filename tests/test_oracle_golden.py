@@ -122,6 +122,14 @@ def run_case(case):
                          filter_order=None if order < 0 else order, zero_phase=zero_phase, taps=taps,
                          fp32=not fp64)
         return manifest, _digest(y if fp64 else y.astype(np.float32))
+    if op == "channelize_poly":
+        xc, hc, M, seed = int(a["x-count"]), int(a["h-count"]), int(a["n-chans"]), int(a["seed"])
+        x = orc.channelize_probe_input(xc, seed, False, dt)
+        h = orc.channelize_probe_input(hc, seed + 1, True, dt)
+        manifest = (f"x:{dt}:[{x.size}]:bytes={x.nbytes}:sha256={_digest(x)}"
+                    f"|h:{dt}:[{h.size}]:bytes={h.nbytes}:sha256={_digest(h)}")
+        y = orc.channelize_poly(x.astype(np.float64), h.astype(np.float64), M)
+        return manifest, _digest(y)  # ComplexFloat values, channel-major
     if op == "firfilter":
         n, ntaps = int(a["n"]), int(a["taps"])
         x = orc.firfilter_probe_signal(n, 29, dt)

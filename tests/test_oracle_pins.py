@@ -315,3 +315,49 @@ def test_decimate_fp32_path_rounds_in_tap_order():
                 s = np.float32(s + np.float32(h[t]) * np.float32(x[src]))
         acc[o] = s
     assert np.array_equal(acc.astype(np.float64), y32)
+
+
+# ---------------------------------------------------------------------------
+# channelize_poly (filtering_typed.cpp:443-466 and its CPU form)
+# ---------------------------------------------------------------------------
+
+def _channelize_fft_form(x, h, M):
+    """An independent formulation: the per-phase FIR as array slicing and the
+    +i DFT as M times numpy's inverse FFT over the phase axis."""
+    T = h.size // M
+    P = x.size // M
+    xb = x[:P * M].reshape(P, M)[:, ::-1]          # xb[g, ph] = x[g*M + M-1-ph]
+    hb = h[:T * M].reshape(T, M) if T else np.zeros((0, M))
+    F = np.zeros((P, M))
+    for t in range(T):
+        F[t:, :] += xb[:P - t, :] * hb[t][None, :]
+    return (M * np.fft.ifft(F, axis=1)).T          # [M, P]
+
+
+@pytest.mark.parametrize("n,nh,M", [(128, 32, 8), (1000, 64, 8), (4096, 256, 16), (777, 30, 6), (96, 0, 4),
+                                    (1024, 1024, 32), (50, 7, 1), (4099, 100, 20)])
+def test_channelize_vs_fft_formulation(n, nh, M):
+    rng = np.random.default_rng(n + M)
+    x = rng.standard_normal(n)
+    h = rng.standard_normal(nh)
+    y = orc.channelize_poly(x, h, M)
+    ref = _channelize_fft_form(x, h, M)
+    assert y.shape == ref.shape == (M, n // M)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.allclose(y, ref, rtol=0, atol=2e-6 * scale)
+
+
+def test_channelize_single_tap_is_blockwise_dft():
+    """With one tap per phase equal to 1, each output column is the +i DFT of
+    the reversed input block."""
+    M = 8
+    x = np.arange(64, dtype=np.float64)
+    y = orc.channelize_poly(x, np.ones(M), M)
+    for p in range(8):
+        blk = x[p * M:(p + 1) * M][::-1]
+        assert np.allclose(y[:, p], M * np.fft.ifft(blk), atol=1e-4)
+
+
+def test_channelize_rejects_long_filters():
+    with pytest.raises(ValueError):
+        orc.channelize_poly(np.ones(64), np.ones(33 * 2), 2)

@@ -1,0 +1,177 @@
+// Polyphase channelizer for gfx950.
+//
+// Follows channelize_poly_device / channelize_poly_typed_cpu
+// (ZKX/cusignal_cpp/src/operators/filtering/filtering_typed.cpp:443-466 plan,
+//  the CPU form after it; cuSignal filtering/channelize_poly.py):
+//
+//   M = n_chans channels, T = floor(len(h)/M) <= 32 taps per phase,
+//   P = floor(N/M) points (a short last group of x and of h is ignored):
+//   F[ph][p] = sum_{t<T, p-t>=0} x[(p-t)M + M-1-ph] * h[ph + tM]
+//   Y[c][p]  = sum_{ph<M} F[ph][p] e^{+2 pi i c ph / M}       (channel-major)
+//
+// Two forms:
+//   * M in {1, 2, 4, 8, 16}: one thread per point filters all M phases and
+//     runs the M-point inverse DFT in registers (the twiddles are 16th roots
+//     of unity, folded at compile time), then stores Y[c][p] for every c --
+//     consecutive threads store consecutive points of a channel.
+//   * any other M: the filtered phases go to a [P][M] complex workspace, the
+//     library's row FFT inverts each row (LDS Stockham for powers of two,
+//     Bluestein otherwise), and a 32x32 LDS transpose writes Y channel-major.
+// All arithmetic is FP32; the reference's GPU path is FP32 as well (its CPU
+// form sums in FP64).
+#include "runtime.hpp"
+#include "fft_lds.hpp"
+#include "../../include/zkx_fft.h"
+
+#include <limits>
+
+namespace zkx {
+int fft_rows_dispatch(int n, const float2* x, long long rows, bool inverse, float scale, float2* y,
+                      hipStream_t st);
+}
+
+namespace {
+
+constexpr int kMaxTaps = 32;
+
+template <int M>
+__global__ void __launch_bounds__(256)
+channelize_small_kernel(const float* __restrict__ x, const float* __restrict__ h, int T, int P,
+                        float2* __restrict__ y)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float f[M];
+#pragma unroll
+    for (int ph = 0; ph < M; ++ph) f[ph] = 0.0f;
+    const int tmax = min(T, p + 1);
+    for (int t = 0; t < tmax; ++t) {
+        const float* xr = x + static_cast<long long>(p - t) * M;
+        const float* hr = h + static_cast<long long>(t) * M;
+#pragma unroll
+        for (int ph = 0; ph < M; ++ph) f[ph] = fmaf(xr[M - 1 - ph], hr[ph], f[ph]);
+    }
+    float2 v[M];
+#pragma unroll
+    for (int ph = 0; ph < M; ++ph) v[ph] = make_float2(f[ph], 0.0f);
+    zkx::fft::Dft<M, true>::run(v);
+#pragma unroll
+    for (int c = 0; c < M; ++c) y[static_cast<long long>(c) * P + p] = v[c];
+}
+
+// F[p][ph] as a complex row per point (imaginary part zero).
+__global__ void __launch_bounds__(256)
+channelize_fir_kernel(const float* __restrict__ x, const float* __restrict__ h, int M, int T, long long total,
+                      float2* __restrict__ ws)
+{
+    for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long p = e / M;
+        const int ph = static_cast<int>(e - p * M);
+        float acc = 0.0f;
+        const int tmax = static_cast<int>(min(static_cast<long long>(T), p + 1));
+        for (int t = 0; t < tmax; ++t)
+            acc = fmaf(x[(p - t) * M + (M - 1 - ph)], h[ph + static_cast<long long>(t) * M], acc);
+        ws[e] = make_float2(acc, 0.0f);
+    }
+}
+
+// y[c][p] = scale * g[p][c] through a 32x32 tile.
+__global__ void __launch_bounds__(256)
+channelize_transpose_kernel(const float2* __restrict__ g, int P, int M, float scale, float2* __restrict__ y)
+{
+    __shared__ float2 tile[32][33];
+    const int p0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32
+    for (int r = ty; r < 32; r += 8) {
+        const int p = p0 + r, c = c0 + tx;
+        if (p < P && c < M) tile[r][tx] = g[static_cast<long long>(p) * M + c];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int c = c0 + r, p = p0 + tx;
+        if (p < P && c < M) {
+            const float2 v = tile[tx][r];
+            y[static_cast<long long>(c) * P + p] = make_float2(v.x * scale, v.y * scale);
+        }
+    }
+}
+
+template <int M>
+int launch_small(const float* x, const float* h, int T, int P, float2* y, hipStream_t st)
+{
+    hipLaunchKernelGGL((channelize_small_kernel<M>), dim3((P + 255) / 256), dim3(256), 0, st, x, h, T, P, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+thread_local zkx::ThreadWorkspace t_chan_ws;
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int zkx_channelize_poly_f32(const float* x, long long x_count, const float* h, long long h_count, int n_chans,
+                            zkx_complex64* out, void* stream)
+{
+    zkx::clear_last_error();
+    if (n_chans <= 0) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "channelize_poly n_chans");
+    if (x_count < 0 || h_count < 0 || x_count > std::numeric_limits<int>::max() ||
+        h_count > std::numeric_limits<int>::max())
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "channelize_poly input exceeds int indexing");
+    const int M = n_chans;
+    const int T = static_cast<int>(h_count / M);
+    if (T > kMaxTaps) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "channelize_poly taps per phase exceed 32");
+    const int P = static_cast<int>(x_count / M);
+    const long long total = static_cast<long long>(M) * P;
+    if (total > std::numeric_limits<int>::max()) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "channelize_poly output exceeds int indexing");
+    if (total == 0) return ZKX_SUCCESS;
+    if (!x || !out || (T > 0 && !h)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "channelize_poly: null pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    float2* y = reinterpret_cast<float2*>(out);
+    switch (M) {
+    case 1: return launch_small<1>(x, h, T, P, y, st);
+    case 2: return launch_small<2>(x, h, T, P, y, st);
+    case 4: return launch_small<4>(x, h, T, P, y, st);
+    case 8: return launch_small<8>(x, h, T, P, y, st);
+    case 16: return launch_small<16>(x, h, T, P, y, st);
+    default: break;
+    }
+    if (!pow2(M) && M > 4096)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "channelize_poly channel counts that are not powers of two need M <= 4096");
+    if (M > 8192) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "channelize_poly needs M <= 8192");
+    void* ws = nullptr;
+    int rc = t_chan_ws.reserve(2 * sizeof(float2) * static_cast<size_t>(total), &ws);
+    if (rc) return rc;
+    float2* f = static_cast<float2*>(ws);
+    float2* g = f + total;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(channelize_fir_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, x, h, M, T,
+                       total, f);
+    ZKX_LAUNCH_CHECK();
+    float scale = 1.0f;
+    if (pow2(M)) {
+        rc = zkx::fft_rows_dispatch(M, f, P, true, 1.0f, g, st);  // unscaled inverse
+        if (rc) return rc;
+    } else {
+        FFTPlanHandle plan = nullptr;
+        if (fft_plan_create_1d(&plan, M, P, FFT_TYPE_C2C, FFT_INVERSE) != FFT_SUCCESS)
+            ZKX_FAIL(ZKX_BACKEND_ERROR, "channelize_poly FFT plan");
+        const FFTResult fr = fft_execute_on_stream(plan, g, f, stream);  // inverse scaled by 1/M
+        // the plan's tables must outlive the queued transform
+        const hipError_t se = hipStreamSynchronize(st);
+        fft_plan_destroy(plan);
+        ZKX_HIP_CHECK(se);
+        if (fr != FFT_SUCCESS) ZKX_FAIL(ZKX_BACKEND_ERROR, "channelize_poly FFT");
+        scale = static_cast<float>(M);
+    }
+    hipLaunchKernelGGL(channelize_transpose_kernel, dim3((P + 31) / 32, (M + 31) / 32), dim3(256), 0, st, g, P, M,
+                       scale, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+}  // extern "C"

@@ -237,8 +237,31 @@ def decimate(x, q, n=None, axis=-1, zero_phase=True, *, widen=True, stream=None)
     return y
 
 
-__all__ = ["upfirdn", "resample_poly", "firfilter", "decimate", "design_resample_poly_coefficients",
-           "design_decimate_filter"]
+def channelize_poly(x, h, n_chans, *, stream=None):
+    """channelize_poly_device (filtering_typed.h:513-523) in cusignal's call
+    form (cusignal/filtering/channelize_poly.py): polyphase filter bank of
+    n_chans channels, complex64 [n_chans, len(x) // n_chans] channel-major.
+    x and h are float32; for higher-rank arrays the first extent selects the
+    samples (the reference's len(x) rule)."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda or x.dtype != torch.float32:
+        raise ValueError("x must be a float32 device tensor")
+    n_chans = int(n_chans)
+    if n_chans <= 0:
+        raise ValueError("channelize_poly n_chans")
+    x = x.contiguous()
+    h = _taps(h)
+    nx = x.shape[0] if x.dim() > 0 else 1
+    nh = h.shape[0] if h.dim() > 0 else 1
+    y = torch.empty((n_chans, nx // n_chans), dtype=torch.complex64, device=x.device)
+    code = lib().zkx_channelize_poly_f32(_ptr(x), nx, _ptr(h), nh, n_chans, _ptr(y), _stream(stream))
+    if code in (1001, 1002):
+        raise ValueError(lib().zkx_last_error_message().decode())
+    check(code)
+    return y
+
+
+__all__ = ["upfirdn", "resample_poly", "firfilter", "decimate", "channelize_poly",
+           "design_resample_poly_coefficients", "design_decimate_filter"]
 
 
 def fftconvolve(in1, in2, mode="full", *, stream=None):
