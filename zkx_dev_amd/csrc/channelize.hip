@@ -10,10 +10,11 @@
 //   Y[c][p]  = sum_{ph<M} F[ph][p] e^{+2 pi i c ph / M}       (channel-major)
 //
 // Two forms:
-//   * M in {1, 2, 4, 8, 16}: one thread per point filters all M phases and
-//     runs the M-point inverse DFT in registers (the twiddles are 16th roots
-//     of unity, folded at compile time), then stores Y[c][p] for every c --
-//     consecutive threads store consecutive points of a channel.
+//   * M in {1, 2, 4, 8, 16}: one thread per point filters all M phases from
+//     an LDS copy of the block's input rows and runs the M-point inverse DFT
+//     in registers (the twiddles are 16th roots of unity, folded at compile
+//     time), then stores Y[c][p] for every c -- consecutive threads store
+//     consecutive points of a channel.
 //   * any other M: the filtered phases go to a [P][M] complex workspace, the
 //     library's row FFT inverts each row (LDS Stockham for powers of two,
 //     Bluestein otherwise), and a 32x32 LDS transpose writes Y channel-major.
@@ -34,20 +35,41 @@ namespace {
 
 constexpr int kMaxTaps = 32;
 
+// A block covers 256 consecutive points: the x groups they need (T - 1
+// before the first point onwards) and the taps are staged in LDS once, rows
+// padded to M + 1 floats so that the lanes (consecutive points) hit distinct
+// banks.  Groups before the start of x are zero: their products add exact
+// zeros, so every sum equals the one that skips them.
+constexpr int kSmallPoints = 256;
+
 template <int M>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSmallPoints)
 channelize_small_kernel(const float* __restrict__ x, const float* __restrict__ h, int T, int P,
                         float2* __restrict__ y)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+    extern __shared__ float sm[];
+    constexpr int RS = M + 1;
+    const int rows = kSmallPoints + T - 1;
+    float* xs = sm;               // [rows][RS]
+    float* hs = sm + rows * RS;   // [T*M]
+    const int p0 = blockIdx.x * kSmallPoints;
+    const long long g0 = static_cast<long long>(p0) - (T - 1);
+    for (int e = threadIdx.x; e < rows * M; e += kSmallPoints) {
+        const int r = e / M, j = e - r * M;
+        const long long g = g0 + r;
+        xs[r * RS + j] = (g >= 0 && g < P) ? x[g * M + j] : 0.0f;
+    }
+    for (int e = threadIdx.x; e < T * M; e += kSmallPoints) hs[e] = h[e];
+    __syncthreads();
+    const int l = threadIdx.x;
+    const int p = p0 + l;
+    if (p >= P) return;  // after the only barrier
     float f[M];
 #pragma unroll
     for (int ph = 0; ph < M; ++ph) f[ph] = 0.0f;
-    const int tmax = min(T, p + 1);
-    for (int t = 0; t < tmax; ++t) {
-        const float* xr = x + static_cast<long long>(p - t) * M;
-        const float* hr = h + static_cast<long long>(t) * M;
+    for (int t = 0; t < T; ++t) {
+        const float* xr = xs + (l + T - 1 - t) * RS;
+        const float* hr = hs + t * M;
 #pragma unroll
         for (int ph = 0; ph < M; ++ph) f[ph] = fmaf(xr[M - 1 - ph], hr[ph], f[ph]);
     }
@@ -100,7 +122,9 @@ channelize_transpose_kernel(const float2* __restrict__ g, int P, int M, float sc
 template <int M>
 int launch_small(const float* x, const float* h, int T, int P, float2* y, hipStream_t st)
 {
-    hipLaunchKernelGGL((channelize_small_kernel<M>), dim3((P + 255) / 256), dim3(256), 0, st, x, h, T, P, y);
+    const size_t lds = sizeof(float) * (static_cast<size_t>(kSmallPoints + T - 1) * (M + 1) + static_cast<size_t>(T) * M);
+    hipLaunchKernelGGL((channelize_small_kernel<M>), dim3((P + kSmallPoints - 1) / kSmallPoints), dim3(kSmallPoints),
+                       lds, st, x, h, T, P, y);
     ZKX_LAUNCH_CHECK();
     return ZKX_SUCCESS;
 }
