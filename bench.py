@@ -64,7 +64,7 @@ def bench_filters(args):
     HBM fraction for each."""
     import torch
     from scipy import signal
-    from zkx_dev_amd import filtering, synth
+    from zkx_dev_amd import convolution, filtering, synth
 
     dev = torch.device("cuda:0")
     x = torch.from_numpy(synth.complex_noise_tones(16, 1 << 20)).to(dev)
@@ -97,6 +97,9 @@ def bench_filters(args):
                                         (1 << 24) * 4 + (1 << 24) * 8 + 512 * 4, (1 << 24) * 32 * 2),
         "channelize_poly_64ch_32taps": (lambda: filtering.channelize_poly(xr, h_ch64, 64),
                                         (1 << 24) * 4 + (1 << 24) * 8 + 2048 * 4, (1 << 24) * 32 * 2),
+        # direct 1-D correlate of the 2^24 samples with the 255-tap lowpass ('same')
+        "correlate_direct_same_255": (lambda: convolution.correlate(xr, h_lp, "same"),
+                                      (1 << 24) * 8 + 255 * 4, (1 << 24) * 255 * 2),
     }
     for name, (fn, nbytes, flops) in cases.items():
         for _ in range(args.warmup):

@@ -480,6 +480,20 @@ int zkx_channelize_poly_f32(const float* x, long long x_count, const float* h, l
                             zkx_complex64* out, void* stream);
 
 /*
+ * 1-D cross-correlation, direct method, in the element type `dtype`
+ * (ZKX_DTYPE_*), out in the same type.  Replaces correlate_device(in1, in2,
+ * out, mode) (convolution_typed.h:298-303; convolution_typed.cpp:134-152,
+ * 285-334).  mode 0 full (L1+L2-1), 1 same (L1), 2 valid (|L1-L2|+1); s =
+ * (full - n)/2 + k:  out[k] = sum_j in1[s - (L2-1-j)] in2[j].  float/half: one
+ * FP32 fma chain in j order (half stored rounded); integers: products summed
+ * modulo 2^bits.  Bit-identical to the reference.  out must not overlap an
+ * input.  zkx_correlate_out_len gives n (-1 for an invalid mode).
+ */
+long long zkx_correlate_out_len(long long l1, long long l2, int mode);
+int zkx_correlate_direct(const void* in1, long long l1, const void* in2, long long l2, int dtype, int mode,
+                         void* out, void* stream);
+
+/*
  * Host-side FIR design for decimate's default filter: firwin with one
  * lowpass cutoff (fs = 2, pass_zero, scale) and a Hamming window, FP64 taps
  * (filter_design_typed.cpp:24-100, 268-296; decimate designs numtaps =

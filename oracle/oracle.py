@@ -58,6 +58,8 @@ def lib():
             "orc_firwin_lowpass_hamming": (i, [i, d, P]),
             "orc_decimate": (i, [P, i, i, i, i, P, i, i, i, P]),
             "orc_channelize_poly": (i, [P, ctypes.c_longlong, P, ctypes.c_longlong, i, P]),
+            "orc_correlate_len": (ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong, i]),
+            "orc_correlate_direct": (i, [P, ctypes.c_longlong, P, ctypes.c_longlong, i, i, P]),
             "orc_cfar_alpha": (d, [d, i]),
             "orc_ca_cfar_count": (i, [i, i, i, i, i]),
             "orc_ca_cfar_alpha_f": (ctypes.c_float, [i, i, i, i, i, d]),
@@ -219,6 +221,44 @@ def channelize_poly(x, h, n_chans):
     if rc:
         raise ValueError(f"channelize_poly rc={rc}")
     return out.view(np.complex64).reshape(M, P)
+
+
+CORRELATE_MODES = {"full": 0, "same": 1, "valid": 2}
+_INT_BITS = {"INT32": 32, "INT16": 16, "INT8": 8}
+
+
+def correlate_probe_signal(count, salt, dtype):
+    """ZKX/test_all/operators/suites/task2_features/task2_features_probe.cu:120-131:
+    4 sin(0.031 i) + 2 cos(0.071 i) + ((17 i + salt) % 13 - 6)/8, rounded half
+    away from zero for integer types, then cast (libm sin/cos through Python's
+    math module)."""
+    import math
+    raw = np.array([4.0 * math.sin(0.031 * i) + 2.0 * math.cos(0.071 * i) + ((i * 17 + salt) % 13 - 6) * 0.125
+                    for i in range(count)], np.float64)
+    if INTEGRAL[dtype]:
+        return np.where(raw >= 0, np.floor(raw + 0.5), -np.floor(-raw + 0.5)).astype(NP_DTYPE[dtype])
+    if dtype == "FP16":
+        return raw.astype(np.float32).astype(np.float16)
+    return raw.astype(np.float32)
+
+
+def correlate(in1, in2, mode="full", dtype="FP32"):
+    """correlate_typed_cpu direct (convolution_typed.cpp:285-334) for 1-D
+    inputs of element type `dtype`: returns the output in that type (float32,
+    float16 rounded from the FP32 sum, or the integer wrapped modulo 2^bits)."""
+    a = np.ascontiguousarray(np.asarray(in1).reshape(-1).astype(np.float64))
+    b = np.ascontiguousarray(np.asarray(in2).reshape(-1).astype(np.float64))
+    m = CORRELATE_MODES[mode]
+    n = lib().orc_correlate_len(a.size, b.size, m)
+    out = np.empty(max(n, 0), np.float64)
+    rc = lib().orc_correlate_direct(_p(a), a.size, _p(b), b.size, m, _INT_BITS.get(dtype, 0), _p(out))
+    if rc:
+        raise ValueError(f"correlate rc={rc}")
+    if INTEGRAL[dtype]:
+        return out.astype(NP_DTYPE[dtype])
+    if dtype == "FP16":
+        return out.astype(np.float32).astype(np.float16)
+    return out.astype(np.float32)
 
 
 def firwin_lowpass_hamming(numtaps, cutoff):

@@ -30,6 +30,8 @@
  *     :1536-1592 firfilter_typed_cpu, :2548-2616 upfirdn_typed_cpu,
  *     :2617-2715 resample_poly_typed_cpu, :318-336 + :2229-2317 decimate_typed_cpu
  *     :443-466 prepare_channelize_poly + channelize_poly_typed_cpu
+ *   ZKX/cusignal_cpp/src/operators/convolution/convolution_typed.cpp
+ *     :134-152 correlate_output_size, :285-334 correlate_typed_cpu (direct)
  *   ZKX/cusignal_cpp/src/operators/filter_design/filter_design_typed.cpp
  *     :14-16 sinc, :24-100 prepare_firwin, :268-296 firwin_typed_cpu
  *   Probe input generators:
@@ -328,6 +330,59 @@ ORC_API int orc_channelize_poly(const double *x, long long n, const double *h, l
             }
             out[2 * ((size_t)c * (size_t)P + (size_t)p)] = (float)re;
             out[2 * ((size_t)c * (size_t)P + (size_t)p) + 1] = (float)im;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* correlate (1-D, direct)                                             */
+/* ------------------------------------------------------------------ */
+
+/* convolution_typed.cpp:134-152 output length, :285-334 correlate_typed_cpu
+ * (direct): out[k] = sum_{j<L2} in1[s - (L2-1-j)] in2[j] over the terms inside
+ * in1, s = (full - out)/2 + k, full = L1 + L2 - 1.  kind 0: FP32 fma chain in j
+ * order (float, and half loaded to float); kind 8/16/32: integer inputs, the
+ * products summed modulo 2^kind in the unsigned type, returned as the signed
+ * value of those bits.  mode 0 full, 1 same, 2 valid.  out gets the value
+ * before the final store to the element type (float results; integers exact). */
+ORC_API long long orc_correlate_len(long long l1, long long l2, int mode)
+{
+    if (l1 == 0 || l2 == 0) return 0;
+    if (mode == 0) return l1 + l2 - 1;
+    if (mode == 1) return l1;
+    return l1 >= l2 ? l1 - l2 + 1 : l2 - l1 + 1;
+}
+
+ORC_API int orc_correlate_direct(const double *in1, long long l1, const double *in2, long long l2, int mode,
+                                 int kind, double *out)
+{
+    if (mode < 0 || mode > 2) return 1001;
+    const long long n = orc_correlate_len(l1, l2, mode);
+    if (n == 0) return 0;
+    const long long full = l1 + l2 - 1;
+    const long long start = (full - n) / 2;
+    for (long long k = 0; k < n; ++k) {
+        const long long s = start + k;
+        float acc = 0.0f;
+        uint32_t macc = 0;
+        for (long long j = 0; j < l2; ++j) {
+            const long long src = s - (l2 - 1 - j);
+            if (src < 0 || src >= l1) continue;
+            if (kind == 0) {
+                acc = fmaf((float)in1[src], (float)in2[j], acc);
+            } else {
+                const uint32_t a = (uint32_t)(int64_t)in1[src], b = (uint32_t)(int64_t)in2[j];
+                macc = macc + a * b;
+            }
+        }
+        if (kind == 0) {
+            out[k] = (double)acc;
+        } else {
+            const uint32_t mask = kind == 32 ? 0xFFFFFFFFu : ((1u << kind) - 1u);
+            const uint32_t bits = macc & mask;
+            const uint32_t sign = 1u << (kind - 1);
+            out[k] = (bits & sign) ? (double)((int64_t)bits - ((int64_t)1 << kind)) : (double)bits;
         }
     }
     return 0;

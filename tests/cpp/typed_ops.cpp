@@ -2,7 +2,7 @@
 // for one element type of the reference's instantiations (float, __half,
 // int32, int16, int8; radartools_typed.cpp:810-811, filtering_typed.cu:683+),
 // plus the complex filtering extensions, fftconvolve, decimate,
-// channelize_poly and ambgfun.  Built with g++ and
+// channelize_poly, correlate and ambgfun.  Built with g++ and
 // linked to libzkx_radar.so.
 //
 // usage: typed_ops IN OUT
@@ -16,6 +16,7 @@
 #include <iostream>
 #include <vector>
 
+#include "cusignal/operators/convolution/convolution_typed.h"
 #include "cusignal/operators/radartools/radartools_typed.h"
 
 namespace {
@@ -169,7 +170,17 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     cusignal::ChannelizePolyDeviceResult cp;
     cusignal::channelize_poly_device<T>(d_xf, d_h, 4, cp);
     w.vec(0, cp.values.to_host());
-    // 18-20: ambiguity function of the template: 2-D (FP32 for float and half,
+    // 18-20: correlate (direct) of the first filtering line with the taps,
+    // full / same / valid, in the input type
+    {
+        const auto d_x1 = DeviceArray<T>::from_host(std::vector<T>(xf.begin(), xf.begin() + n));
+        for (const char* mode : {"full", "same", "valid"}) {
+            DeviceArray<T> co(cusignal::correlate_output_size(static_cast<std::size_t>(n), h.size(), mode));
+            cusignal::correlate_device<T>(d_x1, d_h, co, mode);
+            w.vec(3, co.to_host());
+        }
+    }
+    // 21-23: ambiguity function of the template: 2-D (FP32 for float and half,
     // FP64 storage for integers), delay cut and Doppler cut (FP64 storage)
     cusignal::AmbgfunOptions ao;
     ao.fs = 1000.0;
@@ -188,7 +199,7 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     cusignal::ambgfun_device<T>(d_t, ar, ao);
     w.vec(1, ar.fp64.to_host());
     if constexpr (std::is_same_v<T, float>) {
-        // 21-23: complex filtering extensions on the pulses (float taps)
+        // 24-26: complex filtering extensions on the pulses (float taps)
         cusignal::UpfirdnComplexDeviceResult uc;
         cusignal::upfirdn_device<float>(d_h, d_x, up, down, uc, {{P, R}, 1});
         w.vec(0, uc.fp32.to_host());

@@ -13,6 +13,7 @@ Probe programs that produced these files:
   firfilter:  ZKX/test_all/operators/shared/filtering/task2_filter_probe.cu
   decimate:   ZKX/test_all/operators/cases/filtering/decimate/operator_probe.cu
   channelize: ZKX/test_all/operators/cases/filtering/channelize_poly/operator_probe.cu
+  correlate:  ZKX/test_all/operators/suites/task2_features/task2_features_probe.cu
 
 Run in a container where /root/reference exists:
     python tests/golden/extract_probe_digests.py
@@ -25,7 +26,7 @@ import sys
 
 REF = "/root/reference"
 OPS = ["pulse_compression", "pulse_doppler", "ca_cfar", "cfar_alpha", "ambgfun",
-       "upfirdn", "resample_poly", "firfilter", "decimate", "channelize_poly"]
+       "upfirdn", "resample_poly", "firfilter", "decimate", "channelize_poly", "correlate"]
 KEEP = ["requested_dtype", "output_dtype", "typed_input_manifest",
         "actual_input_digest", "cpu_output_digest", "gpu_output_digest",
         "output_shape", "relative_l2", "relative_linf"]

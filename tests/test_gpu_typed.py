@@ -179,6 +179,12 @@ def test_typed_operators(name, tmp_path):
     val = _c(next(b))
     ref = filtering.channelize_poly(dev["xf"], dev["h"], 4).cpu().numpy()
     assert np.array_equal(val.reshape(ref.shape), ref)
+    # correlate (direct) of the first line with the taps: the reference's
+    # arithmetic in the input type, bit for bit (the oracle reproduces its digests)
+    for mode in ("full", "same", "valid"):
+        val = next(b)
+        ref = orc.correlate(inp["xf"][:N], inp["h"], mode, name)
+        assert np.array_equal(val.view(np.uint8), ref.view(np.uint8)), (name, mode)
     # ambgfun of the template: 2-D FP32 for float/half, FP64 storage for
     # integers (radartools_typed.cpp:119-130); the cuts are FP64 storage
     flag, val = next(b)[0], next(b)

@@ -130,6 +130,13 @@ def run_case(case):
                     f"|h:{dt}:[{h.size}]:bytes={h.nbytes}:sha256={_digest(h)}")
         y = orc.channelize_poly(x.astype(np.float64), h.astype(np.float64), M)
         return manifest, _digest(y)  # ComplexFloat values, channel-major
+    if op == "correlate":
+        n, m = int(a["count"]), int(a["reference-count"])
+        x = orc.correlate_probe_signal(n, 41, dt)
+        r = orc.correlate_probe_signal(m, 7, dt)
+        manifest = (f"signal:{dt}:[{x.size}]:sha256={_digest(x)}"
+                    f"|reference:{dt}:[{r.size}]:sha256={_digest(r)}")
+        return manifest, _digest(orc.correlate(x, r, "same", dt))  # output in the input type
     if op == "firfilter":
         n, ntaps = int(a["n"]), int(a["taps"])
         x = orc.firfilter_probe_signal(n, 29, dt)

@@ -55,6 +55,8 @@ SIGNATURES = {
     "zkx_firfilter_f32": (I, [P, I, P, I, I, I, P, P, P, P]),
     "zkx_decimate_f32": (I, [P, I, P, I, I, I, I, I, P, P]),
     "zkx_channelize_poly_f32": (I, [P, LL, P, LL, I, P, P]),
+    "zkx_correlate_out_len": (LL, [LL, LL, I]),
+    "zkx_correlate_direct": (I, [P, LL, P, LL, I, I, P, P]),
     "zkx_firwin_lowpass_hamming": (I, [I, ctypes.c_float, P]),
     "zkx_fftconvolve_out_len": (LL, [LL, LL, I]),
     "zkx_fftconvolve_c64": (I, [P, I, I, P, I, I, P, P]),
