@@ -267,7 +267,8 @@ def test_decimate_validation(F):
 
 @pytest.mark.parametrize("n,nh,M", [(128, 32, 8), (1024, 64, 8), (16384, 256, 16), (1000, 3, 1), (999, 20, 2),
                                     (4096, 128, 4), (4096, 1024, 32), (3000, 60, 6), (10000, 384, 12),
-                                    (65536, 1000, 100), (1 << 20, 8192, 1024), (5000, 0, 16), (7, 16, 8)])
+                                    (65536, 1000, 100), (1 << 20, 8192, 1024), (5000, 0, 16), (7, 16, 8),
+                                    (65536, 2048, 64), (70021, 200, 24)])
 def test_channelize_poly(F, n, nh, M):
     """Fused form (M in 1..16 powers of two) and the FIR + row FFT + transpose
     form (others) vs the FP64 oracle."""

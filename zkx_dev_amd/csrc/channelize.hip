@@ -81,21 +81,59 @@ channelize_small_kernel(const float* __restrict__ x, const float* __restrict__ h
     for (int c = 0; c < M; ++c) y[static_cast<long long>(c) * P + p] = v[c];
 }
 
-// F[p][ph] as a complex row per point (imaginary part zero).
+// F[p][ph] as a complex row per point (imaginary part zero).  A thread runs
+// one phase over kRun consecutive points: the terms x[(p - t)M + M-1-ph]
+// h[ph + tM] of outputs p0..p0+kRun-1 for one t are kRun consecutive rows of
+// the phase's column, so a window of kRun rows slid down one row per tap
+// feeds kRun chains per load (lanes take consecutive phases: coalesced).
+// Rows before the start of x are skipped per term, as the reference skips
+// them; rows past P only feed outputs that are not stored.
+constexpr int kRun = 16;
+
 __global__ void __launch_bounds__(256)
-channelize_fir_kernel(const float* __restrict__ x, const float* __restrict__ h, int M, int T, long long total,
-                      float2* __restrict__ ws)
+channelize_fir_kernel(const float* __restrict__ x, const float* __restrict__ h, int M, int T, int P,
+                      long long threads, float2* __restrict__ ws)
 {
-    for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < total;
-         e += static_cast<long long>(gridDim.x) * blockDim.x) {
-        const long long p = e / M;
-        const int ph = static_cast<int>(e - p * M);
-        float acc = 0.0f;
-        const int tmax = static_cast<int>(min(static_cast<long long>(T), p + 1));
-        for (int t = 0; t < tmax; ++t)
-            acc = fmaf(x[(p - t) * M + (M - 1 - ph)], h[ph + static_cast<long long>(t) * M], acc);
-        ws[e] = make_float2(acc, 0.0f);
+    const long long e = blockIdx.x * 256LL + threadIdx.x;
+    if (e >= threads) return;
+    const int run = static_cast<int>(e / M);
+    const int ph = static_cast<int>(e - static_cast<long long>(run) * M);
+    const int p0 = run * kRun;
+    const float* col = x + (M - 1 - ph);
+    float w[kRun], acc[kRun];
+#pragma unroll
+    for (int i = 0; i < kRun; ++i) {
+        w[i] = col[static_cast<long long>(min(p0 + i, P - 1)) * M];
+        acc[i] = 0.0f;
     }
+    if (p0 >= T - 1) {
+        // every row p0 + i - t (t < T) is inside x
+        for (int t = 0; t < T; ++t) {
+            const float ht = h[ph + static_cast<long long>(t) * M];
+            const float nx = col[static_cast<long long>(max(p0 - t - 1, 0)) * M];
+#pragma unroll
+            for (int i = 0; i < kRun; ++i) acc[i] = fmaf(w[i], ht, acc[i]);
+#pragma unroll
+            for (int i = kRun - 1; i > 0; --i) w[i] = w[i - 1];
+            w[0] = nx;
+        }
+    } else {
+        for (int t = 0; t < T; ++t) {
+            const float ht = h[ph + static_cast<long long>(t) * M];
+            const float nx = col[static_cast<long long>(max(p0 - t - 1, 0)) * M];
+#pragma unroll
+            for (int i = 0; i < kRun; ++i) {
+                const float y = fmaf(w[i], ht, acc[i]);
+                acc[i] = (p0 + i - t >= 0) ? y : acc[i];
+            }
+#pragma unroll
+            for (int i = kRun - 1; i > 0; --i) w[i] = w[i - 1];
+            w[0] = nx;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kRun; ++i)
+        if (p0 + i < P) ws[static_cast<long long>(p0 + i) * M + ph] = make_float2(acc[i], 0.0f);
 }
 
 // y[c][p] = scale * g[p][c] through a 32x32 tile.
@@ -171,10 +209,9 @@ int zkx_channelize_poly_f32(const float* x, long long x_count, const float* h, l
     if (rc) return rc;
     float2* f = static_cast<float2*>(ws);
     float2* g = f + total;
-    long long blocks = (total + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(channelize_fir_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, x, h, M, T,
-                       total, f);
+    const long long fir_threads = static_cast<long long>(M) * ((P + kRun - 1) / kRun);
+    hipLaunchKernelGGL(channelize_fir_kernel, dim3(static_cast<unsigned>((fir_threads + 255) / 256)), dim3(256), 0, st,
+                       x, h, M, T, P, fir_threads, f);
     ZKX_LAUNCH_CHECK();
     float scale = 1.0f;
     if (pow2(M)) {
