@@ -268,9 +268,11 @@ def test_decimate_validation(F):
 @pytest.mark.parametrize("n,nh,M", [(128, 32, 8), (1024, 64, 8), (16384, 256, 16), (1000, 3, 1), (999, 20, 2),
                                     (4096, 128, 4), (4096, 1024, 32), (3000, 60, 6), (10000, 384, 12),
                                     (65536, 1000, 100), (1 << 20, 8192, 1024), (5000, 0, 16), (7, 16, 8),
-                                    (65536, 2048, 64), (70021, 200, 24)])
+                                    (65536, 2048, 64), (70021, 200, 24), (40000, 1280, 128), (100000, 8192, 256),
+                                    (3000, 100, 32)])
 def test_channelize_poly(F, n, nh, M):
-    """Fused form (M in 1..16 powers of two) and the FIR + row FFT + transpose
+    """Fused forms (M in 1..16 with the DFT in registers; M in 32..256 with
+    the FFT in LDS; partial last blocks) and the FIR + row FFT + transpose
     form (others) vs the FP64 oracle."""
     rng = np.random.default_rng(n + nh + M)
     x = rng.standard_normal(n).astype(np.float32)

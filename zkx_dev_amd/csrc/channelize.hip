@@ -90,17 +90,11 @@ channelize_small_kernel(const float* __restrict__ x, const float* __restrict__ h
 // them; rows past P only feed outputs that are not stored.
 constexpr int kRun = 16;
 
-__global__ void __launch_bounds__(256)
-channelize_fir_kernel(const float* __restrict__ x, const float* __restrict__ h, int M, int T, int P,
-                      long long threads, float2* __restrict__ ws)
+__device__ __forceinline__ void fir_run(const float* __restrict__ x, const float* __restrict__ h, int M, int T,
+                                        int P, int ph, int p0, float (&acc)[kRun])
 {
-    const long long e = blockIdx.x * 256LL + threadIdx.x;
-    if (e >= threads) return;
-    const int run = static_cast<int>(e / M);
-    const int ph = static_cast<int>(e - static_cast<long long>(run) * M);
-    const int p0 = run * kRun;
     const float* col = x + (M - 1 - ph);
-    float w[kRun], acc[kRun];
+    float w[kRun];
 #pragma unroll
     for (int i = 0; i < kRun; ++i) {
         w[i] = col[static_cast<long long>(min(p0 + i, P - 1)) * M];
@@ -131,9 +125,67 @@ channelize_fir_kernel(const float* __restrict__ x, const float* __restrict__ h, 
             w[0] = nx;
         }
     }
+}
+
+__global__ void __launch_bounds__(256)
+channelize_fir_kernel(const float* __restrict__ x, const float* __restrict__ h, int M, int T, int P,
+                      long long threads, float2* __restrict__ ws)
+{
+    const long long e = blockIdx.x * 256LL + threadIdx.x;
+    if (e >= threads) return;
+    const int run = static_cast<int>(e / M);
+    const int ph = static_cast<int>(e - static_cast<long long>(run) * M);
+    const int p0 = run * kRun;
+    float acc[kRun];
+    fir_run(x, h, M, T, P, ph, p0, acc);
 #pragma unroll
     for (int i = 0; i < kRun; ++i)
         if (p0 + i < P) ws[static_cast<long long>(p0 + i) * M + ph] = make_float2(acc[i], 0.0f);
+}
+
+// M in {32, 64, 128, 256}: the whole general form in one block of 256
+// threads for FPB = 4096/M points.  The FIR (a thread per phase and run of
+// 16 points) writes the real rows into each point's LDS slot; each row's
+// M-point unscaled inverse FFT runs in LDS (M/16 threads per row); the
+// spectra go through an LDS [M][FPB+1] tile so that the stores run along
+// points (channel-major output).  One HBM read of x, one write of y.
+template <int M>
+__global__ void __launch_bounds__(256)
+channelize_fused_kernel(const float* __restrict__ x, const float* __restrict__ h, int T, int P,
+                        const float2* __restrict__ tw, float2* __restrict__ y)
+{
+    using C = zkx::fft::RowCfg<M>;
+    static_assert(C::THREADS == 256 && C::FPB % kRun == 0 && (C::FPB / kRun) * M == 256, "launch shape");
+    static_assert(M * (C::FPB + 1) <= C::SLOT_FLOATS * C::FPB, "transpose tile fits the FFT slots");
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    const int p0 = blockIdx.x * C::FPB;
+    {
+        const int ph = tid % M, run = tid / M;
+        float acc[kRun];
+        fir_run(x, h, M, T, P, ph, p0 + run * kRun, acc);
+#pragma unroll
+        for (int i = 0; i < kRun; ++i) lds[(run * kRun + i) * 2 * C::SLOT_FLOATS + ph] = acc[i];
+    }
+    __syncthreads();
+    const int slot = tid / C::T, t = tid % C::T;
+    float* base = lds + slot * 2 * C::SLOT_FLOATS;
+    zkx::fft::RowLds L{base, base + C::SLOT_FLOATS};
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) v[k] = make_float2(base[t + k * C::T], 0.0f);
+    zkx::fft::transform<M, C::E, true>(v, L, tw, t);  // syncs before its first LDS store
+    __syncthreads();                                  // the last pass's LDS reads are done
+    float2* yt = reinterpret_cast<float2*>(lds);      // [M][FPB + 1]
+    constexpr int YS = C::FPB + 1;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) yt[(t + k * C::T) * YS + slot] = v[k];
+    __syncthreads();
+#pragma unroll 4
+    for (int e = tid; e < M * C::FPB; e += 256) {
+        const int c = e / C::FPB, s = e % C::FPB;
+        if (p0 + s < P) y[static_cast<long long>(c) * P + p0 + s] = yt[c * YS + s];
+    }
 }
 
 // y[c][p] = scale * g[p][c] through a 32x32 tile.
@@ -155,6 +207,19 @@ channelize_transpose_kernel(const float2* __restrict__ g, int P, int M, float sc
             y[static_cast<long long>(c) * P + p] = make_float2(v.x * scale, v.y * scale);
         }
     }
+}
+
+template <int M>
+int launch_fused(const float* x, const float* h, int T, int P, float2* y, hipStream_t st)
+{
+    using C = zkx::fft::RowCfg<M>;
+    const float2* tw = nullptr;
+    const int rc = zkx::twiddles(M, &tw);
+    if (rc) return rc;
+    hipLaunchKernelGGL((channelize_fused_kernel<M>), dim3((P + C::FPB - 1) / C::FPB), dim3(256), C::LDS_BYTES, st, x,
+                       h, T, P, tw, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
 }
 
 template <int M>
@@ -199,6 +264,10 @@ int zkx_channelize_poly_f32(const float* x, long long x_count, const float* h, l
     case 4: return launch_small<4>(x, h, T, P, y, st);
     case 8: return launch_small<8>(x, h, T, P, y, st);
     case 16: return launch_small<16>(x, h, T, P, y, st);
+    case 32: return launch_fused<32>(x, h, T, P, y, st);
+    case 64: return launch_fused<64>(x, h, T, P, y, st);
+    case 128: return launch_fused<128>(x, h, T, P, y, st);
+    case 256: return launch_fused<256>(x, h, T, P, y, st);
     default: break;
     }
     if (!pow2(M) && M > 4096)
