@@ -263,7 +263,7 @@ int zkx_channelize_poly_f32(const float* x, long long x_count, const float* h, l
     case 2: return launch_small<2>(x, h, T, P, y, st);
     case 4: return launch_small<4>(x, h, T, P, y, st);
     case 8: return launch_small<8>(x, h, T, P, y, st);
-    case 16: return launch_small<16>(x, h, T, P, y, st);
+    case 16: return launch_fused<16>(x, h, T, P, y, st);
     case 32: return launch_fused<32>(x, h, T, P, y, st);
     case 64: return launch_fused<64>(x, h, T, P, y, st);
     case 128: return launch_fused<128>(x, h, T, P, y, st);
