@@ -9,12 +9,15 @@
 //   F[ph][p] = sum_{t<T, p-t>=0} x[(p-t)M + M-1-ph] * h[ph + tM]
 //   Y[c][p]  = sum_{ph<M} F[ph][p] e^{+2 pi i c ph / M}       (channel-major)
 //
-// Two forms:
-//   * M in {1, 2, 4, 8, 16}: one thread per point filters all M phases from
-//     an LDS copy of the block's input rows and runs the M-point inverse DFT
-//     in registers (the twiddles are 16th roots of unity, folded at compile
-//     time), then stores Y[c][p] for every c -- consecutive threads store
-//     consecutive points of a channel.
+// Three forms:
+//   * M in {1, 2, 4, 8}: one thread per point filters all M phases from an
+//     LDS copy of the block's input rows and runs the M-point inverse DFT in
+//     registers (the twiddles are folded at compile time), then stores
+//     Y[c][p] for every c -- consecutive threads store consecutive points of
+//     a channel.
+//   * M in {16, 32, 64, 128, 256}: one kernel filters by phase over runs of
+//     points, inverts each point's row with the LDS FFT and stores through an
+//     LDS transpose tile (no workspace).
 //   * any other M: the filtered phases go to a [P][M] complex workspace, the
 //     library's row FFT inverts each row (LDS Stockham for powers of two,
 //     Bluestein otherwise), and a 32x32 LDS transpose writes Y channel-major.
