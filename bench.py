@@ -60,8 +60,11 @@ def bench_filters(args):
     custom taps; upfirdn up=1 down=4 with a 255-tap lowpass; real firfilter
     on 2^24 uniform[-1,1] samples with the same 255 taps; decimate by 4 of
     those samples with its designed 81-tap filter (designed per call and as
-    resident custom taps).  Reports per-call time, algorithmic bytes and the
-    HBM fraction for each."""
+    resident custom taps); channelize_poly of them into 16 and 64 channels
+    (32 taps per phase); their direct correlate with the 255 taps; and
+    fftconvolve ('same') of the 16 complex channels with a 1025-tap lowpass.
+    Reports per-call time, algorithmic bytes, the HBM fraction and the
+    direct-form flop rate for each."""
     import torch
     from scipy import signal
     from zkx_dev_amd import convolution, filtering, synth
@@ -75,6 +78,7 @@ def bench_filters(args):
     h_dec = torch.from_numpy(filtering.design_decimate_filter(4).astype(np.float32)).to(dev)
     h_ch16 = torch.from_numpy(signal.firwin(512, 1.0 / 16).astype(np.float32)).to(dev)
     h_ch64 = torch.from_numpy(signal.firwin(2048, 1.0 / 64).astype(np.float32)).to(dev)
+    h_1025 = torch.from_numpy(signal.firwin(1025, 0.1).astype(np.complex64)).to(dev)
     stream = torch.cuda.current_stream()
     out = {}
     cases = {
@@ -100,6 +104,10 @@ def bench_filters(args):
         # direct 1-D correlate of the 2^24 samples with the 255-tap lowpass ('same')
         "correlate_direct_same_255": (lambda: convolution.correlate(xr, h_lp, "same"),
                                       (1 << 24) * 8 + 255 * 4, (1 << 24) * 255 * 2),
+        # overlap-save fftconvolve of 16 x 2^20 complex64 rows with 1025 taps
+        # ('same'); flops counted as the direct form's complex MACs
+        "fftconvolve_same_16x2^20_1025": (lambda: filtering.fftconvolve(x, h_1025, "same"),
+                                          16 * (1 << 20) * 16 + 1025 * 8, 16 * (1 << 20) * 1025 * 8),
     }
     for name, (fn, nbytes, flops) in cases.items():
         for _ in range(args.warmup):
