@@ -60,6 +60,8 @@ def lib():
             "orc_channelize_poly": (i, [P, ctypes.c_longlong, P, ctypes.c_longlong, i, P]),
             "orc_correlate_len": (ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_longlong, i]),
             "orc_correlate_direct": (i, [P, ctypes.c_longlong, P, ctypes.c_longlong, i, i, P]),
+            "orc_correlate2d_shape": (i, [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, i, P, P]),
+            "orc_correlate2d": (i, [P, ctypes.c_longlong, ctypes.c_longlong, P, ctypes.c_longlong, ctypes.c_longlong, i, i, ctypes.c_double, i, P]),
             "orc_cfar_alpha": (d, [d, i]),
             "orc_ca_cfar_count": (i, [i, i, i, i, i]),
             "orc_ca_cfar_alpha_f": (ctypes.c_float, [i, i, i, i, i, d]),
@@ -254,6 +256,69 @@ def correlate(in1, in2, mode="full", dtype="FP32"):
     rc = lib().orc_correlate_direct(_p(a), a.size, _p(b), b.size, m, _INT_BITS.get(dtype, 0), _p(out))
     if rc:
         raise ValueError(f"correlate rc={rc}")
+    if INTEGRAL[dtype]:
+        return out.astype(NP_DTYPE[dtype])
+    if dtype == "FP16":
+        return out.astype(np.float32).astype(np.float16)
+    return out.astype(np.float32)
+
+
+CORRELATE2D_BOUNDARIES = {"fill": 0, "wrap": 1, "symm": 2}
+
+
+def correlate2d_probe_input(count, seed, salt, dtype):
+    """ZKX/test_all/operators/cases/convolution/correlate2d/operator_probe.cu:26-27:
+    r = (17 i + seed + salt) mod 11 - 5 (unsigned 64-bit arithmetic); integers
+    store r, floating types r / 4 (float, then half rounded to nearest)."""
+    i = np.arange(count, dtype=np.uint64)
+    r = ((i * np.uint64(17) + np.uint64(seed) + np.uint64(salt)) % np.uint64(11)).astype(np.int64) - 5
+    if INTEGRAL[dtype]:
+        return r.astype(NP_DTYPE[dtype])
+    x = r.astype(np.float32) * np.float32(0.25)
+    return x.astype(np.float16) if dtype == "FP16" else x
+
+
+def correlate2d_fill(fillvalue, dtype):
+    """convolution_typed.cpp:116-131 / 28-37: the fill value in the element
+    type: integers truncate toward zero and wrap modulo 2^bits (a non-finite
+    value is an error), half rounds the float value to nearest, float rounds."""
+    if INTEGRAL[dtype]:
+        if not math.isfinite(fillvalue):
+            raise ValueError("typed correlate2d integral fillvalue must be finite")
+        bits = _INT_BITS[dtype]
+        w = math.fmod(math.trunc(fillvalue), 2.0 ** bits)
+        if w < 0:
+            w += 2.0 ** bits
+        w = int(w)
+        return float(w - (1 << bits) if w >= (1 << (bits - 1)) else w)
+    if dtype == "FP16":
+        return float(np.float16(np.float32(fillvalue)))
+    return float(np.float32(fillvalue))
+
+
+def correlate2d(in1, in2, mode="full", boundary="fill", fillvalue=0.0, dtype="FP32"):
+    """correlate2d_typed_cpu (convolution_typed.cpp:416-470) for 2-D inputs of
+    element type `dtype`; the result in that type (float32, float16 rounded
+    from the FP32 sum, or the integer wrapped modulo 2^bits)."""
+    a = np.ascontiguousarray(np.asarray(in1, np.float64))
+    b = np.ascontiguousarray(np.asarray(in2, np.float64))
+    if a.ndim != 2 or b.ndim != 2:
+        raise ValueError("correlate2d inputs must be 2-D")
+    m = CORRELATE_MODES[mode]
+    if boundary not in CORRELATE2D_BOUNDARIES:
+        raise ValueError("typed correlate2d boundary must be fill, wrap, or symm")
+    fill = 0.0 if mode == "valid" else (correlate2d_fill(fillvalue, dtype) if boundary == "fill" else 0.0)
+    rows, cols = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    rc = lib().orc_correlate2d_shape(a.shape[0], a.shape[1], b.shape[0], b.shape[1], m, ctypes.byref(rows),
+                                     ctypes.byref(cols))
+    if rc:
+        raise ValueError(f"correlate2d shape rc={rc}")
+    out = np.empty((rows.value, cols.value), np.float64)
+    if out.size:
+        rc = lib().orc_correlate2d(_p(a), a.shape[0], a.shape[1], _p(b), b.shape[0], b.shape[1], m,
+                                   CORRELATE2D_BOUNDARIES[boundary], fill, _INT_BITS.get(dtype, 0), _p(out))
+        if rc:
+            raise ValueError(f"correlate2d rc={rc}")
     if INTEGRAL[dtype]:
         return out.astype(NP_DTYPE[dtype])
     if dtype == "FP16":

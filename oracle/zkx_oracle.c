@@ -388,6 +388,101 @@ ORC_API int orc_correlate_direct(const double *in1, long long l1, const double *
     return 0;
 }
 
+/* convolution_typed.cpp:101-114 (map_correlate2d_index): an index inside
+ * [0, size) is itself; outside, fill (0) gives -1, wrap (1) the index modulo
+ * size, symm (2) the mirror with period 2*size that repeats the edge sample. */
+static long long orc_map2d(long long index, long long size, int boundary)
+{
+    if (index >= 0 && index < size) return index;
+    if (boundary == 0) return -1;
+    if (boundary == 1) {
+        long long w = index % size;
+        return w < 0 ? w + size : w;
+    }
+    const long long period = 2 * size;
+    long long w = index % period;
+    if (w < 0) w += period;
+    return w < size ? w : period - 1 - w;
+}
+
+/* convolution_typed.cpp:203-236 (correlate2d_output_shape): full r1+r2-1,
+ * same r1, valid |r1-r2|+1 when one input dominates in both dimensions (1002
+ * otherwise); any empty dimension gives 0 x 0. */
+ORC_API int orc_correlate2d_shape(long long r1, long long c1, long long r2, long long c2, int mode, long long *rows,
+                                  long long *cols)
+{
+    *rows = *cols = 0;
+    if (mode < 0 || mode > 2) return 1001;
+    if (r1 == 0 || c1 == 0 || r2 == 0 || c2 == 0) return 0;
+    if (mode == 0) {
+        *rows = r1 + r2 - 1;
+        *cols = c1 + c2 - 1;
+    } else if (mode == 1) {
+        *rows = r1;
+        *cols = c1;
+    } else {
+        const int d1 = r1 >= r2 && c1 >= c2, d2 = r2 >= r1 && c2 >= c1;
+        if (!d1 && !d2) return 1002;
+        *rows = (r1 > r2 ? r1 - r2 : r2 - r1) + 1;
+        *cols = (c1 > c2 ? c1 - c2 : c2 - c1) + 1;
+    }
+    return 0;
+}
+
+/* convolution_typed.cpp:416-470 (correlate2d_typed_cpu): output (r, c) sits at
+ * (row_start + r, col_start + c) of the full correlation (start 0 for full,
+ * r2/2 and c2/2 for same, min(r1,r2)-1 and min(c1,c2)-1 for valid); its terms
+ * run over in2 in row-major (kr, kc) order with sample
+ * in1[map(fr - (r2-1-kr))][map(fc - (c2-1-kc))], or `fill` (already stored in
+ * the element type) where either index maps to -1.  kind 0: one FP32 fma chain;
+ * kind 8/16/32: products summed modulo 2^kind, read back as two's complement.
+ * Valid mode uses the fill boundary with fill 0 (every sample is inside). */
+ORC_API int orc_correlate2d(const double *in1, long long r1, long long c1, const double *in2, long long r2,
+                            long long c2, int mode, int boundary, double fill, int kind, double *out)
+{
+    long long rows, cols;
+    const int rc = orc_correlate2d_shape(r1, c1, r2, c2, mode, &rows, &cols);
+    if (rc) return rc;
+    if (boundary < 0 || boundary > 2) return 1003;
+    long long rs = 0, cs = 0;
+    if (mode == 1) {
+        rs = r2 / 2;
+        cs = c2 / 2;
+    } else if (mode == 2) {
+        rs = (r1 < r2 ? r1 : r2) - 1;
+        cs = (c1 < c2 ? c1 : c2) - 1;
+        boundary = 0;
+        fill = 0.0;
+    }
+    for (long long r = 0; r < rows; ++r) {
+        for (long long c = 0; c < cols; ++c) {
+            const long long fr = rs + r, fc = cs + c;
+            float acc = 0.0f;
+            uint32_t macc = 0;
+            for (long long kr = 0; kr < r2; ++kr) {
+                const long long ir = orc_map2d(fr - (r2 - 1 - kr), r1, boundary);
+                for (long long kc = 0; kc < c2; ++kc) {
+                    const long long ic = orc_map2d(fc - (c2 - 1 - kc), c1, boundary);
+                    const double x = (ir < 0 || ic < 0) ? fill : in1[ir * c1 + ic];
+                    const double h = in2[kr * c2 + kc];
+                    if (kind == 0) acc = fmaf((float)x, (float)h, acc);
+                    else macc = macc + (uint32_t)(int64_t)x * (uint32_t)(int64_t)h;
+                }
+            }
+            double v;
+            if (kind == 0) {
+                v = (double)acc;
+            } else {
+                const uint32_t mask = kind == 32 ? 0xFFFFFFFFu : ((1u << kind) - 1u);
+                const uint32_t bits = macc & mask;
+                v = (bits & (1u << (kind - 1))) ? (double)((int64_t)bits - ((int64_t)1 << kind)) : (double)bits;
+            }
+            out[r * cols + c] = v;
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* ambiguity function                                                  */
 /* ------------------------------------------------------------------ */

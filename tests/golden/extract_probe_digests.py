@@ -26,7 +26,7 @@ import sys
 
 REF = "/root/reference"
 OPS = ["pulse_compression", "pulse_doppler", "ca_cfar", "cfar_alpha", "ambgfun",
-       "upfirdn", "resample_poly", "firfilter", "decimate", "channelize_poly", "correlate"]
+       "upfirdn", "resample_poly", "firfilter", "decimate", "channelize_poly", "correlate", "correlate2d"]
 KEEP = ["requested_dtype", "output_dtype", "typed_input_manifest",
         "actual_input_digest", "cpu_output_digest", "gpu_output_digest",
         "output_shape", "relative_l2", "relative_linf"]

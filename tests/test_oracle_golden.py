@@ -137,6 +137,16 @@ def run_case(case):
         manifest = (f"signal:{dt}:[{x.size}]:sha256={_digest(x)}"
                     f"|reference:{dt}:[{r.size}]:sha256={_digest(r)}")
         return manifest, _digest(orc.correlate(x, r, "same", dt))  # output in the input type
+    if op == "correlate2d":
+        r1, c1, r2, c2 = (int(a[k]) for k in ("rows1", "cols1", "rows2", "cols2"))
+        seed = int(a["seed"])
+        x = orc.correlate2d_probe_input(r1 * c1, seed, 3, dt)
+        h = orc.correlate2d_probe_input(r2 * c2, seed, 7, dt)
+        manifest = (f"in1:{dt}:[{r1}x{c1}]:bytes={x.nbytes}:sha256={_digest(x)}"
+                    f"|in2:{dt}:[{r2}x{c2}]:bytes={h.nbytes}:sha256={_digest(h)}")
+        y = orc.correlate2d(x.reshape(r1, c1), h.reshape(r2, c2), a["mode"], a["boundary"],
+                            float(a["fillvalue"]), dt)
+        return manifest, _digest(y)  # row-major output in the input type
     if op == "firfilter":
         n, ntaps = int(a["n"]), int(a["taps"])
         x = orc.firfilter_probe_signal(n, 29, dt)

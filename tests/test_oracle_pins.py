@@ -361,3 +361,48 @@ def test_channelize_single_tap_is_blockwise_dft():
 def test_channelize_rejects_long_filters():
     with pytest.raises(ValueError):
         orc.channelize_poly(np.ones(64), np.ones(33 * 2), 2)
+
+
+@pytest.mark.parametrize("boundary", ["fill", "wrap", "symm"])
+@pytest.mark.parametrize("mode", ["full", "same", "valid"])
+@pytest.mark.parametrize("shape1,shape2", [((13, 17), (3, 5)), ((9, 6), (5, 3)), ((4, 5), (7, 9)), ((20, 11), (1, 1))])
+def test_correlate2d_vs_scipy(boundary, mode, shape1, shape2):
+    """correlate2d against scipy.signal.correlate2d (FP64) on random floats
+    for every boundary (fill with a nonzero fill value) and mode; odd kernels,
+    where scipy's 'same' centring is the reference's r2/2, c2/2."""
+    from scipy import signal
+    rng = np.random.default_rng(sum(shape1) * 7 + sum(shape2))
+    a = rng.standard_normal(shape1).astype(np.float32)
+    b = rng.standard_normal(shape2).astype(np.float32)
+    if mode == "valid" and not (shape1[0] >= shape2[0] and shape1[1] >= shape2[1]) \
+            and not (shape2[0] >= shape1[0] and shape2[1] >= shape1[1]):
+        pytest.skip("valid needs one input to dominate")
+    got = orc.correlate2d(a, b, mode, boundary, 0.75, "FP32")
+    ref = signal.correlate2d(a.astype(np.float64), b.astype(np.float64), mode=mode, boundary=boundary,
+                             fillvalue=0.75)
+    assert got.shape == ref.shape
+    assert orc.rel_l2(got, ref) < 1e-6
+
+
+def test_correlate2d_integer_wrap_and_fill():
+    """INT8 sums wrap modulo 2^8 (no saturation) and the integer fill value is
+    truncated toward zero, then wrapped: brute force in Python integers."""
+    a = np.full((3, 3), 100, np.int8)
+    b = np.full((2, 2), 3, np.int8)
+    got = orc.correlate2d(a, b, "full", "fill", -258.9, "INT8")
+    fill = -258 % 256  # trunc(-258.9) = -258 -> 254 -> -2 as int8
+    fill = fill - 256 if fill >= 128 else fill
+    assert orc.correlate2d_fill(-258.9, "INT8") == fill
+    exp = np.zeros((4, 4), np.int64)
+    for r in range(4):
+        for c in range(4):
+            s = 0
+            for kr in range(2):
+                for kc in range(2):
+                    ir, ic = r - 1 + kr, c - 1 + kc
+                    x = 100 if 0 <= ir < 3 and 0 <= ic < 3 else fill
+                    s += x * 3
+            exp[r, c] = ((s + 128) % 256) - 128
+    assert np.array_equal(got.astype(np.int64), exp)
+    with pytest.raises(ValueError):
+        orc.correlate2d(a, b, "same", "fill", float("nan"), "INT8")
