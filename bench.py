@@ -62,7 +62,8 @@ def bench_filters(args):
     those samples with its designed 81-tap filter (designed per call and as
     resident custom taps); channelize_poly of them into 16 and 64 channels
     (32 taps per phase); their direct correlate with the 255 taps; and
-    fftconvolve ('same') of the 16 complex channels with a 1025-tap lowpass.
+    fftconvolve ('same') of the 16 complex channels with a 1025-tap lowpass;
+    correlate2d ('same', symm) of a 2048 x 2048 float32 image with 7 x 7 taps.
     Reports per-call time, algorithmic bytes, the HBM fraction and the
     direct-form flop rate for each."""
     import torch
@@ -79,6 +80,8 @@ def bench_filters(args):
     h_ch16 = torch.from_numpy(signal.firwin(512, 1.0 / 16).astype(np.float32)).to(dev)
     h_ch64 = torch.from_numpy(signal.firwin(2048, 1.0 / 64).astype(np.float32)).to(dev)
     h_1025 = torch.from_numpy(signal.firwin(1025, 0.1).astype(np.complex64)).to(dev)
+    img = torch.from_numpy(np.random.default_rng(8).standard_normal((2048, 2048)).astype(np.float32)).to(dev)
+    k7 = torch.from_numpy(np.random.default_rng(9).standard_normal((7, 7)).astype(np.float32)).to(dev)
     stream = torch.cuda.current_stream()
     out = {}
     cases = {
@@ -108,6 +111,10 @@ def bench_filters(args):
         # ('same'); flops counted as the direct form's complex MACs
         "fftconvolve_same_16x2^20_1025": (lambda: filtering.fftconvolve(x, h_1025, "same"),
                                           16 * (1 << 20) * 16 + 1025 * 8, 16 * (1 << 20) * 1025 * 8),
+        # direct 2-D correlate of a 2048 x 2048 image with a 7 x 7 kernel
+        # ('same', mirrored boundary)
+        "correlate2d_same_symm_2048sq_7x7": (lambda: convolution.correlate2d(img, k7, "same", "symm"),
+                                             2048 * 2048 * 8 + 49 * 4, 2048 * 2048 * 49 * 2),
     }
     for name, (fn, nbytes, flops) in cases.items():
         for _ in range(args.warmup):

@@ -494,6 +494,23 @@ int zkx_correlate_direct(const void* in1, long long l1, const void* in2, long lo
                          void* out, void* stream);
 
 /*
+ * 2-D cross-correlation, direct, row-major, in the element type `dtype`.
+ * Replaces correlate2d_output_shape / correlate2d_device(in1, in2, out,
+ * workspace) (convolution_typed.h:218, 374-379; convolution_typed.cpp:101-131,
+ * 203-260, 416-470).  mode 0 full, 1 same (origin r2/2, c2/2), 2 valid (one
+ * input must dominate in both dimensions); boundary 0 fill (fillvalue in the
+ * element type: integers truncated and wrapped, a non-finite integer fill is
+ * INVALID_ARGUMENT), 1 wrap, 2 symm (edge-inclusive mirror); valid ignores
+ * boundary and fill.  Terms in row-major order of in2; float/half one FP32
+ * fma chain, integers modulo 2^bits.  Bit-identical to the reference.  out
+ * must not overlap an input.
+ */
+int zkx_correlate2d_out_shape(long long r1, long long c1, long long r2, long long c2, int mode, long long* rows,
+                              long long* cols);
+int zkx_correlate2d(const void* in1, long long r1, long long c1, const void* in2, long long r2, long long c2, int dtype,
+                    int mode, int boundary, double fillvalue, void* out, void* stream);
+
+/*
  * Host-side FIR design for decimate's default filter: firwin with one
  * lowpass cutoff (fs = 2, pass_zero, scale) and a Hamming window, FP64 taps
  * (filter_design_typed.cpp:24-100, 268-296; decimate designs numtaps =

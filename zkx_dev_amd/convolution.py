@@ -2,7 +2,7 @@
 
 Follows ZKX/cusignal_cpp/include/cusignal/operators/convolution/
 convolution_typed.h (correlate_device(in1, in2, out, mode), the direct
-method) in cusignal.correlate's call form: 1-D inputs of one element type
+method, and correlate2d_device) in cusignal's call forms: 1-D inputs of one element type
 (float32, float16, int32, int16, int8), output in that type.  The direct
 method keeps the reference's arithmetic bit for bit (an FP32 fma chain for
 floating types, sums modulo 2^bits for integers).  All compute runs in
@@ -49,4 +49,47 @@ def correlate(in1, in2, mode="full", method="auto", *, out=None, stream=None):
     return out
 
 
-__all__ = ["correlate"]
+_BOUNDARIES = {"fill": 0, "wrap": 1, "symm": 2}
+
+
+def correlate2d(in1, in2, mode="full", boundary="fill", fillvalue=0, *, out=None, stream=None):
+    """2-D cross-correlation of two device matrices of the same dtype
+    (cusignal.correlate2d's call form; correlate2d_device,
+    convolution_typed.h:374-379).  boundary 'fill' pads with `fillvalue`
+    stored in the element type, 'wrap' is circular, 'symm' mirrors with the
+    edge repeated; 'valid' ignores both."""
+    import ctypes
+
+    if mode not in _MODES:
+        raise ValueError("typed correlate2d mode must be full, same, or valid")
+    if boundary not in _BOUNDARIES:
+        raise ValueError("typed correlate2d boundary must be fill, wrap, or symm")
+    for t, name in ((in1, "in1"), (in2, "in2")):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise ValueError(f"{name} must be a device tensor")
+    if in1.dtype != in2.dtype or in1.dtype not in _DTYPES:
+        raise ValueError("correlate2d inputs must share one of float32, float16, int32, int16, int8")
+    if in1.dim() != 2 or in2.dim() != 2:
+        raise ValueError("correlate2d inputs must both be 2-D arrays")
+    a, b = in1.contiguous(), in2.contiguous()
+    rows, cols = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    code = lib().zkx_correlate2d_out_shape(a.shape[0], a.shape[1], b.shape[0], b.shape[1], _MODES[mode],
+                                           ctypes.byref(rows), ctypes.byref(cols))
+    if code in (1001, 1002):
+        raise ValueError(lib().zkx_last_error_message().decode())
+    check(code)
+    shape = (rows.value, cols.value)
+    if out is None:
+        out = torch.empty(shape, dtype=a.dtype, device=a.device)
+    elif tuple(out.shape) != shape or out.dtype != a.dtype or not out.is_contiguous():
+        raise ValueError("typed correlate2d output shape mismatch")
+    code = lib().zkx_correlate2d(_ptr(a), a.shape[0], a.shape[1], _ptr(b), b.shape[0], b.shape[1],
+                                 _DTYPES[a.dtype], _MODES[mode], _BOUNDARIES[boundary], float(fillvalue),
+                                 _ptr(out), _stream(stream))
+    if code in (1001, 1002):
+        raise ValueError(lib().zkx_last_error_message().decode())
+    check(code)
+    return out
+
+
+__all__ = ["correlate", "correlate2d"]

@@ -18,7 +18,9 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <limits>
 #include <type_traits>
 
@@ -171,6 +173,151 @@ int launch(const void* a, long long l1, const void* b, long long l2, long long s
     return ZKX_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// 2-D direct correlation (convolution_typed.cpp:101-114, 203-236, 416-470):
+// output (r, c) sits at (rs + r, cs + c) of the full correlation; its terms
+// run over in2 in row-major (kr, kc) order with the sample at in1 row
+// map(rs + r - (r2-1-kr)) and column map(cs + c - (c2-1-kc)), or the fill
+// value where either maps outside (fill boundary).  Rows and columns map
+// independently, so a block stages its whole in1 window with the boundary
+// already applied and the inner loop has no index logic.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int map2d(long long index, int size, int boundary)
+{
+    if (index >= 0 && index < size) return static_cast<int>(index);
+    if (boundary == 0) return -1;
+    if (boundary == 1) {
+        long long w = index % size;
+        return static_cast<int>(w < 0 ? w + size : w);
+    }
+    const long long period = 2LL * size;
+    long long w = index % period;
+    if (w < 0) w += period;
+    return static_cast<int>(w < size ? w : period - 1 - w);
+}
+
+struct Corr2d {
+    int r1, c1, r2, c2;  // input shapes
+    int rows, cols;      // output shape
+    int rs, cs;          // output origin in the full correlation
+    int boundary;        // 0 fill, 1 wrap, 2 symm
+};
+
+constexpr int k2Rows = 16, k2Cols = 64, k2RowsPerThread = 4;  // 16x64 outputs per 256 threads
+
+template <class T>
+__global__ void __launch_bounds__(256)
+correlate2d_tile_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d p, Acc<T> fill, T* __restrict__ out)
+{
+    extern __shared__ unsigned char smem[];
+    Acc<T>* tile = reinterpret_cast<Acc<T>*>(smem);
+    const int TR = k2Rows + p.r2 - 1, TC = k2Cols + p.c2 - 1;
+    const int oy0 = blockIdx.y * k2Rows, ox0 = blockIdx.x * k2Cols;
+    // tile[i][j] = sample at full-correlation row oy0+rs+i-(r2-1), column ox0+cs+j-(c2-1)
+    const long long vr0 = static_cast<long long>(oy0) + p.rs - (p.r2 - 1);
+    const long long vc0 = static_cast<long long>(ox0) + p.cs - (p.c2 - 1);
+    for (int e = threadIdx.x; e < TR * TC; e += 256) {
+        const int i = e / TC, j = e - i * TC;
+        const int ir = map2d(vr0 + i, p.r1, p.boundary);
+        const int ic = map2d(vc0 + j, p.c1, p.boundary);
+        tile[e] = (ir < 0 || ic < 0) ? fill : load_acc(a[static_cast<long long>(ir) * p.c1 + ic]);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & (k2Cols - 1), ty = threadIdx.x / k2Cols;
+    const int ly = ty * k2RowsPerThread;
+    Acc<T> acc[k2RowsPerThread];
+#pragma unroll
+    for (int q = 0; q < k2RowsPerThread; ++q) acc[q] = Acc<T>(0);
+    for (int kr = 0; kr < p.r2; ++kr) {
+        const Acc<T>* trow = tile + (ly + kr) * TC + tx;
+        const T* brow = b + static_cast<long long>(kr) * p.c2;
+        for (int kc = 0; kc < p.c2; ++kc) {
+            const Acc<T> h = load_acc(brow[kc]);
+#pragma unroll
+            for (int q = 0; q < k2RowsPerThread; ++q) acc[q] = mac<T>(trow[q * TC + kc], h, acc[q]);
+        }
+    }
+    const int oc = ox0 + tx;
+    if (oc >= p.cols) return;
+#pragma unroll
+    for (int q = 0; q < k2RowsPerThread; ++q) {
+        const int orow = oy0 + ly + q;
+        if (orow < p.rows) out[static_cast<long long>(orow) * p.cols + oc] = store_acc<T>(acc[q]);
+    }
+}
+
+// in2 too large for the staged window: one output per thread, terms mapped
+// one by one from global memory (same order).
+template <class T>
+__global__ void __launch_bounds__(256)
+correlate2d_direct_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d p, Acc<T> fill,
+                          T* __restrict__ out)
+{
+    const long long e = blockIdx.x * 256LL + threadIdx.x;
+    if (e >= static_cast<long long>(p.rows) * p.cols) return;
+    const int r = static_cast<int>(e / p.cols), c = static_cast<int>(e - static_cast<long long>(r) * p.cols);
+    const long long fr = static_cast<long long>(p.rs) + r, fc = static_cast<long long>(p.cs) + c;
+    Acc<T> acc = Acc<T>(0);
+    for (int kr = 0; kr < p.r2; ++kr) {
+        const int ir = map2d(fr - (p.r2 - 1 - kr), p.r1, p.boundary);
+        for (int kc = 0; kc < p.c2; ++kc) {
+            const int ic = map2d(fc - (p.c2 - 1 - kc), p.c1, p.boundary);
+            const Acc<T> x = (ir < 0 || ic < 0) ? fill : load_acc(a[static_cast<long long>(ir) * p.c1 + ic]);
+            acc = mac<T>(x, load_acc(b[static_cast<long long>(kr) * p.c2 + kc]), acc);
+        }
+    }
+    out[e] = store_acc<T>(acc);
+}
+
+constexpr size_t k2TileLdsMax = 64 * 1024;
+
+template <class T>
+int launch2d(const void* a, const void* b, const Corr2d& p, Acc<T> fill, void* out, hipStream_t st)
+{
+    const size_t lds = sizeof(Acc<T>) * static_cast<size_t>(k2Rows + p.r2 - 1) * static_cast<size_t>(k2Cols + p.c2 - 1);
+    if (lds <= k2TileLdsMax && (p.rows + k2Rows - 1) / k2Rows <= 65535) {
+        if (lds > 48 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&correlate2d_tile_kernel<T>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     static_cast<int>(k2TileLdsMax));
+            ZKX_HIP_CHECK(e);
+        }
+        const dim3 grid((p.cols + k2Cols - 1) / k2Cols, (p.rows + k2Rows - 1) / k2Rows);
+        hipLaunchKernelGGL((correlate2d_tile_kernel<T>), grid, dim3(256), lds, st, static_cast<const T*>(a),
+                           static_cast<const T*>(b), p, fill, static_cast<T*>(out));
+    } else {
+        const long long n = static_cast<long long>(p.rows) * p.cols;
+        hipLaunchKernelGGL((correlate2d_direct_kernel<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                           st, static_cast<const T*>(a), static_cast<const T*>(b), p, fill, static_cast<T*>(out));
+    }
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+// The fill value in the element type, widened to the accumulator
+// (convolution_typed.cpp:28-37, 116-131): integers truncate toward zero and
+// wrap modulo 2^bits; half rounds the float value to nearest; float rounds.
+bool fill_bits(int dtype, double v, float* f, uint32_t* u)
+{
+    *f = 0.0f;
+    *u = 0;
+    switch (dtype) {
+    case ZKX_DTYPE_F32: *f = static_cast<float>(v); return true;
+    case ZKX_DTYPE_F16: *f = static_cast<float>(static_cast<_Float16>(static_cast<float>(v))); return true;
+    default: break;
+    }
+    if (!std::isfinite(v)) return false;
+    const int bits = dtype == ZKX_DTYPE_I32 ? 32 : dtype == ZKX_DTYPE_I16 ? 16 : 8;
+    const double modulus = std::ldexp(1.0, bits);
+    double w = std::fmod(std::trunc(v), modulus);
+    if (w < 0.0) w += modulus;
+    const uint32_t raw = static_cast<uint32_t>(w);
+    // sign-extend the bits-wide value to 32 bits, as load_acc does for T
+    const uint32_t sign = 1u << (bits - 1);
+    *u = bits == 32 ? raw : ((raw & sign) ? (raw | ~((sign << 1) - 1u)) : raw);
+    return true;
+}
+
 size_t elem_size(int dtype)
 {
     switch (dtype) {
@@ -225,6 +372,76 @@ int zkx_correlate_direct(const void* in1, long long l1, const void* in2, long lo
     case ZKX_DTYPE_I32: return launch<int32_t>(in1, l1, in2, l2, start, n, out, st);
     case ZKX_DTYPE_I16: return launch<int16_t>(in1, l1, in2, l2, start, n, out, st);
     default: return launch<int8_t>(in1, l1, in2, l2, start, n, out, st);
+    }
+}
+
+int zkx_correlate2d_out_shape(long long r1, long long c1, long long r2, long long c2, int mode, long long* rows,
+                              long long* cols)
+{
+    zkx::clear_last_error();
+    if (!rows || !cols) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate2d: null shape pointer");
+    *rows = *cols = 0;
+    if (mode < 0 || mode > 2) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d mode must be full, same, or valid");
+    constexpr long long kMax = std::numeric_limits<int>::max();
+    if (r1 < 0 || c1 < 0 || r2 < 0 || c2 < 0) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d negative dimension");
+    if (r1 * c1 > kMax || r2 * c2 > kMax) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d input exceeds 32-bit indexing");
+    if (r1 == 0 || c1 == 0 || r2 == 0 || c2 == 0) return ZKX_SUCCESS;
+    long long R, C;
+    if (mode == 0) {
+        R = r1 + r2 - 1;
+        C = c1 + c2 - 1;
+        if (R > kMax || C > kMax) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d full shape exceeds 32-bit indexing");
+    } else if (mode == 1) {
+        R = r1;
+        C = c1;
+    } else {
+        const bool d1 = r1 >= r2 && c1 >= c2, d2 = r2 >= r1 && c2 >= c1;
+        if (!d1 && !d2)
+            ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d valid mode requires one input to dominate in every dimension");
+        R = std::llabs(r1 - r2) + 1;
+        C = std::llabs(c1 - c2) + 1;
+    }
+    if (R * C > kMax) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d output exceeds 32-bit indexing");
+    *rows = R;
+    *cols = C;
+    return ZKX_SUCCESS;
+}
+
+int zkx_correlate2d(const void* in1, long long r1, long long c1, const void* in2, long long r2, long long c2, int dtype,
+                    int mode, int boundary, double fillvalue, void* out, void* stream)
+{
+    long long rows = 0, cols = 0;
+    int rc = zkx_correlate2d_out_shape(r1, c1, r2, c2, mode, &rows, &cols);
+    if (rc) return rc;
+    if (boundary < 0 || boundary > 2) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d boundary must be fill, wrap, or symm");
+    const size_t es = elem_size(dtype);
+    if (!es) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate2d element type");
+    float ff = 0.0f;
+    uint32_t fu = 0;
+    if (mode != 2 && boundary == 0 && !fill_bits(dtype, fillvalue, &ff, &fu))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d integral fillvalue must be finite");
+    if (rows * cols == 0) return ZKX_SUCCESS;
+    if (!in1 || !in2 || !out) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate2d: null pointer");
+    if (overlap(out, es * rows * cols, in1, es * r1 * c1) || overlap(out, es * rows * cols, in2, es * r2 * c2))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate2d output must not alias an input");
+    Corr2d p{static_cast<int>(r1), static_cast<int>(c1), static_cast<int>(r2), static_cast<int>(c2),
+             static_cast<int>(rows), static_cast<int>(cols), 0, 0, mode == 2 ? 0 : boundary};
+    if (mode == 1) {
+        p.rs = p.r2 / 2;
+        p.cs = p.c2 / 2;
+    } else if (mode == 2) {
+        p.rs = std::min(p.r1, p.r2) - 1;
+        p.cs = std::min(p.c1, p.c2) - 1;
+        ff = 0.0f;
+        fu = 0;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (dtype) {
+    case ZKX_DTYPE_F32: return launch2d<float>(in1, in2, p, ff, out, st);
+    case ZKX_DTYPE_F16: return launch2d<__half>(in1, in2, p, ff, out, st);
+    case ZKX_DTYPE_I32: return launch2d<int32_t>(in1, in2, p, fu, out, st);
+    case ZKX_DTYPE_I16: return launch2d<int16_t>(in1, in2, p, fu, out, st);
+    default: return launch2d<int8_t>(in1, in2, p, fu, out, st);
     }
 }
 
