@@ -2,8 +2,10 @@
 // convolution/convolution_typed.h) on libzkx_radar.so, 1-D direct method:
 // correlate_output_size and correlate_device(in1, in2, out, mode) for float,
 // __half, int32, int16 and int8, with the reference's arithmetic (an FP32 fma
-// chain for floating types, modular integer sums) and exceptions.  The FFT
-// workspace overloads and the N-D / 2-D forms are not provided.
+// chain for floating types, modular integer sums) and exceptions; and the 2-D
+// direct form correlate2d_output_shape / Correlate2DWorkspace /
+// correlate2d_device with fill, wrap and symm boundaries.  The FFT workspace
+// overloads and the N-D form are not provided.
 #pragma once
 
 #include <cstddef>
@@ -56,6 +58,100 @@ void correlate_device(const DeviceArray<T>& in1, const DeviceArray<T>& in2, Devi
     detail::check_abi(zkx_correlate_direct(in1.data(), static_cast<long long>(in1.size()), in2.data(),
                                            static_cast<long long>(in2.size()), detail::dtype_code<T>(), code,
                                            out.data(), nullptr));
+}
+
+// convolution_typed.h:204-218 (Correlate2DOptions / Correlate2DOutputShape)
+struct Correlate2DOptions {
+    int rows1 = 0;
+    int cols1 = 0;
+    int rows2 = 0;
+    int cols2 = 0;
+    std::string mode = "same";
+    std::string boundary = "fill";
+    double fillvalue = 0.0;
+};
+
+struct Correlate2DOutputShape {
+    int rows = 0;
+    int cols = 0;
+    [[nodiscard]] std::size_t size() const noexcept
+    {
+        return static_cast<std::size_t>(rows) * static_cast<std::size_t>(cols);
+    }
+};
+
+namespace detail {
+inline int correlate2d_boundary_code(const std::string& boundary)
+{
+    if (boundary == "fill") return 0;
+    if (boundary == "wrap") return 1;
+    if (boundary == "symm") return 2;
+    throw std::invalid_argument("typed correlate2d boundary must be fill, wrap, or symm");
+}
+}  // namespace detail
+
+// convolution_typed.cpp:203-236: invalid mode, boundary or shape is
+// std::invalid_argument; an empty dimension gives 0 x 0.
+inline Correlate2DOutputShape correlate2d_output_shape(const Correlate2DOptions& options)
+{
+    if (options.mode != "full" && options.mode != "same" && options.mode != "valid")
+        throw std::invalid_argument("typed correlate2d mode must be full, same, or valid");
+    (void)detail::correlate2d_boundary_code(options.boundary);
+    long long rows = 0, cols = 0;
+    const int rc = zkx_correlate2d_out_shape(options.rows1, options.cols1, options.rows2, options.cols2,
+                                             detail::correlate_mode_code(options.mode), &rows, &cols);
+    if (rc) throw std::invalid_argument(zkx_last_error_message());
+    return {static_cast<int>(rows), static_cast<int>(cols)};
+}
+
+// convolution_typed.h:220-250 (Correlate2DWorkspace): host parameters and the
+// output shape; no device scratch.
+struct Correlate2DWorkspace {
+    int rows1 = 0, cols1 = 0, rows2 = 0, cols2 = 0;
+    std::string mode = "same";
+    std::string boundary = "fill";
+    double fillvalue = 0.0;
+    int out_rows = 0, out_cols = 0;
+
+    Correlate2DWorkspace() = default;
+    explicit Correlate2DWorkspace(const Correlate2DOptions& options) { reset(options); }
+    void reset(const Correlate2DOptions& options)
+    {
+        const Correlate2DOutputShape shape = correlate2d_output_shape(options);
+        rows1 = options.rows1;
+        cols1 = options.cols1;
+        rows2 = options.rows2;
+        cols2 = options.cols2;
+        mode = options.mode;
+        boundary = options.boundary;
+        fillvalue = options.fillvalue;
+        out_rows = shape.rows;
+        out_cols = shape.cols;
+    }
+    [[nodiscard]] std::size_t output_size() const noexcept
+    {
+        return static_cast<std::size_t>(out_rows) * static_cast<std::size_t>(out_cols);
+    }
+};
+
+// convolution_typed.h:374-379: row-major in1 [rows1 x cols1], in2
+// [rows2 x cols2], out [out_rows x out_cols] preallocated by the caller.
+template <typename T>
+void correlate2d_device(const DeviceArray<T>& in1, const DeviceArray<T>& in2, DeviceArray<T>& out,
+                        Correlate2DWorkspace& workspace)
+{
+    if (in1.size() != static_cast<std::size_t>(workspace.rows1) * static_cast<std::size_t>(workspace.cols1) ||
+        in2.size() != static_cast<std::size_t>(workspace.rows2) * static_cast<std::size_t>(workspace.cols2) ||
+        out.size() != workspace.output_size())
+        throw std::invalid_argument("typed correlate2d shape mismatch");
+    if (workspace.output_size() == 0) return;
+    const int rc = zkx_correlate2d(in1.data(), workspace.rows1, workspace.cols1, in2.data(), workspace.rows2,
+                                   workspace.cols2, detail::dtype_code<T>(),
+                                   detail::correlate_mode_code(workspace.mode),
+                                   detail::correlate2d_boundary_code(workspace.boundary), workspace.fillvalue,
+                                   out.data(), nullptr);
+    if (rc == ZKX_INVALID_ARGUMENT || rc == ZKX_INVALID_SHAPE) throw std::invalid_argument(zkx_last_error_message());
+    detail::check_abi(rc);
 }
 
 }  // namespace cusignal

@@ -2,7 +2,7 @@
 // for one element type of the reference's instantiations (float, __half,
 // int32, int16, int8; radartools_typed.cpp:810-811, filtering_typed.cu:683+),
 // plus the complex filtering extensions, fftconvolve, decimate,
-// channelize_poly, correlate and ambgfun.  Built with g++ and
+// channelize_poly, correlate, ambgfun and correlate2d.  Built with g++ and
 // linked to libzkx_radar.so.
 //
 // usage: typed_ops IN OUT
@@ -198,8 +198,29 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
     ao.cut_value = 37.0;
     cusignal::ambgfun_device<T>(d_t, ar, ao);
     w.vec(1, ar.fp64.to_host());
+    // 24-26: correlate2d of the filtering samples as a 30 x 100 matrix with
+    // the first 30 taps as 5 x 6: same/symm, full/fill (fill value 2),
+    // valid/wrap, in the input type
+    {
+        const auto d_k = DeviceArray<T>::from_host(std::vector<T>(h.begin(), h.begin() + 30));
+        const char* forms[3][2] = {{"same", "symm"}, {"full", "fill"}, {"valid", "wrap"}};
+        for (const auto& f : forms) {
+            cusignal::Correlate2DOptions o;
+            o.rows1 = 30;
+            o.cols1 = 100;
+            o.rows2 = 5;
+            o.cols2 = 6;
+            o.mode = f[0];
+            o.boundary = f[1];
+            o.fillvalue = 2.0;
+            cusignal::Correlate2DWorkspace ws(o);
+            DeviceArray<T> c2(ws.output_size());
+            cusignal::correlate2d_device<T>(d_xf, d_k, c2, ws);
+            w.vec(3, c2.to_host());
+        }
+    }
     if constexpr (std::is_same_v<T, float>) {
-        // 24-26: complex filtering extensions on the pulses (float taps)
+        // 27-29: complex filtering extensions on the pulses (float taps)
         cusignal::UpfirdnComplexDeviceResult uc;
         cusignal::upfirdn_device<float>(d_h, d_x, up, down, uc, {{P, R}, 1});
         w.vec(0, uc.fp32.to_host());

@@ -197,6 +197,11 @@ def test_typed_operators(name, tmp_path):
     val = next(b)
     ref = radar.ambgfun(tm, 1000.0, 100.0, cut="doppler", cutValue=37.0).cpu().numpy()
     assert val.dtype == np.float64 and np.array_equal(val, ref, equal_nan=True)
+    # correlate2d of the filtering samples as 30 x 100 with 30 taps as 5 x 6
+    for mode, boundary in (("same", "symm"), ("full", "fill"), ("valid", "wrap")):
+        val = next(b)
+        ref = orc.correlate2d(inp["xf"].reshape(30, 100), inp["h"][:30].reshape(5, 6), mode, boundary, 2.0, name)
+        assert np.array_equal(val.view(np.uint8), ref.reshape(-1).view(np.uint8)), (name, mode, boundary)
     if name == "FP32":
         # complex extensions: real taps act on each component
         uc = _c(next(b))
