@@ -494,6 +494,19 @@ int zkx_correlate_direct(const void* in1, long long l1, const void* in2, long lo
                          void* out, void* stream);
 
 /*
+ * 1-D cross-correlation, FFT method.  Replaces the workspace overload
+ * correlate_device(in1, in2, out, CorrelateDeviceWorkspace&)
+ * (convolution_typed.h:307-327): inputs converted to complex FP32, in2
+ * reversed, the full convolution by the overlap-save FFT, the centred crop
+ * stored in the element type (half rounded to nearest; integers rounded half
+ * to even and wrapped modulo 2^bits).  FP32 FFT results, not the direct sums
+ * bit for bit.  The shorter input may have at most 4096 samples
+ * (UNSUPPORTED_OPERATION beyond); the library holds the workspace.
+ */
+int zkx_correlate_fft(const void* in1, long long l1, const void* in2, long long l2, int dtype, int mode, void* out,
+                      void* stream);
+
+/*
  * 2-D cross-correlation, direct, row-major, in the element type `dtype`.
  * Replaces correlate2d_output_shape / correlate2d_device(in1, in2, out,
  * workspace) (convolution_typed.h:218, 374-379; convolution_typed.cpp:101-131,

@@ -68,3 +68,43 @@ def test_correlate_validation():
     with pytest.raises(ValueError):
         convolution.correlate(a, a[:4], "same", out=a)  # output aliases an input
     assert convolution.correlate(a[:0], a[:4]).numel() == 0
+
+
+def _small_ints(name, n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(-9, 10, n).astype(NP[name])
+
+
+@pytest.mark.parametrize("name", list(NP))
+@pytest.mark.parametrize("l1,l2", [(1000, 31), (31, 1000), (20000, 700), (9000, 4096), (1, 1), (7, 7)])
+@pytest.mark.parametrize("mode", ["full", "same", "valid"])
+def test_correlate_fft_method(name, l1, l2, mode):
+    """method='fft' (complex FP32 FFT, the reference's workspace overload)
+    against the direct sums of the oracle: floats within FP32 FFT round-off;
+    integers with sums well inside FP32 precision are exact after rounding,
+    including INT8's wrap modulo 2^8."""
+    from zkx_dev_amd import convolution
+    print(f"correlate fft {name} l1={l1} l2={l2} {mode}", flush=True)
+    if orc.INTEGRAL[name]:
+        a, b = _small_ints(name, l1, l1), _small_ints(name, l2, l2 + 3)
+    else:
+        a, b = _inputs(name, l1, l1), _inputs(name, l2, l2 + 1)
+    y = convolution.correlate(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mode, "fft").cpu().numpy()
+    ref = orc.correlate(a, b, mode, name)
+    assert y.dtype == ref.dtype and y.shape == ref.shape
+    if orc.INTEGRAL[name]:
+        assert np.array_equal(y, ref)
+    else:
+        tol = 1e-5 if name == "FP32" else 2e-3  # FP16 output rounding
+        assert orc.rel_l2(y.astype(np.float64), ref.astype(np.float64)) < tol
+
+
+def test_correlate_fft_limits():
+    from zkx_dev_amd import convolution
+    from zkx_dev_amd._lib import ZkxError
+    a = torch.ones(5000, device="cuda")
+    with pytest.raises(ZkxError):
+        convolution.correlate(a, a[:4097], "full", "fft")  # both inputs longer than 4096
+    with pytest.raises(ValueError):
+        convolution.correlate(a, a[:4], "same", "fft", out=a)  # aliasing
+    assert convolution.correlate(a[:0], a[:4], "full", "fft").numel() == 0

@@ -57,6 +57,7 @@ SIGNATURES = {
     "zkx_channelize_poly_f32": (I, [P, LL, P, LL, I, P, P]),
     "zkx_correlate_out_len": (LL, [LL, LL, I]),
     "zkx_correlate_direct": (I, [P, LL, P, LL, I, I, P, P]),
+    "zkx_correlate_fft": (I, [P, LL, P, LL, I, I, P, P]),
     "zkx_correlate2d_out_shape": (I, [LL, LL, LL, LL, I, P, P]),
     "zkx_correlate2d": (I, [P, LL, LL, P, LL, LL, I, I, I, ctypes.c_double, P, P]),
     "zkx_firwin_lowpass_hamming": (I, [I, ctypes.c_float, P]),

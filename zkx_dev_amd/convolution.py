@@ -21,13 +21,14 @@ _MODES = {"full": 0, "same": 1, "valid": 2}
 
 def correlate(in1, in2, mode="full", method="auto", *, out=None, stream=None):
     """Cross-correlation of two 1-D device tensors of the same dtype.
-    method: "direct" or "auto" (the reference's choice for 1-D is direct)."""
+    method: "direct" (the reference's direct sums, bit for bit), "fft" (the
+    reference's workspace overload: complex FP32 FFT, integers rounded half to
+    even and wrapped) or "auto" (direct, as the reference's overload without a
+    workspace)."""
     if mode not in _MODES:
         raise ValueError("typed correlate mode must be full, same, or valid")
     if method not in ("direct", "auto", "fft"):
         raise ValueError("typed correlate method must be direct, fft, or auto")
-    if method == "fft":
-        raise NotImplementedError("correlate: only the direct method is built")
     for t, name in ((in1, "in1"), (in2, "in2")):
         if not isinstance(t, torch.Tensor) or not t.is_cuda:
             raise ValueError(f"{name} must be a device tensor")
@@ -41,8 +42,8 @@ def correlate(in1, in2, mode="full", method="auto", *, out=None, stream=None):
         out = torch.empty(max(n, 0), dtype=a.dtype, device=a.device)
     elif out.numel() != n or out.dtype != a.dtype:
         raise ValueError("typed correlate output shape mismatch")
-    code = lib().zkx_correlate_direct(_ptr(a), a.numel(), _ptr(b), b.numel(), _DTYPES[a.dtype], _MODES[mode],
-                                      _ptr(out), _stream(stream))
+    fn = lib().zkx_correlate_fft if method == "fft" else lib().zkx_correlate_direct
+    code = fn(_ptr(a), a.numel(), _ptr(b), b.numel(), _DTYPES[a.dtype], _MODES[mode], _ptr(out), _stream(stream))
     if code in (1001, 1002):
         raise ValueError(lib().zkx_last_error_message().decode())
     check(code)

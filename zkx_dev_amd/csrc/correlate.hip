@@ -318,6 +318,76 @@ bool fill_bits(int dtype, double v, float* f, uint32_t* u)
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// FFT method (convolution_typed.h:307-327): the inputs converted to complex
+// FP32, in2 reversed, the full convolution by the overlap-save FFT kernel,
+// then the centred crop stored in the element type: float as is, half rounded
+// to nearest, integers rounded half to even and wrapped modulo 2^bits (no
+// saturation).  FP32 FFT arithmetic: not the direct sums bit for bit.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256)
+to_c64_kernel(const T* __restrict__ x, long long n, int reverse, float2* __restrict__ y)
+{
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i < n) y[i] = make_float2(load_f(x[reverse ? n - 1 - i : i]), 0.0f);
+}
+
+template <class T>
+__device__ __forceinline__ T store_rounded(float v)
+{
+    if constexpr (std::is_integral_v<T>) {
+        using U = std::make_unsigned_t<T>;
+        constexpr double modulus = static_cast<double>(std::numeric_limits<U>::max()) + 1.0;
+        if (!isfinite(v)) return T(0);
+        double w = fmod(rint(static_cast<double>(v)), modulus);
+        if (w < 0.0) w += modulus;
+        return static_cast<T>(static_cast<U>(w));
+    } else {
+        return store_f<T>(v);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+from_c64_kernel(const float2* __restrict__ y, long long start, long long n, T* __restrict__ out)
+{
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i < n) out[i] = store_rounded<T>(y[start + i].x);
+}
+
+thread_local zkx::ThreadWorkspace t_corr_fft_ws;
+
+template <class T>
+int correlate_fft(const void* in1, long long l1, const void* in2, long long l2, long long start, long long n,
+                  void* out, hipStream_t st)
+{
+    const long long full = l1 + l2 - 1;
+    void* ws = nullptr;
+    int rc = t_corr_fft_ws.reserve(sizeof(float2) * static_cast<size_t>(l1 + l2 + full), &ws);
+    if (rc) return rc;
+    float2* A = static_cast<float2*>(ws);
+    float2* B = A + l1;
+    float2* Y = B + l2;
+    hipLaunchKernelGGL((to_c64_kernel<T>), dim3(static_cast<unsigned>((l1 + 255) / 256)), dim3(256), 0, st,
+                       static_cast<const T*>(in1), l1, 0, A);
+    ZKX_LAUNCH_CHECK();
+    hipLaunchKernelGGL((to_c64_kernel<T>), dim3(static_cast<unsigned>((l2 + 255) / 256)), dim3(256), 0, st,
+                       static_cast<const T*>(in2), l2, 1, B);
+    ZKX_LAUNCH_CHECK();
+    // the full convolution is symmetric in its operands: the longer one is
+    // the signal, the shorter (at most 4096 samples) the kernel
+    const bool swap = l2 > l1;
+    rc = zkx_fftconvolve_c64(reinterpret_cast<const zkx_complex64*>(swap ? B : A), 1, static_cast<int>(swap ? l2 : l1),
+                             reinterpret_cast<const zkx_complex64*>(swap ? A : B), static_cast<int>(swap ? l1 : l2), 0,
+                             reinterpret_cast<zkx_complex64*>(Y), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL((from_c64_kernel<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, Y, start,
+                       n, static_cast<T*>(out));
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
 size_t elem_size(int dtype)
 {
     switch (dtype) {
@@ -339,6 +409,33 @@ bool overlap(const void* p, size_t pb, const void* q, size_t qb)
 }  // namespace
 
 extern "C" {
+
+int zkx_correlate_fft(const void* in1, long long l1, const void* in2, long long l2, int dtype, int mode, void* out,
+                      void* stream)
+{
+    zkx::clear_last_error();
+    if (mode < 0 || mode > 2) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate mode must be full, same, or valid");
+    const size_t es = elem_size(dtype);
+    if (!es) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate element type");
+    if (l1 < 0 || l2 < 0 || l1 + l2 - 1 > std::numeric_limits<int>::max())
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate input exceeds 32-bit indexing");
+    const long long n = zkx_correlate_out_len(l1, l2, mode);
+    if (n == 0) return ZKX_SUCCESS;
+    if (std::min(l1, l2) > 4096)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "typed correlate fft method needs one input of at most 4096 samples");
+    if (!in1 || !in2 || !out) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate: null pointer");
+    if (overlap(out, es * n, in1, es * l1) || overlap(out, es * n, in2, es * l2))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate output must not alias an input");
+    const long long start = (l1 + l2 - 1 - n) / 2;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (dtype) {
+    case ZKX_DTYPE_F32: return correlate_fft<float>(in1, l1, in2, l2, start, n, out, st);
+    case ZKX_DTYPE_F16: return correlate_fft<__half>(in1, l1, in2, l2, start, n, out, st);
+    case ZKX_DTYPE_I32: return correlate_fft<int32_t>(in1, l1, in2, l2, start, n, out, st);
+    case ZKX_DTYPE_I16: return correlate_fft<int16_t>(in1, l1, in2, l2, start, n, out, st);
+    default: return correlate_fft<int8_t>(in1, l1, in2, l2, start, n, out, st);
+    }
+}
 
 long long zkx_correlate_out_len(long long l1, long long l2, int mode)
 {
