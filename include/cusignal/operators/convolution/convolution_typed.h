@@ -4,8 +4,8 @@
 // __half, int32, int16 and int8, with the reference's arithmetic (an FP32 fma
 // chain for floating types, modular integer sums) and exceptions; and the 2-D
 // direct form correlate2d_output_shape / Correlate2DWorkspace /
-// correlate2d_device with fill, wrap and symm boundaries.  The FFT workspace
-// overloads and the N-D form are not provided.
+// correlate2d_device with fill, wrap and symm boundaries; the FFT method
+// through the 1-D workspace overload.  The N-D form is not provided.
 #pragma once
 
 #include <cstddef>
@@ -58,6 +58,49 @@ void correlate_device(const DeviceArray<T>& in1, const DeviceArray<T>& in2, Devi
     detail::check_abi(zkx_correlate_direct(in1.data(), static_cast<long long>(in1.size()), in2.data(),
                                            static_cast<long long>(in2.size()), detail::dtype_code<T>(), code,
                                            out.data(), nullptr));
+}
+
+// convolution_typed.h:120-158 (CorrelateDeviceParams / CorrelateDeviceWorkspace):
+// the FFT method's parameters and output length; the library holds the
+// complex FP32 scratch, so the workspace carries no device arrays.
+struct CorrelateDeviceParams {
+    int len1 = 0;
+    int len2 = 0;
+    std::string mode = "same";
+};
+
+struct CorrelateDeviceWorkspace {
+    int len1 = 0;
+    int len2 = 0;
+    std::string mode = "same";
+    std::size_t out_len = 0;
+
+    CorrelateDeviceWorkspace() = default;
+    explicit CorrelateDeviceWorkspace(const CorrelateDeviceParams& params) { reset(params); }
+    void reset(const CorrelateDeviceParams& params)
+    {
+        if (params.len1 < 0 || params.len2 < 0) throw std::invalid_argument("typed correlate negative length");
+        out_len = correlate_output_size(static_cast<std::size_t>(params.len1), static_cast<std::size_t>(params.len2),
+                                        params.mode);
+        len1 = params.len1;
+        len2 = params.len2;
+        mode = params.mode;
+    }
+};
+
+// convolution_typed.h:307-327: FFT method with the caller's workspace.
+template <typename T>
+void correlate_device(const DeviceArray<T>& in1, const DeviceArray<T>& in2, DeviceArray<T>& out,
+                      CorrelateDeviceWorkspace& workspace)
+{
+    if (in1.size() != static_cast<std::size_t>(workspace.len1) ||
+        in2.size() != static_cast<std::size_t>(workspace.len2) || out.size() != workspace.out_len)
+        throw std::invalid_argument("typed correlate workspace shape mismatch");
+    if (workspace.out_len == 0) return;
+    const int rc = zkx_correlate_fft(in1.data(), workspace.len1, in2.data(), workspace.len2, detail::dtype_code<T>(),
+                                     detail::correlate_mode_code(workspace.mode), out.data(), nullptr);
+    if (rc == ZKX_INVALID_ARGUMENT || rc == ZKX_INVALID_SHAPE) throw std::invalid_argument(zkx_last_error_message());
+    detail::check_abi(rc);
 }
 
 // convolution_typed.h:204-218 (Correlate2DOptions / Correlate2DOutputShape)

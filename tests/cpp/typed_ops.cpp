@@ -179,6 +179,11 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
             cusignal::correlate_device<T>(d_x1, d_h, co, mode);
             w.vec(3, co.to_host());
         }
+        // 20b: the FFT method through the workspace overload, 'same'
+        cusignal::CorrelateDeviceWorkspace cw({n, static_cast<int>(h.size()), "same"});
+        DeviceArray<T> cf(cw.out_len);
+        cusignal::correlate_device<T>(d_x1, d_h, cf, cw);
+        w.vec(3, cf.to_host());
     }
     // 21-23: ambiguity function of the template: 2-D (FP32 for float and half,
     // FP64 storage for integers), delay cut and Doppler cut (FP64 storage)

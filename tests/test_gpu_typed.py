@@ -86,7 +86,7 @@ def _c(a):
 
 @pytest.mark.parametrize("name", list(DTYPES))
 def test_typed_operators(name, tmp_path):
-    from zkx_dev_amd import filtering, radar
+    from zkx_dev_amd import convolution, filtering, radar
     from zkx_dev_amd._lib import check, lib
     rng = np.random.default_rng(20261203 + DTYPES[name][0])
     inp = _inputs(name, rng)
@@ -185,6 +185,11 @@ def test_typed_operators(name, tmp_path):
         val = next(b)
         ref = orc.correlate(inp["xf"][:N], inp["h"], mode, name)
         assert np.array_equal(val.view(np.uint8), ref.view(np.uint8)), (name, mode)
+    # the FFT method (workspace overload) equals the Python layer's FFT method
+    val = next(b)
+    ref = convolution.correlate(torch.from_numpy(np.ascontiguousarray(inp["xf"][:N])).cuda(),
+                                torch.from_numpy(inp["h"]).cuda(), "same", "fft").cpu().numpy()
+    assert np.array_equal(val.view(np.uint8), ref.view(np.uint8)), (name, "fft")
     # ambgfun of the template: 2-D FP32 for float/half, FP64 storage for
     # integers (radartools_typed.cpp:119-130); the cuts are FP64 storage
     flag, val = next(b)[0], next(b)
