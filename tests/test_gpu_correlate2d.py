@@ -33,12 +33,14 @@ def _dominates(s1, s2):
 
 @pytest.mark.parametrize("name", list(NP))
 @pytest.mark.parametrize("s1,s2", [((13, 17), (3, 5)), ((40, 100), (7, 9)), ((70, 130), (1, 1)), ((5, 4), (8, 9)),
-                                   ((33, 65), (4, 6)), ((20, 30), (60, 70))])
+                                   ((33, 65), (4, 6)), ((20, 30), (60, 70)), ((50, 600), (5, 7))])
 @pytest.mark.parametrize("mode", ["full", "same", "valid"])
 @pytest.mark.parametrize("boundary", ["fill", "wrap", "symm"])
 def test_correlate2d_bit_identical(name, s1, s2, mode, boundary):
-    """16x64 output tiles with partial last tiles, in2 larger than in1, even
-    kernel sizes ('same' origin r2/2, c2/2), a nonzero fill value."""
+    """16x256 output tiles (4 x 4 outputs per thread) with partial last tiles
+    and several column blocks, the 16x64 form for kernels whose wide window
+    exceeds its LDS budget (60 x 70), in2 larger than in1, even kernel sizes
+    ('same' origin r2/2, c2/2), a nonzero fill value."""
     from zkx_dev_amd import convolution
     if mode == "valid" and not _dominates(s1, s2):
         pytest.skip("valid needs one input to dominate")

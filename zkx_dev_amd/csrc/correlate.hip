@@ -203,6 +203,39 @@ struct Corr2d {
     int boundary;        // 0 fill, 1 wrap, 2 symm
 };
 
+// tile[i * pitch + j] = sample at full-correlation row vr0 + i, column
+// vc0 + j (boundary applied; fill outside for the fill boundary), for
+// j < width; 0 in the padding columns.  Eight elements per thread per round
+// with unconditional clamped loads, so that eight loads are in flight before
+// the first wait.
+template <class T>
+__device__ __forceinline__ void stage2d(const T* __restrict__ a, const Corr2d& p, Acc<T> fill, long long vr0,
+                                        long long vc0, int rows, int width, int pitch, Acc<T>* tile)
+{
+    constexpr int U = 8;
+    const int total = rows * pitch;
+    for (int e0 = threadIdx.x; e0 < total; e0 += 256 * U) {
+        Acc<T> v[U];
+        int j[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = min(e0 + u * 256, total - 1);
+            const int i = e / pitch;
+            j[u] = e - i * pitch;
+            const int ir = map2d(vr0 + i, p.r1, p.boundary);
+            const int ic = map2d(vc0 + min(j[u], width - 1), p.c1, p.boundary);
+            in[u] = ir >= 0 && ic >= 0;
+            v[u] = load_acc(a[in[u] ? static_cast<long long>(ir) * p.c1 + ic : 0]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * 256;
+            if (e < total) tile[e] = j[u] >= width ? Acc<T>(0) : (in[u] ? v[u] : fill);
+        }
+    }
+}
+
 constexpr int k2Rows = 16, k2Cols = 64, k2RowsPerThread = 4;  // 16x64 outputs per 256 threads
 
 template <class T>
@@ -216,12 +249,7 @@ correlate2d_tile_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d
     // tile[i][j] = sample at full-correlation row oy0+rs+i-(r2-1), column ox0+cs+j-(c2-1)
     const long long vr0 = static_cast<long long>(oy0) + p.rs - (p.r2 - 1);
     const long long vc0 = static_cast<long long>(ox0) + p.cs - (p.c2 - 1);
-    for (int e = threadIdx.x; e < TR * TC; e += 256) {
-        const int i = e / TC, j = e - i * TC;
-        const int ir = map2d(vr0 + i, p.r1, p.boundary);
-        const int ic = map2d(vc0 + j, p.c1, p.boundary);
-        tile[e] = (ir < 0 || ic < 0) ? fill : load_acc(a[static_cast<long long>(ir) * p.c1 + ic]);
-    }
+    stage2d(a, p, fill, vr0, vc0, TR, TC, TC, tile);
     __syncthreads();
     const int tx = threadIdx.x & (k2Cols - 1), ty = threadIdx.x / k2Cols;
     const int ly = ty * k2RowsPerThread;
@@ -243,6 +271,69 @@ correlate2d_tile_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d
     for (int q = 0; q < k2RowsPerThread; ++q) {
         const int orow = oy0 + ly + q;
         if (orow < p.rows) out[static_cast<long long>(orow) * p.cols + oc] = store_acc<T>(acc[q]);
+    }
+}
+
+// Wide form: 16 x 256 outputs per block, each thread 4 rows x 4 consecutive
+// columns.  Tile rows are padded to a multiple of 4 words (plus 8) so that a
+// thread reads the 8 words from its first column on with two aligned 16-byte
+// loads per row and group of 4 taps; each loaded word feeds up to 4 chains.
+constexpr int k2wCols = 256, k2wColsPerThread = 4;
+
+__host__ __device__ constexpr int wide_pitch(int c2) { return ((k2wCols + c2 - 1 + 3) / 4) * 4 + 8; }
+
+template <class T>
+__global__ void __launch_bounds__(256)
+correlate2d_wide_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d p, Acc<T> fill, T* __restrict__ out)
+{
+    extern __shared__ unsigned char smem[];
+    Acc<T>* tile = reinterpret_cast<Acc<T>*>(smem);
+    const int TR = k2Rows + p.r2 - 1, TCv = k2wCols + p.c2 - 1, PITCH = wide_pitch(p.c2);
+    const int oy0 = blockIdx.y * k2Rows, ox0 = blockIdx.x * k2wCols;
+    const long long vr0 = static_cast<long long>(oy0) + p.rs - (p.r2 - 1);
+    const long long vc0 = static_cast<long long>(ox0) + p.cs - (p.c2 - 1);
+    stage2d(a, p, fill, vr0, vc0, TR, TCv, PITCH, tile);  // padding columns are read but never used
+    __syncthreads();
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int ly = ty * k2RowsPerThread, lx = tx * k2wColsPerThread;
+    Acc<T> acc[k2RowsPerThread][k2wColsPerThread];
+#pragma unroll
+    for (int q = 0; q < k2RowsPerThread; ++q)
+#pragma unroll
+        for (int j = 0; j < k2wColsPerThread; ++j) acc[q][j] = Acc<T>(0);
+    for (int kr = 0; kr < p.r2; ++kr) {
+        const T* brow = b + static_cast<long long>(kr) * p.c2;
+        for (int kc0 = 0; kc0 < p.c2; kc0 += 4) {
+            Acc<T> h[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) h[kk] = kc0 + kk < p.c2 ? load_acc(brow[kc0 + kk]) : Acc<T>(0);
+            const int nk = min(4, p.c2 - kc0);
+#pragma unroll
+            for (int q = 0; q < k2RowsPerThread; ++q) {
+                // words lx + kc0 .. + 7 of tile row ly + q + kr (16-byte aligned)
+                const Acc<T>* src = tile + (ly + q + kr) * PITCH + lx + kc0;
+                Acc<T> w[8];
+                *reinterpret_cast<uint4*>(&w[0]) = *reinterpret_cast<const uint4*>(src);
+                *reinterpret_cast<uint4*>(&w[4]) = *reinterpret_cast<const uint4*>(src + 4);
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    if (kk < nk) {
+#pragma unroll
+                        for (int j = 0; j < k2wColsPerThread; ++j) acc[q][j] = mac<T>(w[j + kk], h[kk], acc[q][j]);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < k2RowsPerThread; ++q) {
+        const int orow = oy0 + ly + q;
+        if (orow >= p.rows) break;
+#pragma unroll
+        for (int j = 0; j < k2wColsPerThread; ++j) {
+            const int oc = ox0 + lx + j;
+            if (oc < p.cols) out[static_cast<long long>(orow) * p.cols + oc] = store_acc<T>(acc[q][j]);
+        }
     }
 }
 
@@ -274,8 +365,17 @@ constexpr size_t k2TileLdsMax = 64 * 1024;
 template <class T>
 int launch2d(const void* a, const void* b, const Corr2d& p, Acc<T> fill, void* out, hipStream_t st)
 {
+    const bool rows_ok = (p.rows + k2Rows - 1) / k2Rows <= 65535;
+    const size_t lds_w = sizeof(Acc<T>) * static_cast<size_t>(k2Rows + p.r2 - 1) * static_cast<size_t>(wide_pitch(p.c2));
+    if (lds_w <= 48 * 1024 && rows_ok) {
+        const dim3 grid((p.cols + k2wCols - 1) / k2wCols, (p.rows + k2Rows - 1) / k2Rows);
+        hipLaunchKernelGGL((correlate2d_wide_kernel<T>), grid, dim3(256), lds_w, st, static_cast<const T*>(a),
+                           static_cast<const T*>(b), p, fill, static_cast<T*>(out));
+        ZKX_LAUNCH_CHECK();
+        return ZKX_SUCCESS;
+    }
     const size_t lds = sizeof(Acc<T>) * static_cast<size_t>(k2Rows + p.r2 - 1) * static_cast<size_t>(k2Cols + p.c2 - 1);
-    if (lds <= k2TileLdsMax && (p.rows + k2Rows - 1) / k2Rows <= 65535) {
+    if (lds <= k2TileLdsMax && rows_ok) {
         if (lds > 48 * 1024) {
             const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&correlate2d_tile_kernel<T>),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize,
