@@ -274,11 +274,11 @@ correlate2d_tile_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d
     }
 }
 
-// Wide form: 16 x 256 outputs per block, each thread 4 rows x 4 consecutive
-// columns.  Tile rows are padded to a multiple of 4 words (plus 8) so that a
+// Wide form: 8 x 256 outputs per block, each thread 2 rows x 4 consecutive
+// columns (twice the blocks of 4-row threads: the kernel is latency-bound).  Tile rows are padded to a multiple of 4 words (plus 8) so that a
 // thread reads the 8 words from its first column on with two aligned 16-byte
 // loads per row and group of 4 taps; each loaded word feeds up to 4 chains.
-constexpr int k2wCols = 256, k2wColsPerThread = 4;
+constexpr int k2wCols = 256, k2wColsPerThread = 4, k2wRowsPerThread = 2, k2wRows = 4 * k2wRowsPerThread;
 
 __host__ __device__ constexpr int wide_pitch(int c2) { return ((k2wCols + c2 - 1 + 3) / 4) * 4 + 8; }
 
@@ -288,17 +288,17 @@ correlate2d_wide_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d
 {
     extern __shared__ unsigned char smem[];
     Acc<T>* tile = reinterpret_cast<Acc<T>*>(smem);
-    const int TR = k2Rows + p.r2 - 1, TCv = k2wCols + p.c2 - 1, PITCH = wide_pitch(p.c2);
-    const int oy0 = blockIdx.y * k2Rows, ox0 = blockIdx.x * k2wCols;
+    const int TR = k2wRows + p.r2 - 1, TCv = k2wCols + p.c2 - 1, PITCH = wide_pitch(p.c2);
+    const int oy0 = blockIdx.y * k2wRows, ox0 = blockIdx.x * k2wCols;
     const long long vr0 = static_cast<long long>(oy0) + p.rs - (p.r2 - 1);
     const long long vc0 = static_cast<long long>(ox0) + p.cs - (p.c2 - 1);
     stage2d(a, p, fill, vr0, vc0, TR, TCv, PITCH, tile);  // padding columns are read but never used
     __syncthreads();
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int ly = ty * k2RowsPerThread, lx = tx * k2wColsPerThread;
-    Acc<T> acc[k2RowsPerThread][k2wColsPerThread];
+    const int ly = ty * k2wRowsPerThread, lx = tx * k2wColsPerThread;
+    Acc<T> acc[k2wRowsPerThread][k2wColsPerThread];
 #pragma unroll
-    for (int q = 0; q < k2RowsPerThread; ++q)
+    for (int q = 0; q < k2wRowsPerThread; ++q)
 #pragma unroll
         for (int j = 0; j < k2wColsPerThread; ++j) acc[q][j] = Acc<T>(0);
     for (int kr = 0; kr < p.r2; ++kr) {
@@ -309,7 +309,7 @@ correlate2d_wide_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d
             for (int kk = 0; kk < 4; ++kk) h[kk] = kc0 + kk < p.c2 ? load_acc(brow[kc0 + kk]) : Acc<T>(0);
             const int nk = min(4, p.c2 - kc0);
 #pragma unroll
-            for (int q = 0; q < k2RowsPerThread; ++q) {
+            for (int q = 0; q < k2wRowsPerThread; ++q) {
                 // words lx + kc0 .. + 7 of tile row ly + q + kr (16-byte aligned)
                 const Acc<T>* src = tile + (ly + q + kr) * PITCH + lx + kc0;
                 Acc<T> w[8];
@@ -326,7 +326,7 @@ correlate2d_wide_kernel(const T* __restrict__ a, const T* __restrict__ b, Corr2d
         }
     }
 #pragma unroll
-    for (int q = 0; q < k2RowsPerThread; ++q) {
+    for (int q = 0; q < k2wRowsPerThread; ++q) {
         const int orow = oy0 + ly + q;
         if (orow >= p.rows) break;
 #pragma unroll
@@ -366,9 +366,9 @@ template <class T>
 int launch2d(const void* a, const void* b, const Corr2d& p, Acc<T> fill, void* out, hipStream_t st)
 {
     const bool rows_ok = (p.rows + k2Rows - 1) / k2Rows <= 65535;
-    const size_t lds_w = sizeof(Acc<T>) * static_cast<size_t>(k2Rows + p.r2 - 1) * static_cast<size_t>(wide_pitch(p.c2));
-    if (lds_w <= 48 * 1024 && rows_ok) {
-        const dim3 grid((p.cols + k2wCols - 1) / k2wCols, (p.rows + k2Rows - 1) / k2Rows);
+    const size_t lds_w = sizeof(Acc<T>) * static_cast<size_t>(k2wRows + p.r2 - 1) * static_cast<size_t>(wide_pitch(p.c2));
+    if (lds_w <= 48 * 1024 && (p.rows + k2wRows - 1) / k2wRows <= 65535) {
+        const dim3 grid((p.cols + k2wCols - 1) / k2wCols, (p.rows + k2wRows - 1) / k2wRows);
         hipLaunchKernelGGL((correlate2d_wide_kernel<T>), grid, dim3(256), lds_w, st, static_cast<const T*>(a),
                            static_cast<const T*>(b), p, fill, static_cast<T*>(out));
         ZKX_LAUNCH_CHECK();
