@@ -94,3 +94,28 @@ def test_fused_doppler_cfar_windows_vs_fp64(mods, guard, ref):
     near = np.abs(pw.astype(np.float64) - exact) <= 1e-6 * np.abs(exact)
     bad = (det.cpu().numpy() != (pw.astype(np.float64) > exact)) & ~near
     assert not bad.any()
+
+
+@pytest.mark.parametrize("seed,snr_db", [(7, -10.0), (11, 10.0)])
+def test_fused_threshold_accuracy_vs_fp64(mods, seed, snr_db):
+    """Full-size dwell, configured window: every live threshold of the fused
+    Doppler+CFAR kernel lies within 5e-7 relative of the exact FP64 statistic
+    on the same power map, also beside a strong (+10 dB per sample) target,
+    whose rounded horizontal sums must cancel out of the annulus."""
+    radar, synth = mods
+    d = synth.make_dwell(128, 4096, 256, seed=seed, snr_db=snr_db)
+    x = torch.from_numpy(d.x).cuda()
+    pipe = radar.DwellPipeline(d.template, 128, 4096)
+    pc = radar.pulse_compression_prepared(x, pipe.spectrum)
+    pw = torch.empty((128, 4096), dtype=torch.float32, device="cuda")
+    thr, det = radar.doppler_cfar(pc, GUARD, REF, PFA, window="hann", power=pw)
+    torch.cuda.synchronize()
+    p, t, dd = pw.cpu().numpy(), thr.cpu().numpy(), det.cpu().numpy()
+    exact = orc.ca_cfar_exact(p, GUARD, REF, PFA)
+    live = exact != 0
+    rel = np.abs(t.astype(np.float64) - exact)[live] / np.abs(exact[live])
+    print(f"seed {seed}: max threshold rel err {rel.max():.2e}", flush=True)
+    assert rel.max() < 5e-7
+    assert np.all(t[~live] == 0)
+    near = np.abs(p - exact) <= 1e-6 * np.abs(exact)
+    assert not np.any((dd != (p.astype(np.float64) > exact)) & ~near)

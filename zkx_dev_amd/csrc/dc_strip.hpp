@@ -1,0 +1,312 @@
+// Strip-streaming Doppler FFT + |.|^2 + FP64 windowed CA-CFAR.
+//
+// One 256-thread block walks a strip of range bins of one dwell in chunks of
+// CW = 256 / T columns (T = N / 16 threads per Doppler column).  Each chunk is
+// transformed once: its |Y|^2 columns enter a ring in LDS that also carries
+// the last 2*cm columns of the previous chunk, so the CFAR window of every
+// output column is complete without transforming any column twice (only the
+// 2*cm columns either side of a strip are shared with its neighbours).  The
+// samples of the next chunk are loaded into registers while the current chunk
+// is transformed and its CFAR computed (barriers wait on LDS only, so the
+// loads stay in flight across them).
+//
+// Per chunk, with the transform leading the outputs by cm columns:
+//   ring slots [0, 2cm)         columns c_k - 2cm .. c_k - 1 (carried)
+//   ring slots [2cm, 2cm + CW)  chunk k, columns c_k .. c_k + CW - 1
+//   output chunk k - 1          columns c_k - cm .. c_k - cm + CW - 1
+// with c_k = S0 + cm - CW + k*CW for a strip starting at S0 (chunk 0 only
+// fills the carry).  The CFAR statistic is the same as the tile kernel's:
+// FP64 horizontal window sums rounded to float, FP64 vertical sliding sums,
+// threshold (alpha/N)*(outer - inner) in FP64, strict x > T, T = 0 on border
+// cells.  Horizontal runs cover 16 outputs aligned at multiples of 16 and
+// vertical runs 16 rows aligned at multiples of 16, so the result of a cell
+// does not depend on the strip or batch decomposition.
+#pragma once
+
+#include "fft_lds.hpp"
+#include "radar_tiles.hpp"
+
+namespace zkx {
+namespace strip {
+
+// Column-interleaved FFT exchange: element i of column `slot` at row
+// (i ^ bit4(i)) of TC floats.  With TC = 32 a wave holds two row groups t and
+// t + 1; the swizzle puts the rows they touch in one access (i and i + 16, or
+// i and i + 1) at opposite parity, so the two half-waves use disjoint banks.
+template <int TC>
+struct ColLdsSw {
+    float* re;
+    float* im;
+    int slot;
+    __device__ __forceinline__ int a(int i) const { return (i ^ ((i >> 4) & 1)) * TC + slot; }
+    __device__ __forceinline__ float2 load(int i) const
+    {
+        const int k = a(i);
+        return make_float2(re[k], im[k]);
+    }
+    __device__ __forceinline__ void store(int i, float2 v) const
+    {
+        const int k = a(i);
+        re[k] = v.x;
+        im[k] = v.y;
+    }
+};
+
+template <int N, int G0, int G1, int R0, int R1>
+struct StripCfg {
+    static constexpr int E = fft::elems_for(N);
+    static constexpr int T = N / E;
+    static constexpr int THREADS = 256;
+    static constexpr int CW = THREADS / T;      // columns per chunk
+    static constexpr int CM = G1 + R1;          // range half-window
+    static constexpr int RM = G0 + R0;          // Doppler half-window
+    static constexpr int RUN = 16;              // outputs per horizontal task, rows per vertical task
+    static constexpr int RS = CW + 2 * CM + 1;  // ring row stride (odd: rows of a wave on distinct banks)
+    static constexpr int HS = N + 2 * RM + 1;   // window-sum column stride, RM zero rows either side
+    static constexpr int RING_FLOATS = N * RS;
+    static constexpr int H_FLOATS = 2 * CW * HS;
+    static constexpr int FFT_FLOATS = 2 * N * CW;
+    static constexpr int TW_FLOATS = 3 * N;      // LDS copies of the N-point twiddle table and the window
+    static constexpr int LDS_FLOATS = RING_FLOATS + (H_FLOATS > FFT_FLOATS ? H_FLOATS : FFT_FLOATS) + TW_FLOATS;
+    static constexpr size_t LDS_BYTES = sizeof(float) * LDS_FLOATS;
+    static_assert(E == 16 && N % RUN == 0 && CW % RUN == 0, "strip shape");
+    static_assert(CW >= 2 * CM, "the carried columns must come from one chunk");
+    static_assert(N * CW == THREADS * RUN, "one horizontal task per thread");
+};
+
+struct StripArgs {
+    const float2* x;   // [batch][N][R] (P == N)
+    int R, window;
+    int SW;            // outputs per strip (multiple of CW)
+    double alpha_over_n;
+    const float2* tw;  // exp(-2 pi i k / N), k < N
+    float* thr;        // [batch][N][R]
+    unsigned char* det;
+    float2* y;         // optional range-Doppler map [batch][N][R]
+    float* power;      // optional |Y|^2 [batch][N][R]
+};
+
+// Raw buffer access: 32-bit byte offsets from a per-dwell base.  A lane whose
+// offset is kOff (beyond every buffer's size) reads zeros and its stores are
+// dropped by the hardware range check, so edge columns need no branches.
+constexpr unsigned kOff = 0x80000000u;
+constexpr int kBufFlags = 0x00020000;  // raw 32-bit data, gfx9 word 3
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0), static_cast<int>(bytes),
+                                             kBufFlags);
+}
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+// (whole-vector bit casts: subscripting the builtin's u2v result element by
+// element has been narrowed to a single dword load by this compiler)
+__device__ __forceinline__ float2 ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(__amdgpu_buffer_rsrc_t r, float2 v, unsigned voff, unsigned soff)
+{
+    const f2v w = {v.x, v.y};
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, w), r, voff, soff, 0);
+}
+__device__ __forceinline__ void st1(__amdgpu_buffer_rsrc_t r, float v, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void stb(__amdgpu_buffer_rsrc_t r, unsigned char v, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, 0);
+}
+
+// Samples of column `col` (rows t + e*T) of the dwell behind `src`; zeros
+// outside [0, R).
+template <int N, int G0, int G1, int R0, int R1>
+__device__ __forceinline__ void strip_load(__amdgpu_buffer_rsrc_t src, int R, int col, int t,
+                                           float2 (&buf)[StripCfg<N, G0, G1, R0, R1>::E])
+{
+    using C = StripCfg<N, G0, G1, R0, R1>;
+    const unsigned voff = (col >= 0 && col < R) ? (static_cast<unsigned>(t) * R + col) * 8u : kOff;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) buf[e] = ld2(src, voff, static_cast<unsigned>(e) * C::T * R * 8u);
+}
+
+// The whole strip of dwell `dwell`, outputs [S0, S1).  Every thread of the
+// block must call this (it holds barriers).  The launcher guarantees
+// N * R * 8 < 2^31 (all byte offsets fit the buffer range check).
+template <int N, int G0, int G1, int R0, int R1>
+__device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell, int S0, int S1, float* lds)
+{
+    using C = StripCfg<N, G0, G1, R0, R1>;
+    constexpr int E = C::E, T = C::T, CW = C::CW, CM = C::CM, RM = C::RM, RS = C::RS, HS = C::HS;
+    constexpr int RUN = C::RUN, NT = C::THREADS;
+    const int tid = threadIdx.x;
+    const int R = a.R;
+    const long long cells = static_cast<long long>(N) * R;
+    const unsigned ncell = static_cast<unsigned>(cells);
+    const auto src = rsrc(a.x + dwell * cells, ncell * 8u);
+    const auto rthr = rsrc(a.thr + dwell * cells, ncell * 4u);
+    const auto rdet = rsrc(a.det + dwell * cells, ncell);
+    const bool maps = a.y != nullptr || a.power != nullptr;
+    const auto ry = rsrc(a.y ? a.y + dwell * cells : nullptr, a.y ? ncell * 8u : 0u);
+    const auto rpw = rsrc(a.power ? a.power + dwell * cells : nullptr, a.power ? ncell * 4u : 0u);
+
+    // window sums first (LDS offset 0 keeps their ds_read2 offsets in range),
+    // then the ring, then the twiddle and window tables
+    float* hreg = lds;  // window sums; also the FFT exchange buffer
+    float* ring = lds + (C::H_FLOATS > C::FFT_FLOATS ? C::H_FLOATS : C::FFT_FLOATS);
+    float* ho = hreg;                    // outer horizontal sums
+    float* hs = hreg + CW * HS;          // side bands: outer minus inner horizontal sums
+    float2* twl = reinterpret_cast<float2*>(lds + C::LDS_FLOATS - C::TW_FLOATS);
+    float* wtab = lds + C::LDS_FLOATS - N;
+
+    const int s = tid % CW, t = tid / CW;
+    // the window (built once per block in FP64 and rounded) and the twiddle
+    // table, both kept in LDS for the whole strip
+    for (int p = tid; p < N; p += NT) {
+        wtab[p] = static_cast<float>(tiles::window_value(a.window, p, N));
+        twl[p] = a.tw[p];
+    }
+    __syncthreads();
+
+    const int nout = (S1 - S0 + CW - 1) / CW;
+    const ColLdsSw<CW> L{hreg, hreg + N * CW, s};
+    const fft::LdsTwiddles TWL{twl};
+
+    // vertical task of this thread: column vo, rows f0 .. f0 + RUN - 1; with
+    // CW = 32 the two half-waves take runs 32 rows apart (disjoint banks)
+    int vo, vrun;
+    if constexpr (CW == 32) {
+        const int w = tid >> 6, h = (tid >> 5) & 1;
+        vo = tid & 31;
+        vrun = (w >> 1) * 4 + (w & 1) + 2 * h;
+    } else {
+        vo = tid % CW;
+        vrun = tid / CW;
+    }
+    const int f0 = vrun * RUN;
+    // horizontal task: row hf, outputs hrun*RUN ..
+    const int hf = tid % N, hrun = tid / N;
+    // pad rows of the window sums this thread clears each chunk: ZPT of the
+    // 2*RM pad rows of one (plane, column)
+    constexpr int ZTPC = NT / (2 * CW);                    // threads per (plane, column)
+    constexpr int ZPT = (2 * RM + ZTPC - 1) / ZTPC;        // pad rows per thread
+    const int zpc = tid / ZTPC, zpart = tid % ZTPC;
+    float* zbase = hreg + zpc * HS;  // (plane, column) zpc: plane * CW * HS + column * HS
+    // carried columns this thread copies: CPT of the 2*CM of one row
+    constexpr int CTPR = NT / N;                           // threads per ring row
+    constexpr int CPT = (2 * CM + CTPR - 1) / CTPR;
+    const int crow = tid / CTPR, cpart = tid % CTPR;
+    float* cbase = ring + crow * RS + cpart * CPT;
+    static_assert(NT % (2 * CW) == 0 && NT % N == 0, "pad/carry split");
+
+    float2 buf[E];
+    strip_load<N, G0, G1, R0, R1>(src, R, S0 + CM - CW + s, t, buf);
+    for (int k = 0; k <= nout; ++k) {
+        const int ck = S0 + CM - CW + k * CW;  // first column of chunk k
+        float2 v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cscale(buf[e], wtab[t + e * T]);
+        // the samples are consumed before the next chunk's loads reuse their
+        // registers (keeps the scheduler from hoisting the loads and copying)
+        __builtin_amdgcn_sched_barrier(0);
+        // (after the last chunk the column is out of range: zeros, no phi)
+        strip_load<N, G0, G1, R0, R1>(src, R, k < nout ? ck + CW + s : -1, t, buf);
+        fft::transform<N, E, false>(v, L, TWL, t);
+        // |Y|^2 into ring slots 2CM + s
+#pragma unroll
+        for (int e = 0; e < E; ++e) ring[(t + e * T) * RS + 2 * CM + s] = power_of(v[e]);
+        if (maps) {  // optional map/power outputs for the strip's own columns
+            const int col = ck + s;
+            const bool own = col >= S0 && col < S1;
+            const unsigned vo8 = own ? (static_cast<unsigned>(t) * R + col) * 8u : kOff;
+            const unsigned vo4 = own ? (static_cast<unsigned>(t) * R + col) * 4u : kOff;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const unsigned rowoff = static_cast<unsigned>(e) * T * R;
+                st2(ry, v[e], vo8, rowoff * 8u);
+                st1(rpw, power_of(v[e]), vo4, rowoff * 4u);
+            }
+        }
+        __syncthreads();
+        if (k > 0) {
+            // ---- horizontal FP64 window sums of output chunk k-1 ------------
+            // zero the RM rows either side of every column's sums (the FFT
+            // exchange above reused this region)
+#pragma unroll
+            for (int j = 0; j < ZPT; ++j) {
+                const int z = zpart * ZPT + j;
+                if (z < 2 * RM) zbase[z < RM ? z : N + z] = 0.f;
+            }
+            {
+                const float* rr = ring + hf * RS + hrun * RUN;  // output i's window: rr[i .. i + 2CM]
+                // FP64 prefix sums of the run's window values: P[q] = sum of
+                // the first q values, so every window sum is one difference
+                double pre[RUN + 2 * CM + 1];
+                pre[0] = 0.0;
+#pragma unroll
+                for (int q = 0; q < RUN + 2 * CM; ++q) pre[q + 1] = pre[q] + static_cast<double>(rr[q]);
+                float* hoc = ho + (hrun * RUN) * HS + RM + hf;
+                float* hsc = hs + (hrun * RUN) * HS + RM + hf;
+#pragma unroll
+                for (int i = 0; i < RUN; ++i) {
+                    const double so = pre[i + 2 * CM + 1] - pre[i];
+                    const double si = pre[i + CM + G1 + 1] - pre[i + CM - G1];
+                    // the side bands are formed in FP64 before rounding, so a
+                    // strong cell inside the guard columns cancels exactly
+                    hoc[i * HS] = static_cast<float>(so);
+                    hsc[i * HS] = static_cast<float>(so - si);
+                }
+            }
+            __syncthreads();
+            // ---- vertical FP64 sliding sums, threshold, detection ------------
+            // annulus = (rows |df| <= RM of H_outer) - (rows |df| <= G0 of
+            // H_outer) + (rows |df| <= G0 of the side bands); every rounded
+            // horizontal sum that holds a guard cell enters both of the first
+            // two terms and cancels.
+            {
+                const float* hoc = ho + vo * HS + f0;           // row f0 - RM + q at hoc[q]
+                const float* hsc = hs + vo * HS + f0 + RM - G0;  // row f0 - G0 + q at hsc[q]
+                // FP64 prefix sums down the column of outer sums (Q) and of
+                // side bands (Z)
+                double qv[RUN + 2 * RM + 1], zv[RUN + 2 * G0 + 1];
+                qv[0] = 0.0;
+                zv[0] = 0.0;
+#pragma unroll
+                for (int q = 0; q < RUN + 2 * RM; ++q) qv[q + 1] = qv[q] + static_cast<double>(hoc[q]);
+#pragma unroll
+                for (int q = 0; q < RUN + 2 * G0; ++q) zv[q + 1] = zv[q] + static_cast<double>(hsc[q]);
+                const int c = ck - CM + vo;  // output column of ring slot vo (window slots vo .. vo + 2CM)
+                const bool colin = c >= CM && c < R - CM;
+                const bool store = c < S1 && c < R;
+                const unsigned voff4 = store ? (static_cast<unsigned>(f0) * R + c) * 4u : kOff;
+                const unsigned voff1 = store ? static_cast<unsigned>(f0) * R + c : kOff;
+                const float* xr = ring + f0 * RS + CM + vo;
+#pragma unroll
+                for (int i = 0; i < RUN; ++i) {
+                    // rows of H_outer at |df| in (G0, RM] (the two bands above
+                    // and below the guard rows) plus the guard rows' side bands
+                    const double bands = (qv[i + 2 * RM + 1] - qv[i + RM + G0 + 1]) + (qv[i + RM - G0] - qv[i]);
+                    const double side = zv[i + 2 * G0 + 1] - zv[i];
+                    const int f = f0 + i;
+                    const bool live = colin && f >= RM && f < N - RM;
+                    const double th = live ? a.alpha_over_n * (bands + side) : 0.0;
+                    const float xv = xr[i * RS];
+                    const unsigned row = static_cast<unsigned>(i) * R;
+                    st1(rthr, static_cast<float>(th), voff4, row * 4u);
+                    stb(rdet, static_cast<double>(xv) > th ? 1 : 0, voff1, row);
+                }
+            }
+        }
+        __syncthreads();
+        // carry the last 2CM columns of this chunk to ring slots [0, 2CM)
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            if (cpart * CPT + j < 2 * CM) cbase[j] = cbase[CW + j];
+        }
+    }
+}
+
+}  // namespace strip
+}  // namespace zkx

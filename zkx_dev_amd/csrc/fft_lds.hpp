@@ -120,10 +120,37 @@ struct ColLds {
     }
 };
 
+// Twiddle sources.  A pass of radix R at stride Ns needs w^r, r < R, with
+// w = exp(-2 pi i jm / (Ns*R)) = tw[jm * N/(Ns*R)].
+// From the global table: one read per butterfly, the other powers by a
+// product tree of depth <= 4 (w^r = w^p * w^(r-p), p the largest power of two
+// <= r).
+template <int N, int Ns, int R>
+__device__ __forceinline__ void twiddle_powers(const float2* __restrict__ tw, int jm, float2 (&wp)[R])
+{
+    wp[1] = tw[jm * (N / (Ns * R))];
+#pragma unroll
+    for (int r = 2; r < R; ++r) {
+        const int p = (r & 8) ? 8 : (r & 4) ? 4 : (r & 2) ? 2 : 1;
+        wp[r] = (p == r) ? cmul(wp[r / 2], wp[r / 2]) : cmul(wp[p], wp[r - p]);
+    }
+}
+// From a copy of the N-entry table in LDS: every power read directly
+// (jm * r < Ns * R <= N, so no index wraps).
+struct LdsTwiddles {
+    const float2* t;
+};
+template <int N, int Ns, int R>
+__device__ __forceinline__ void twiddle_powers(LdsTwiddles tw, int jm, float2 (&wp)[R])
+{
+#pragma unroll
+    for (int r = 1; r < R; ++r) wp[r] = tw.t[jm * r * (N / (Ns * R))];
+}
+
 // Stockham passes from stride Ns on.  tw[k] = exp(-2 pi i k / N), k < N.
 // Every thread of the block must call this (it contains barriers).
-template <int N, int E, int Ns, bool INV, class Lds>
-__device__ __forceinline__ void passes(float2 (&v)[E], const Lds& lds, const float2* __restrict__ tw, int t)
+template <int N, int E, int Ns, bool INV, class Lds, class TW>
+__device__ __forceinline__ void passes(float2 (&v)[E], const Lds& lds, TW tw, int t)
 {
     constexpr int REM = N / Ns;
     constexpr int R = REM >= E ? E : REM;
@@ -142,15 +169,8 @@ __device__ __forceinline__ void passes(float2 (&v)[E], const Lds& lds, const flo
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int jm = (t + b * T) & (Ns - 1);
-            // one table read per butterfly; powers w^r by a product tree of
-            // depth <= 4 (w^r = w^p * w^(r-p), p the largest power of two <= r)
             float2 wp[R];
-            wp[1] = tw[jm * (N / (Ns * R))];
-#pragma unroll
-            for (int r = 2; r < R; ++r) {
-                const int p = (r & 8) ? 8 : (r & 4) ? 4 : (r & 2) ? 2 : 1;
-                wp[r] = (p == r) ? cmul(wp[r / 2], wp[r / 2]) : cmul(wp[p], wp[r - p]);
-            }
+            twiddle_powers<N, Ns, R>(tw, jm, wp);
 #pragma unroll
             for (int r = 1; r < R; ++r) v[b + r * NB] = INV ? cmulc(v[b + r * NB], wp[r]) : cmul(v[b + r * NB], wp[r]);
         }
@@ -180,8 +200,8 @@ __device__ __forceinline__ void passes(float2 (&v)[E], const Lds& lds, const flo
 
 // Full transform: v holds x[t + k*T] on entry and X[t + k*T] on exit.
 // The inverse is unscaled here; callers fold 1/N into their store.
-template <int N, int E, bool INV, class Lds>
-__device__ __forceinline__ void transform(float2 (&v)[E], const Lds& lds, const float2* __restrict__ tw, int t)
+template <int N, int E, bool INV, class Lds, class TW>
+__device__ __forceinline__ void transform(float2 (&v)[E], const Lds& lds, TW tw, int t)
 {
     passes<N, E, 1, INV>(v, lds, tw, t);
 }

@@ -11,6 +11,7 @@
 //   Task1 chain        ZKX/Task1/pipeline/src/steps/step{2,3,4}/step{2,3,4}.cu
 #include "fft_lds.hpp"
 #include "radar_tiles.hpp"
+#include "dc_strip.hpp"
 #include "runtime.hpp"
 
 #include <algorithm>
@@ -860,6 +861,17 @@ doppler_cfar_kernel(const float2* __restrict__ x, int P, int R, int window, cons
     tiles::doppler_cfar_tile<N, CG0, CG1, CR0, CR1>(a, tile, threadIdx.x, blockDim.x, lds, tw);
 }
 
+// Strip-streaming form (dc_strip.hpp): block (strip, dwell) walks the strip's
+// range bins in chunks, transforming every column once.
+template <int N, int G0, int G1, int R0, int R1>
+__global__ void __launch_bounds__(256, 2) doppler_cfar_strip_kernel(strip::StripArgs a)
+{
+    extern __shared__ float lds[];
+    const int S0 = blockIdx.x * a.SW;
+    const int S1 = min(S0 + a.SW, a.R);
+    strip::doppler_cfar_strip<N, G0, G1, R0, R1>(a, blockIdx.y, S0, S1, lds);
+}
+
 // ---------------------------------------------------------------------------
 // dispatch helpers
 // ---------------------------------------------------------------------------
@@ -1165,6 +1177,42 @@ int cfar_window_2d(const float* x, int batch, int rows, int cols, int g0, int g1
     return ZKX_SUCCESS;
 }
 
+// Doppler+CFAR form: 0 strip-streaming (configured cells, 32 <= N <= 128),
+// 1 one tile of 64 transformed bins per block; and the strip kernel's target
+// block count per launch (strips per dwell = ceil(target / batch)).
+int g_dc_variant = 0;
+int g_dc_strip_blocks = 1024;
+
+template <int N>
+int launch_doppler_cfar_strip(const float2* x, int batch, int R, int window, double alpha_over_n,
+                              float* thr, unsigned char* det, float2* y, float* power, const float2* tw,
+                              hipStream_t st)
+{
+    using C = strip::StripCfg<N, 2, 2, 8, 8>;
+    int rc = allow_lds(doppler_cfar_strip_kernel<N, 2, 2, 8, 8>, C::LDS_BYTES);
+    if (rc) return rc;
+    const int chunks = (R + C::CW - 1) / C::CW;
+    int strips = (g_dc_strip_blocks + batch - 1) / batch;
+    strips = std::max(1, std::min(strips, chunks));
+    const int sw = ((chunks + strips - 1) / strips) * C::CW;
+    strips = (R + sw - 1) / sw;
+    strip::StripArgs a;
+    a.x = x;
+    a.R = R;
+    a.window = window;
+    a.SW = sw;
+    a.alpha_over_n = alpha_over_n;
+    a.tw = tw;
+    a.thr = thr;
+    a.det = det;
+    a.y = y;
+    a.power = power;
+    hipLaunchKernelGGL((doppler_cfar_strip_kernel<N, 2, 2, 8, 8>), dim3(strips, batch), dim3(C::THREADS),
+                       C::LDS_BYTES, st, a);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
 template <int N>
 int launch_doppler_cfar(const float2* x, int batch, int P, int R, int window, int g0, int g1, int r0, int r1,
                         double pfa, float* thr, unsigned char* det, float2* y, float* power, hipStream_t st,
@@ -1189,6 +1237,14 @@ int launch_doppler_cfar(const float2* x, int batch, int P, int R, int window, in
     const int count = 2 * r0 * (2 * r1 + 2 * g1 + 1) + 2 * (2 * g0 + 1) * r1;
     const double n = static_cast<double>(count);
     const double alpha = n * (std::pow(pfa, -1.0 / n) - 1.0);
+    if constexpr (N >= 32) {
+        // (buffer byte offsets of a dwell must stay below 2^31)
+        if (fixed && g_dc_variant == 0 && P == N && static_cast<long long>(N) * R * 8 < (1LL << 31)) {
+            rc = launch_doppler_cfar_strip<N>(x, batch, R, window, alpha / n, thr, det, y, power, tw, st);
+            if (rc == ZKX_SUCCESS) *done = true;
+            return rc;
+        }
+    }
     dim3 grid((R + TCO - 1) / TCO, batch);
     if (fixed)
         hipLaunchKernelGGL((doppler_cfar_kernel<N, 2, 2, 8, 8>), grid, dim3(C::THREADS), lds, st, x, P, R, window, tw,
@@ -1271,6 +1327,16 @@ int zkx_set_kernel_variant(const char* op, int variant)
         g_fir_fft = variant;
         return ZKX_SUCCESS;
     }
+    if (name == "doppler_cfar") {
+        if (variant < 0 || variant > 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "doppler_cfar variant out of range");
+        g_dc_variant = variant;
+        return ZKX_SUCCESS;
+    }
+    if (name == "doppler_cfar_strip_blocks") {
+        if (variant < 1 || variant > (1 << 20)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "strip block target out of range");
+        g_dc_strip_blocks = variant;
+        return ZKX_SUCCESS;
+    }
 
     ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unknown kernel variant op");
 }
@@ -1282,6 +1348,8 @@ int zkx_get_kernel_variant(const char* op)
     if (name == "pulse_compression") return g_pc_variant;
     if (name == "pulse_doppler") return g_doppler_variant;
     if (name == "fir_fft") return g_fir_fft;
+    if (name == "doppler_cfar") return g_dc_variant;
+    if (name == "doppler_cfar_strip_blocks") return g_dc_strip_blocks;
     return -1;
 }
 
