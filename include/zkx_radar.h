@@ -148,6 +148,22 @@ int zkx_ca_cfar_f32(const float* x, int rank, const int* shape, const int* guard
                     float* workspace, void* stream);
 
 /*
+ * zkx_ca_cfar_f32 without any allocation: the integral image lives in a fixed
+ * device array of zkx_ca_cfar_noalloc_max_cells() cells (4194304 FP32, 16 MiB,
+ * held on every device the library runs on until the process ends).  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Maps larger
+ * than the scratch return ZKX_INVALID_SHAPE before any launch; results are
+ * bit-identical to zkx_ca_cfar_f32.  Every call on one device shares that
+ * device's scratch: calls that overlap on one device (other streams or host
+ * threads) race and must be ordered by the caller.
+ */
+long long zkx_ca_cfar_noalloc_max_cells(void);
+int zkx_ca_cfar_f32_noalloc(const float* x, int rank, const int* shape, const int* guard,
+                            const int* reference, double pfa, float* threshold,
+                            uint8_t* detections, void* stream);
+
+/*
  * Windowed CA-CFAR for rank 2 with FP64 box sums built as 2-D prefix sums in
  * LDS: threshold T = (alpha/N)*(outer-inner) with alpha and the sums in FP64,
  * det = x > T compared in FP64, T stored as float.  Borders as above.  This is

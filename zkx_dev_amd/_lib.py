@@ -34,6 +34,8 @@ SIGNATURES = {
     "zkx_pulse_doppler_c64": (I, [P, I, I, I, I, P, P, P, P]),
     "zkx_cfar_alpha": (I, [D, I, ctypes.POINTER(D)]),
     "zkx_ca_cfar_f32": (I, [P, I, P, P, P, D, P, P, P, P]),
+    "zkx_ca_cfar_noalloc_max_cells": (LL, []),
+    "zkx_ca_cfar_f32_noalloc": (I, [P, I, P, P, P, D, P, P, P]),
     "zkx_ca_cfar_window_f32": (I, [P, I, I, I, I, I, I, D, P, P, P]),
     "zkx_widen_c64_to_c128": (I, [P, P, SZ, P]),
     "zkx_widen_f32_to_f64": (I, [P, P, SZ, P]),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <string>
 
 namespace zkx {
@@ -1303,7 +1304,68 @@ int pc_template_dispatch(int n, const float2* h, int taps, int normalize, int wi
 
 using namespace zkx;
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch for the allocation-free CA-CFAR entry point: the FP32
+// integral image of one map of at most kNoAllocCfarCells cells (16 MiB, held
+// by every device the library runs on until the process ends).  Host code
+// never names `scratch` directly: its device address is looked up once per
+// device and cached.
+// ---------------------------------------------------------------------------
+constexpr long long kNoAllocCfarCells = 1LL << 22;
+static __device__ __align__(256) float scratch[kNoAllocCfarCells];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device, with that device current).
+int scratch_address(float** out)
+{
+    int dev = 0;
+    ZKX_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) ZKX_FAIL(ZKX_INTERNAL_ERROR, "ca_cfar_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_last_error(e != hipSuccess ? hip_status_to_code(e) : ZKX_RUNTIME_ERROR,
+                           "ca_cfar_noalloc: scratch address lookup failed");
+            return e != hipSuccess ? hip_status_to_code(e) : ZKX_RUNTIME_ERROR;
+        }
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ZKX_SUCCESS;
+}
+}  // namespace
+
 extern "C" {
+
+long long zkx_ca_cfar_noalloc_max_cells(void) { return kNoAllocCfarCells; }
+
+int zkx_ca_cfar_f32_noalloc(const float* x, int rank, const int* shape, const int* guard, const int* reference,
+                            double pfa, float* threshold, uint8_t* detections, void* stream)
+{
+    clear_last_error();
+    if (rank == 1 || rank == 2) {
+        if (!shape) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ca_cfar: null pointer");
+        long long count = 1;
+        for (int d = 0; d < rank; ++d) count *= shape[d] > 0 ? shape[d] : 0;
+        if (count > kNoAllocCfarCells)
+            ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar_noalloc: the map exceeds the 4194304-cell scratch");
+    }
+    float* ws = nullptr;
+    if (rank == 1 || rank == 2) {
+        const int rc = scratch_address(&ws);
+        if (rc) return rc;
+    }
+    // the integral image is written in full before it is read, so the
+    // scratch needs no clearing between calls
+    return zkx_ca_cfar_f32(x, rank, shape, guard, reference, pfa, threshold, detections, ws, stream);
+}
 
 int zkx_max_lds_fft_length(void) { return kMaxLdsFft; }
 
