@@ -12,9 +12,12 @@
  *   - plans are opaque handles;
  *   - FFTResult codes 0 ok, 1 invalid value, 2 alloc failed, 3 exec failed,
  *     99 internal error.
- * Lengths: powers of two 1..8192 (one LDS-resident pass per row) and any
- * other length up to 4096 (Bluestein, one fused kernel per row); longer
- * plans return FFT_INVALID_VALUE.  Plan kinds: 1-D C2C, R2C, C2R (with the
+ * Lengths: powers of two 1..8192 (one LDS-resident pass per row), other
+ * lengths up to 4096 (Bluestein, one fused kernel per row), and beyond those
+ * powers of two up to 2^26 (four-step over the LDS passes) and other lengths
+ * up to 2^25 (Bluestein in global passes); longer plans return
+ * FFT_INVALID_VALUE.  A plan allocates its tables and scratch at creation,
+ * so execution allocates nothing.  Plan kinds: 1-D C2C, R2C, C2R (with the
  * reference backend's half-spectrum mirroring and scaling,
  * fft_thrust.cu:1096-1160) and 2-D (complex data for every type, as
  * fft_thrust.cu:1014-1058).  fft_execute* run on the default stream, as the

@@ -42,9 +42,11 @@ def test_fft_parseval_and_roundtrip(Fm):
 
 def test_fft_plan_errors(Fm):
     with pytest.raises(Fm.FFTError):
-        Fm.Plan(5000)  # Bluestein lengths up to 4096
+        Fm.Plan(0)
     with pytest.raises(Fm.FFTError):
-        Fm.Plan(16384)
+        Fm.Plan((1 << 26) * 2)  # powers of two up to 2^26
+    with pytest.raises(Fm.FFTError):
+        Fm.Plan((1 << 25) + 1)  # other lengths up to 2^25
     with pytest.raises(Fm.FFTError):
         Fm.Plan(64, batch=0)
     with pytest.raises(Fm.FFTError):
@@ -64,7 +66,31 @@ def test_fft_bluestein_vs_numpy(Fm, n, inverse):
     assert orc.rel_l2(y, ref) < 1e-5
 
 
-@pytest.mark.parametrize("n", [8, 64, 4096, 6, 9, 100, 1001])
+@pytest.mark.parametrize("n,batch", [(16384, 3), (32768, 2), (1 << 16, 2), (1 << 20, 1), (1 << 22, 1),
+                                     (4097, 3), (5000, 2), (8191, 2), (10007, 2), (65537, 1), (100000, 1),
+                                     (1 << 21 | 1, 1)])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fft_long_lengths_vs_numpy(Fm, n, batch, inverse):
+    """Lengths beyond one LDS pass: four-step for powers of two above 8192,
+    global-pass Bluestein for other lengths above 4096 (the reference backend
+    takes any length)."""
+    print(f"fft n={n} batch={batch} inverse={inverse}", flush=True)
+    rng = np.random.default_rng(n % 1000 + 3)
+    x = (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))).astype(np.complex64)
+    y = Fm.fft(torch.from_numpy(x).cuda(), inverse=inverse).cpu().numpy()
+    ref = np.fft.ifft(x.astype(np.complex128), axis=1) if inverse else np.fft.fft(x.astype(np.complex128), axis=1)
+    assert orc.rel_l2(y, ref) < 1e-5
+
+
+@pytest.mark.parametrize("ny,nx", [(3, 10007), (20000, 5), (2, 16384)])
+def test_fft2_long_axis_vs_numpy(Fm, ny, nx):
+    rng = np.random.default_rng(ny + nx)
+    x = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(np.complex64)
+    y = Fm.fft2(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert orc.rel_l2(y, np.fft.fft2(x.astype(np.complex128))) < 1e-5
+
+
+@pytest.mark.parametrize("n", [8, 64, 4096, 6, 9, 100, 1001, 10007, 16384])
 def test_rfft_and_irfft_reference_semantics(Fm, n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal((3, n)).astype(np.float32)

@@ -87,8 +87,10 @@ def test_pulse_compression_delta_template_is_shift(R):
 
 @pytest.mark.parametrize("P,Rg,nfft", [(16, 64, -1), (32, 100, -1), (128, 256, -1), (128, 96, 256),
                                        (64, 33, 32), (512, 48, -1), (8, 5, 8), (2, 7, 4),
-                                       # other lengths: split-BF16 matrix-core DFT (error ~4e-6)
-                                       (100, 64, -1), (48, 30, 60), (130, 20, 100), (3, 9, 5), (1000, 8, -1)])
+                                       # other lengths: FP32 row transforms after a corner turn
+                                       (100, 64, -1), (48, 30, 60), (130, 20, 100), (3, 9, 5), (1000, 8, -1),
+                                       (40, 6, 1), (5000, 4, -1), (300, 5, 5003), (20000, 3, 16384),
+                                       (64, 7, 16384)])
 @pytest.mark.parametrize("window", ["none", "hann", "hamming"])
 def test_pulse_doppler_vs_fp64(R, P, Rg, nfft, window):
     rng = np.random.default_rng(P * 7 + Rg)
@@ -225,3 +227,37 @@ def test_cfar_alpha(R):
         R.cfar_alpha(0.0, 10)
     with pytest.raises(ValueError):
         R.cfar_alpha(0.5, 0)
+
+
+def test_output_tensors_validated_before_launch(R):
+    """Every caller-supplied output is checked before any kernel launch: a
+    host tensor, a short one and an expanded (non-contiguous) one each raise
+    ValueError."""
+    from zkx_dev_amd import convolution, fft
+    rng = np.random.default_rng(5)
+    x = dev(rng_c64(rng, (16, 64)))
+    spec = R.pc_template_spectrum(dev(rng_c64(rng, (8,))), 64)
+    pw = dev(np.abs(rng_c64(rng, (32, 64))).astype(np.float32))
+    a = torch.ones(100, device="cuda")
+    bad_c64 = [torch.empty((16, 64), dtype=torch.complex64), torch.empty(16 * 64 - 1, dtype=torch.complex64,
+               device="cuda"), torch.empty(1, dtype=torch.complex64, device="cuda").expand(16 * 64)]
+    bad_f32 = [torch.empty((32, 64)), torch.empty(32 * 64 - 1, device="cuda"),
+               torch.empty(1, device="cuda").expand(32 * 64)]
+    for out in bad_c64:
+        with pytest.raises(ValueError):
+            R.pulse_compression_prepared(x, spec, out=out)
+        with pytest.raises(ValueError):
+            R.pulse_doppler(x, out=out)
+        with pytest.raises(ValueError):
+            fft.Plan(64, batch=16).execute(x.reshape(-1), out=out)
+    for out in bad_f32:
+        with pytest.raises(ValueError):
+            R.ca_cfar(pw, (1, 1), (2, 2), 1e-3, threshold=out)
+        with pytest.raises(ValueError):
+            R.ca_cfar_windowed(pw, (1, 1), (2, 2), 1e-3, threshold=out)
+    for out in (torch.empty(100), torch.empty(99, device="cuda"), torch.empty(1, device="cuda").expand(100)):
+        with pytest.raises(ValueError):
+            convolution.correlate(a, a[:1], "same", out=out)
+    with pytest.raises(ValueError):
+        R.doppler_cfar(dev(rng_c64(rng, (32, 128))), (2, 2), (8, 8), 1e-6,
+                       power=torch.empty(32 * 128 - 1, device="cuda"))

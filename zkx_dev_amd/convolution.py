@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import torch
 
+from ._check import out_tensor
 from ._lib import check, lib
 from .radar import _ptr, _stream
 
@@ -42,6 +43,7 @@ def correlate(in1, in2, mode="full", method="auto", *, out=None, stream=None):
         out = torch.empty(max(n, 0), dtype=a.dtype, device=a.device)
     elif out.numel() != n or out.dtype != a.dtype:
         raise ValueError("typed correlate output shape mismatch")
+    out_tensor(out, "out", a, a.dtype, max(n, 0))
     fn = lib().zkx_correlate_fft if method == "fft" else lib().zkx_correlate_direct
     code = fn(_ptr(a), a.numel(), _ptr(b), b.numel(), _DTYPES[a.dtype], _MODES[mode], _ptr(out), _stream(stream))
     if code in (1001, 1002):
@@ -84,6 +86,7 @@ def correlate2d(in1, in2, mode="full", boundary="fill", fillvalue=0, *, out=None
         out = torch.empty(shape, dtype=a.dtype, device=a.device)
     elif tuple(out.shape) != shape or out.dtype != a.dtype or not out.is_contiguous():
         raise ValueError("typed correlate2d output shape mismatch")
+    out_tensor(out, "out", a, a.dtype, shape[0] * shape[1])
     code = lib().zkx_correlate2d(_ptr(a), a.shape[0], a.shape[1], _ptr(b), b.shape[0], b.shape[1],
                                  _DTYPES[a.dtype], _MODES[mode], _BOUNDARIES[boundary], float(fillvalue),
                                  _ptr(out), _stream(stream))

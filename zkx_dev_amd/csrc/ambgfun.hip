@@ -348,7 +348,8 @@ int zkx_ambgfun_c64(const zkx_complex64* x, size_t x_count, int nx, const zkx_co
     const size_t norm_bytes = 256;
     const size_t ws_bytes = norm_bytes + (cut == 2 ? sizeof(float2) * static_cast<size_t>(nx + ny) : 0);
     void* ws = nullptr;
-    int rc = t_amb_ws.reserve(ws_bytes, &ws);
+    zkx::WorkspaceLease lease_(t_amb_ws, st);
+    int rc = lease_.get(ws_bytes, &ws);
     if (rc) return rc;
     float* norms = static_cast<float*>(ws);
     hipLaunchKernelGGL(amb_norm_kernel, dim3(2), dim3(kNormThreads), 0, st, xd, static_cast<long long>(x_count),

@@ -277,7 +277,8 @@ int zkx_channelize_poly_f32(const float* x, long long x_count, const float* h, l
         ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "channelize_poly channel counts that are not powers of two need M <= 4096");
     if (M > 8192) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "channelize_poly needs M <= 8192");
     void* ws = nullptr;
-    int rc = t_chan_ws.reserve(2 * sizeof(float2) * static_cast<size_t>(total), &ws);
+    zkx::WorkspaceLease lease_(t_chan_ws, st);
+    int rc = lease_.get(2 * sizeof(float2) * static_cast<size_t>(total), &ws);
     if (rc) return rc;
     float2* f = static_cast<float2*>(ws);
     float2* g = f + total;

@@ -13,6 +13,7 @@
 // A thread runs the chains of 8 consecutive outputs over one sliding window of
 // in1 staged in LDS; each chain keeps the reference's order, so results are
 // bit-identical to it.
+#include "fft_any.hpp"
 #include "runtime.hpp"
 
 #include <hip/hip_fp16.h>
@@ -463,8 +464,14 @@ int correlate_fft(const void* in1, long long l1, const void* in2, long long l2, 
                   void* out, hipStream_t st)
 {
     const long long full = l1 + l2 - 1;
+    // both operands longer than the overlap-save kernel takes: one
+    // whole-length transform pair instead
+    const bool whole = std::min(l1, l2) > 4096;
+    const size_t conv_bytes = whole ? zkx::fft_convolve_workspace_bytes(l1, l2) : 0;
     void* ws = nullptr;
-    int rc = t_corr_fft_ws.reserve(sizeof(float2) * static_cast<size_t>(l1 + l2 + full), &ws);
+    zkx::WorkspaceLease lease_(t_corr_fft_ws, st);
+    const size_t io_bytes = (sizeof(float2) * static_cast<size_t>(l1 + l2 + full) + 255) / 256 * 256;
+    int rc = lease_.get(io_bytes + conv_bytes, &ws);
     if (rc) return rc;
     float2* A = static_cast<float2*>(ws);
     float2* B = A + l1;
@@ -475,6 +482,14 @@ int correlate_fft(const void* in1, long long l1, const void* in2, long long l2, 
     hipLaunchKernelGGL((to_c64_kernel<T>), dim3(static_cast<unsigned>((l2 + 255) / 256)), dim3(256), 0, st,
                        static_cast<const T*>(in2), l2, 1, B);
     ZKX_LAUNCH_CHECK();
+    if (whole) {
+        rc = zkx::fft_convolve_full(A, l1, B, l2, Y, static_cast<char*>(ws) + io_bytes, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL((from_c64_kernel<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, Y,
+                           start, n, static_cast<T*>(out));
+        ZKX_LAUNCH_CHECK();
+        return ZKX_SUCCESS;
+    }
     // the full convolution is symmetric in its operands: the longer one is
     // the signal, the shorter (at most 4096 samples) the kernel
     const bool swap = l2 > l1;
@@ -521,8 +536,8 @@ int zkx_correlate_fft(const void* in1, long long l1, const void* in2, long long 
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate input exceeds 32-bit indexing");
     const long long n = zkx_correlate_out_len(l1, l2, mode);
     if (n == 0) return ZKX_SUCCESS;
-    if (std::min(l1, l2) > 4096)
-        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "typed correlate fft method needs one input of at most 4096 samples");
+    if (l1 + l2 - 1 > zkx::kMaxAnyFftPow2)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "typed correlate fft method needs l1 + l2 - 1 <= 2^26");
     if (!in1 || !in2 || !out) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate: null pointer");
     if (overlap(out, es * n, in1, es * l1) || overlap(out, es * n, in2, es * l2))
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate output must not alias an input");

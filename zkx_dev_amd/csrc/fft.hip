@@ -5,7 +5,8 @@
 //   * C2C of any length: powers of two up to 8192 in one LDS pass per row,
 //     other lengths up to 4096 by Bluestein's chirp-z method (one fused
 //     kernel per row: chirp, FFT_M, x B, IFFT_M, chirp; M = 2^ceil(log2(2n))),
-//     inverse scaled by 1/n (fft_thrust.cu:700-800);
+//     longer ones (powers of two to 2^26, others to 2^25) by the global-pass
+//     forms of fft_any.hip; inverse scaled by 1/n (fft_thrust.cu:700-800);
 //   * R2C: forward transform of real rows, first n/2+1 bins
 //     (fft_thrust.cu:1096-1120);
 //   * C2R: the half spectrum is mirrored as the reference does
@@ -19,6 +20,7 @@
 //   * shift, frequency and error-string helpers as fft_thrust.cu:520-550,
 //     1228-1361, including fft_shift == fft_inverse_shift for 1-D arrays.
 #include "../../include/zkx_fft.h"
+#include "fft_any.hpp"
 #include "fft_lds.hpp"
 #include "runtime.hpp"
 
@@ -63,6 +65,7 @@ struct Plan {
     Chirp cx, cy;
     float2* work = nullptr;  // scratch: full-length complex rows / transposes
     size_t work_elems = 0;
+    void* any_ws = nullptr;  // scratch of the global-pass (long) transforms
 };
 
 // ---------------------------------------------------------------------------
@@ -251,15 +254,16 @@ int bluestein(const float2* x, long long rows, int n, const Chirp& c, bool inver
 // Transform rows of length n (any supported length) with the 1/n-style
 // scale factor `scale` applied to the result.
 int rows_fft(const float2* x, long long rows, int n, const Chirp& c, bool inverse, float scale, float2* y,
-             hipStream_t st)
+             void* any_ws, hipStream_t st)
 {
     if (n == 1) {
         // a 1-point transform is the identity (every scale here is 1/1)
         if (x != y) ZKX_HIP_CHECK(hipMemcpyAsync(y, x, sizeof(float2) * rows, hipMemcpyDeviceToDevice, st));
         return ZKX_SUCCESS;
     }
-    if (pow2(n)) return zkx::fft_rows_dispatch(n, x, rows, inverse, scale, y, st);
-    return bluestein(x, rows, n, c, inverse, scale, y, st);
+    if (pow2(n) && n <= kMaxPow2) return zkx::fft_rows_dispatch(n, x, rows, inverse, scale, y, st);
+    if (!pow2(n) && n <= kMaxBluestein) return bluestein(x, rows, n, c, inverse, scale, y, st);
+    return zkx::fft_any_rows(x, rows, n, inverse, scale, y, any_ws, st);
 }
 
 void free_chirp(Chirp& c)
@@ -275,6 +279,8 @@ void free_chirp(Chirp& c)
 FFTResult make_chirp(int n, Chirp& c)
 {
     if (pow2(n) || n == 1) return FFT_SUCCESS;
+    if (n > kMaxBluestein)  // the global-pass form keeps its own tables
+        return zkx::fft_any_prepare(n) == ZKX_SUCCESS ? FFT_SUCCESS : FFT_ALLOC_FAILED;
     c.n = n;
     c.M = next_pow2(2 * n - 1);
     std::vector<float2> ch(n), bf(c.M, make_float2(0.f, 0.f)), bi(c.M, make_float2(0.f, 0.f));
@@ -311,7 +317,19 @@ FFTResult make_chirp(int n, Chirp& c)
     return FFT_SUCCESS;
 }
 
-bool length_ok(int n) { return n >= 1 && (pow2(n) ? n <= kMaxPow2 : n <= kMaxBluestein); }
+bool length_ok(int n) { return zkx::fft_any_length_ok(n); }
+
+// Workspace of the global-pass transforms a plan may run, allocated once.
+FFTResult alloc_any_ws(Plan* p, size_t bytes)
+{
+    if (!bytes) return FFT_SUCCESS;
+    if (hipMalloc(&p->any_ws, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        p->any_ws = nullptr;
+        return FFT_ALLOC_FAILED;
+    }
+    return FFT_SUCCESS;
+}
 
 FFTResult to_result(int rc)
 {
@@ -332,12 +350,12 @@ FFTResult run_plan(FFTPlanHandle handle, void* out, const void* in, hipStream_t 
         float2* w = p->work;  // [total] scratch for the transposed array
         const float2* src = static_cast<const float2*>(in);
         float2* dst = static_cast<float2*>(out);
-        rc = rows_fft(src, ny, nx, p->cx, inv, inv ? 1.0f / nx : 1.0f, dst, st);
+        rc = rows_fft(src, ny, nx, p->cx, inv, inv ? 1.0f / nx : 1.0f, dst, p->any_ws, st);
         if (!rc) {
             hipLaunchKernelGGL(transpose_kernel, dim3((nx + 31) / 32, (ny + 31) / 32), dim3(256), 0, st, dst, ny, nx, w);
             rc = hipGetLastError() == hipSuccess ? ZKX_SUCCESS : ZKX_RUNTIME_ERROR;
         }
-        if (!rc) rc = rows_fft(w, nx, ny, p->cy, inv, inv ? 1.0f / ny : 1.0f, w, st);
+        if (!rc) rc = rows_fft(w, nx, ny, p->cy, inv, inv ? 1.0f / ny : 1.0f, w, p->any_ws, st);
         if (!rc) {
             hipLaunchKernelGGL(transpose_kernel, dim3((ny + 31) / 32, (nx + 31) / 32), dim3(256), 0, st, w, nx, ny, dst);
             rc = hipGetLastError() == hipSuccess ? ZKX_SUCCESS : ZKX_RUNTIME_ERROR;
@@ -350,13 +368,13 @@ FFTResult run_plan(FFTPlanHandle handle, void* out, const void* in, hipStream_t 
     switch (p->type) {
     case FFT_TYPE_C2C:
         rc = rows_fft(static_cast<const float2*>(in), rows, nx, p->cx, inv, inv ? 1.0f / nx : 1.0f,
-                      static_cast<float2*>(out), st);
+                      static_cast<float2*>(out), p->any_ws, st);
         break;
     case FFT_TYPE_R2C: {
         const long long total = rows * nx;
         hipLaunchKernelGGL(real_to_complex_kernel, dim3(blocks_for(total)), dim3(256), 0, st,
                            static_cast<const float*>(in), p->work, total);
-        rc = rows_fft(p->work, rows, nx, p->cx, false, 1.0f, p->work, st);
+        rc = rows_fft(p->work, rows, nx, p->cx, false, 1.0f, p->work, p->any_ws, st);
         if (!rc)
             hipLaunchKernelGGL(half_rows_kernel, dim3(blocks_for(rows * half)), dim3(256), 0, st, p->work, nx, half,
                                rows, static_cast<float2*>(out));
@@ -369,7 +387,7 @@ FFTResult run_plan(FFTPlanHandle handle, void* out, const void* in, hipStream_t 
         // unnormalised inverse, then 1/n (a non-power-of-two inverse carries
         // its own 1/n as well, as in the reference)
         const float inner = pow2(nx) || nx == 1 ? 1.0f : 1.0f / nx;
-        rc = rows_fft(p->work, rows, nx, p->cx, true, inner, p->work, st);
+        rc = rows_fft(p->work, rows, nx, p->cx, true, inner, p->work, p->any_ws, st);
         if (!rc)
             hipLaunchKernelGGL(real_part_kernel, dim3(blocks_for(total)), dim3(256), 0, st, p->work, total,
                                1.0f / nx, static_cast<float*>(out));
@@ -400,6 +418,7 @@ void destroy(Plan* p)
     free_chirp(p->cx);
     free_chirp(p->cy);
     if (p->work) (void)hipFree(p->work);
+    if (p->any_ws) (void)hipFree(p->any_ws);
     delete p;
 }
 
@@ -426,10 +445,11 @@ FFTResult fft_plan_create_1d(FFTPlanHandle* plan, int nx, int batch, FFTType typ
         if (hipMalloc(reinterpret_cast<void**>(&p->work), sizeof(float2) * p->work_elems) != hipSuccess)
             r = FFT_ALLOC_FAILED;
     }
-    // build the twiddle tables now so execution never allocates
+    // build the twiddle tables and scratch now so execution never allocates
     const float2* tw = nullptr;
-    if (r == FFT_SUCCESS && pow2(nx) && nx > 1 && zkx::twiddles(nx, &tw) != ZKX_SUCCESS) r = FFT_ALLOC_FAILED;
+    if (r == FFT_SUCCESS && pow2(nx) && nx > 1 && zkx::fft_any_prepare(nx) != ZKX_SUCCESS) r = FFT_ALLOC_FAILED;
     if (r == FFT_SUCCESS && p->cx.M && zkx::twiddles(p->cx.M, &tw) != ZKX_SUCCESS) r = FFT_ALLOC_FAILED;
+    if (r == FFT_SUCCESS) r = alloc_any_ws(p, zkx::fft_any_workspace_bytes(batch, nx));
     if (r != FFT_SUCCESS) {
         destroy(p);
         return r;
@@ -456,9 +476,12 @@ FFTResult fft_plan_create_2d(FFTPlanHandle* plan, int nx, int ny, FFTType type, 
     p->work_elems = static_cast<size_t>(nx) * ny;
     if (r == FFT_SUCCESS && hipMalloc(reinterpret_cast<void**>(&p->work), sizeof(float2) * p->work_elems) != hipSuccess)
         r = FFT_ALLOC_FAILED;
-    const float2* tw = nullptr;
     for (int n : {nx, ny, p->cx.M, p->cy.M})
-        if (r == FFT_SUCCESS && n > 1 && pow2(n) && zkx::twiddles(n, &tw) != ZKX_SUCCESS) r = FFT_ALLOC_FAILED;
+        if (r == FFT_SUCCESS && n > 1 && pow2(n) && zkx::fft_any_prepare(n) != ZKX_SUCCESS) r = FFT_ALLOC_FAILED;
+    if (r == FFT_SUCCESS) {
+        const size_t a = zkx::fft_any_workspace_bytes(ny, nx), b = zkx::fft_any_workspace_bytes(nx, ny);
+        r = alloc_any_ws(p, a > b ? a : b);
+    }
     if (r != FFT_SUCCESS) {
         destroy(p);
         return r;

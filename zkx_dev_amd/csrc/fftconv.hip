@@ -132,7 +132,8 @@ int zkx_fftconvolve_c64(const zkx_complex64* in1, int batch, int n1, const zkx_c
     while (N < full && N < 8192) N *= 2;
     while (N < 2LL * n2 && N < 8192) N *= 2;
     void* ws = nullptr;
-    int rc = t_conv_ws.reserve(sizeof(float2) * static_cast<size_t>(N), &ws);
+    WorkspaceLease lease_(t_conv_ws, st);
+    int rc = lease_.get(sizeof(float2) * static_cast<size_t>(N), &ws);
     if (rc) return rc;
     float2* S = static_cast<float2*>(ws);
     hipLaunchKernelGGL(pad_kernel, dim3((N + 255) / 256), dim3(256), 0, st, reinterpret_cast<const float2*>(in2), n2,

@@ -132,7 +132,8 @@ int launch_fir_fft(const float* h, int K, const V* x, int lines, long long n, in
     const int Kmax = (K + D - 1) / D;
     const int L = N - Kmax + 1;
     void* ws = nullptr;
-    int rc = t_spec_ws.reserve(sizeof(float2) * static_cast<size_t>(D) * N, &ws);
+    WorkspaceLease lease_(t_spec_ws, st);
+    int rc = lease_.get(sizeof(float2) * static_cast<size_t>(D) * N, &ws);
     if (rc) return rc;
     float2* Hs = static_cast<float2*>(ws);
     hipLaunchKernelGGL(phase_taps_kernel, dim3(std::min<long long>((static_cast<long long>(D) * N + 255) / 256, 4096)),

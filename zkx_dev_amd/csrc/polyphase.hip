@@ -307,7 +307,8 @@ int launch_tiled(const float* h, const V* x, V* y, const PolyPlan& p, long long 
     const long long blocks = (Tcount + TT - 1) / TT;
     if (blocks > std::numeric_limits<int>::max() || lines > 65535) return ZKX_SUCCESS;
     void* wsp = nullptr;
-    int rc = t_taps_ws.reserve(sizeof(float) * static_cast<size_t>(tab_count), &wsp);
+    WorkspaceLease lease_(t_taps_ws, st);
+    int rc = lease_.get(sizeof(float) * static_cast<size_t>(tab_count), &wsp);
     if (rc) return rc;
     float* tab = static_cast<float*>(wsp);
     hipLaunchKernelGGL(taps_table_kernel, dim3(static_cast<unsigned>(std::min<long long>((tab_count + 255) / 256, 1024))),
@@ -502,7 +503,8 @@ int firfilter_impl(const float* b, int taps, const float* x, int outer, int n, i
     // full convolution into scratch, then zi / zf split
     const long long full_n = static_cast<long long>(n) + state;
     void* ws = nullptr;
-    rc = t_fir_ws.reserve(sizeof(float) * static_cast<size_t>(outer) * inner * full_n, &ws);
+    WorkspaceLease lease_(t_fir_ws, st);
+    rc = lease_.get(sizeof(float) * static_cast<size_t>(outer) * inner * full_n, &ws);
     if (rc) return rc;
     float* full = static_cast<float*>(ws);
     bool done = false;

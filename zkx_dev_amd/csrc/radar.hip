@@ -12,6 +12,7 @@
 #include "fft_lds.hpp"
 #include "radar_tiles.hpp"
 #include "dc_strip.hpp"
+#include "fft_any.hpp"
 #include "runtime.hpp"
 
 #include <algorithm>
@@ -874,6 +875,99 @@ __global__ void __launch_bounds__(256, 2) doppler_cfar_strip_kernel(strip::Strip
 }
 
 // ---------------------------------------------------------------------------
+// Doppler transform of any nfft (not a power of two, or beyond the column
+// kernels): corner turn into [R][nfft] rows, any-length row FFT, corner turn
+// back.  FP32 throughout, like the power-of-two path.
+// ---------------------------------------------------------------------------
+// wtab[p] = window value of pulse p (p < used)
+__global__ void doppler_window_kernel(int used, int P, int window, const float* __restrict__ explicit_window,
+                                      float* __restrict__ wtab)
+{
+    const bool expl = window == ZKX_WINDOW_EXPLICIT_FP32 || window == ZKX_WINDOW_EXPLICIT_FP64;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < used; p += gridDim.x * blockDim.x)
+        wtab[p] = expl ? explicit_window[p] : static_cast<float>(window_value(window, p, P));
+}
+
+// rows[r][f] = w[f] * x[f][r] for f < used, 0 for used <= f < nfft (32x32 tiles)
+__global__ void __launch_bounds__(256) doppler_gather_kernel(const float2* __restrict__ x, int R, int used, int nfft,
+                                                             const float* __restrict__ wtab, float2* __restrict__ rows)
+{
+    __shared__ float2 tile[32][33];
+    const int r0 = blockIdx.x * 32, f0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        const int f = f0 + j, r = r0 + tx;
+        float2 v = make_float2(0.f, 0.f);
+        if (f < used && r < R) v = cscale(x[static_cast<long long>(f) * R + r], wtab[f]);
+        tile[j][tx] = v;
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int r = r0 + j, f = f0 + tx;
+        if (r < R && f < nfft) rows[static_cast<long long>(r) * nfft + f] = tile[tx][j];
+    }
+}
+
+// y[f][r] = rows[r][f], power[f][r] = |.|^2 (either output may be NULL)
+__global__ void __launch_bounds__(256) doppler_scatter_kernel(const float2* __restrict__ rows, int R, int nfft,
+                                                              float2* __restrict__ y, float* __restrict__ power)
+{
+    __shared__ float2 tile[32][33];
+    const int f0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        const int r = r0 + j, f = f0 + tx;
+        if (r < R && f < nfft) tile[j][tx] = rows[static_cast<long long>(r) * nfft + f];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int f = f0 + j, r = r0 + tx;
+        if (f < nfft && r < R) {
+            const float2 v = tile[tx][j];
+            const long long o = static_cast<long long>(f) * R + r;
+            if (y) y[o] = v;
+            if (power) power[o] = power_of(v);
+        }
+    }
+}
+
+thread_local ThreadWorkspace t_doppler_ws;
+
+int doppler_any(const float2* x, int P, int R, int nfft, int window, const float* explicit_window, float2* y,
+                float* power, hipStream_t st)
+{
+    if (!fft_any_length_ok(nfft)) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_doppler: nfft beyond the FFT range");
+    // the corner turns put nfft / 32 and R / 32 tiles on a grid's y axis
+    if ((R + 31) / 32 > 65535 || (nfft + 31) / 32 > 65535)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_doppler: map too large for the corner turn");
+    const int used = P < nfft ? P : nfft;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t wbytes = al(sizeof(float) * used);
+    const size_t rbytes = al(sizeof(float2) * static_cast<size_t>(R) * nfft);
+    const size_t fbytes = fft_any_workspace_bytes(R, nfft);
+    int rc = fft_any_prepare(nfft);
+    if (rc) return rc;
+    void* ws = nullptr;
+    WorkspaceLease lease_(t_doppler_ws, st);
+    if ((rc = lease_.get(wbytes + rbytes + fbytes, &ws))) return rc;
+    char* base = static_cast<char*>(ws);
+    float* wtab = reinterpret_cast<float*>(base);
+    float2* rows = reinterpret_cast<float2*>(base + wbytes);
+    hipLaunchKernelGGL(doppler_window_kernel, dim3((used + 255) / 256), dim3(256), 0, st, used, P, window,
+                       explicit_window, wtab);
+    ZKX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(doppler_gather_kernel, dim3((R + 31) / 32, (nfft + 31) / 32), dim3(256), 0, st, x, R, used,
+                       nfft, wtab, rows);
+    ZKX_LAUNCH_CHECK();
+    rc = fft_any_rows(rows, R, nfft, false, 1.0f, rows, fbytes ? base + wbytes + rbytes : nullptr, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(doppler_scatter_kernel, dim3((nfft + 31) / 32, (R + 31) / 32), dim3(256), 0, st, rows, R, nfft,
+                       y, power);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
 // dispatch helpers
 // ---------------------------------------------------------------------------
 constexpr int kMaxLdsFft = 8192;
@@ -1468,7 +1562,8 @@ int zkx_pulse_compression_c64(const zkx_complex64* x, int num_pulses, int sample
         const int N = pc_segment_length(nfft, keep, &L, &nseg);
         if (!N) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_compression: template too long for this nfft");
         void* H = nullptr;
-        int rc = ws.reserve(sizeof(float2) * static_cast<size_t>(N), &H);
+        WorkspaceLease lease_(ws, (hipStream_t)stream);
+        int rc = lease_.get(sizeof(float2) * static_cast<size_t>(N), &H);
         if (rc) return rc;
         rc = pc_template(N, reinterpret_cast<const float2*>(tmpl), template_len, normalize, window, explicit_window,
                          static_cast<float2*>(H), (hipStream_t)stream, keep);
@@ -1478,7 +1573,8 @@ int zkx_pulse_compression_c64(const zkx_complex64* x, int num_pulses, int sample
                            static_cast<const float2*>(H), reinterpret_cast<float2*>(out), (hipStream_t)stream);
     }
     void* H = nullptr;
-    int rc = ws.reserve(sizeof(float2) * static_cast<size_t>(nfft), &H);
+    WorkspaceLease lease_(ws, (hipStream_t)stream);
+    int rc = lease_.get(sizeof(float2) * static_cast<size_t>(nfft), &H);
     if (rc) return rc;
     rc = zkx_pc_template_prepare(tmpl, template_len, nfft, normalize, window, explicit_window,
                                  static_cast<zkx_complex64*>(H), stream);
@@ -1501,12 +1597,10 @@ int zkx_pulse_doppler_c64(const zkx_complex64* x, int num_pulses, int samples_pe
     if (explicit_kind(window) != (explicit_window != nullptr))
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_doppler window is invalid");
     if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft) {
-        // other lengths: the DFT as a split-BF16 matrix-core contraction
-        // (the reference uses Bluestein for these lengths)
-        if (nfft > 4096 || num_pulses > 4096)
-            ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse_doppler: non-power-of-two nfft must be <= 4096");
-        return doppler_dft(reinterpret_cast<const float2*>(x), 1, num_pulses, samples_per_pulse, nfft, window,
-                           explicit_window, 3, reinterpret_cast<float2*>(out), power, (hipStream_t)stream);
+        // other lengths: FP32 row transforms of any length (Bluestein beyond
+        // the powers of two, as the reference's backend does)
+        return doppler_any(reinterpret_cast<const float2*>(x), num_pulses, samples_per_pulse, nfft, window,
+                           explicit_window, reinterpret_cast<float2*>(out), power, (hipStream_t)stream);
     }
     return doppler(nfft, reinterpret_cast<const float2*>(x), 1, num_pulses, samples_per_pulse, window,
                    explicit_window, reinterpret_cast<float2*>(out), power, nullptr, (hipStream_t)stream);
@@ -1545,9 +1639,10 @@ int zkx_ca_cfar_f32(const float* x, int rank, const int* shape, const int* guard
     const long long count = rank == 2 ? static_cast<long long>(shape[0]) * shape[1] : shape[0];
     if (count > std::numeric_limits<int>::max()) ZKX_FAIL(ZKX_INVALID_SHAPE, "ca_cfar input size or capacity is invalid");
     float* I = workspace;
+    WorkspaceLease lease_(t_cfar_ws, st);  // held until the kernels below are enqueued
     if (!I) {
         void* p = nullptr;
-        int rc = t_cfar_ws.reserve(sizeof(float) * static_cast<size_t>(count), &p);
+        int rc = lease_.get(sizeof(float) * static_cast<size_t>(count), &p);
         if (rc) return rc;
         I = static_cast<float*>(p);
     }

@@ -51,24 +51,58 @@ int twiddles(int n, const float2** out)
     return ZKX_SUCCESS;
 }
 
-int ThreadWorkspace::reserve(size_t bytes, void** out)
+namespace {
+bool capturing(hipStream_t st)
+{
+    hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &status) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return status != hipStreamCaptureStatusNone;
+}
+}  // namespace
+
+int ThreadWorkspace::acquire(size_t bytes, hipStream_t st, void** out)
 {
     int dev = 0;
     ZKX_HIP_CHECK(hipGetDevice(&dev));
-    if (dev != device_ || bytes > bytes_) {
-        if (ptr_ != nullptr && dev == device_) {
+    if (dev != device_) {
+        // another device: its event and buffer belong to the old one
+        pending_ = false;
+        if (ev_) (void)hipEventDestroy(ev_);
+        ev_ = nullptr;
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    if (bytes > bytes_) {
+        if (ptr_ != nullptr) {
             // the previous buffer may still be read by queued work on this device
             ZKX_HIP_CHECK(hipDeviceSynchronize());
             ZKX_HIP_CHECK(hipFree(ptr_));
+            pending_ = false;
         }
         ptr_ = nullptr;
         bytes_ = 0;
         ZKX_HIP_CHECK(hipMalloc(&ptr_, bytes));
         bytes_ = bytes;
-        device_ = dev;
     }
+    device_ = dev;
+    if (pending_ && !capturing(st)) ZKX_HIP_CHECK(hipStreamWaitEvent(st, ev_, 0));
     *out = ptr_;
     return ZKX_SUCCESS;
+}
+
+void ThreadWorkspace::release(hipStream_t st)
+{
+    if (capturing(st)) return;
+    if (!ev_ && hipEventCreateWithFlags(&ev_, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        ev_ = nullptr;
+        return;
+    }
+    pending_ = hipEventRecord(ev_, st) == hipSuccess;
+    if (!pending_) (void)hipGetLastError();
 }
 
 namespace {

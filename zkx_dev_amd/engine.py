@@ -96,8 +96,11 @@ class DwellEngine:
         for t, dt in ((x, torch.complex64), (threshold, torch.float32), (detections, torch.uint8)):
             if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dt or not t.is_contiguous():
                 raise ValueError("engine buffers must be contiguous device tensors of the documented dtypes")
-        if rd_map is not None and (rd_map.dtype != torch.complex64 or not rd_map.is_cuda):
-            raise ValueError("rd_map must be a complex64 device tensor")
+        if rd_map is not None and (rd_map.dtype != torch.complex64 or not rd_map.is_cuda or
+                                   not rd_map.is_contiguous() or rd_map.device != x.device):
+            raise ValueError("rd_map must be a contiguous complex64 tensor on the input's device")
+        if threshold.device != x.device or detections.device != x.device:
+            raise ValueError("engine outputs must be on the input's device")
         if len(d):
             need_in = int(np.max(d["input_offset"] + d["num_pulses"].astype(np.int64) * d["range_bins"]))
             need_out = int(np.max(d["output_offset"] + d["num_pulses"].astype(np.int64) * d["range_bins"]))

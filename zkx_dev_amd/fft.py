@@ -10,6 +10,8 @@ import ctypes
 
 import torch
 
+from ._check import out_tensor
+
 from ._lib import lib
 
 FFT_FORWARD, FFT_INVERSE = 1, -1
@@ -90,6 +92,7 @@ class Plan:
             raise ValueError(f"input must be {din} with {nin} elements on the device")
         x = x.contiguous()
         out = torch.empty(nout, dtype=dout, device=x.device) if out is None else out
+        out_tensor(out, "out", x, dout, nout)
         s = stream if stream is not None else torch.cuda.current_stream()
         _check(_L().fft_execute_on_stream(self.handle, out.data_ptr(), x.data_ptr(), s.cuda_stream))
         return out

@@ -14,6 +14,7 @@ import ctypes
 
 import torch
 
+from ._check import out_tensor
 from ._lib import ZkxError, check, lib
 
 WINDOWS = {"none": 0, "hann": 1, "hamming": 2, "explicit_fp32": 3, "explicit_fp64": 4}
@@ -104,6 +105,7 @@ def pulse_compression_prepared(x, spectrum, *, out=None, stream=None):
     shape = tuple(x.shape[:-1]) + (n,)
     if out is None:
         out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    out_tensor(out, "out", x, torch.complex64, rows * n)
     check(lib().zkx_pulse_compression_prepared_c64(_ptr(x), rows, S, _ptr(spectrum), n, _ptr(out),
                                                    _stream(stream)))
     return out
@@ -131,6 +133,8 @@ def pulse_doppler(x, *, nfft=-1, window="none", explicit_window=None, out=None, 
         out = torch.empty((n, R), dtype=torch.complex64, device=x.device)
     elif out.numel() != n * R or out.dtype != torch.complex64:
         raise ValueError("pulse_doppler preallocated output shape mismatch")
+    out_tensor(out, "out", x, torch.complex64, n * R)
+    out_tensor(power, "power", x, torch.float32, n * R)
     check(lib().zkx_pulse_doppler_c64(_ptr(x), P, R, n, code, _ptr(ew), _ptr(out), _ptr(power),
                                       _stream(stream)))
     if not preallocated and widen and _result_is_fp64(window):
@@ -162,10 +166,8 @@ def doppler_dft_mfma(x, *, window="none", terms=3, out=None, power=None, stream=
     if window not in codes:
         raise ValueError("doppler_dft_mfma window must be none, hann or hamming")
     y = out if out is not None else torch.empty_like(cube)
-    if y.shape != cube.shape or y.dtype != torch.complex64 or not y.is_contiguous():
-        raise ValueError("out must be a contiguous complex64 tensor of the input's shape")
-    if power is not None and (power.numel() != cube.numel() or power.dtype != torch.float32):
-        raise ValueError("power must be float32 with the input's element count")
+    out_tensor(y, "out", cube, torch.complex64, cube.numel())
+    out_tensor(power, "power", cube, torch.float32, cube.numel())
     check(lib().zkx_doppler_dft_mfma_c64(_ptr(cube), C, P, R, codes[window], int(terms), _ptr(y), _ptr(power),
                                          _stream(stream)))
     return y if x.dim() == 3 else y[0]
@@ -261,6 +263,8 @@ def ca_cfar(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=None, detect
         detections = torch.empty(x.shape, dtype=torch.uint8, device=x.device)
     if threshold.numel() != x.numel() or detections.numel() != x.numel():
         raise ValueError("ca_cfar preallocated output shape must match input")
+    out_tensor(threshold, "threshold", x, torch.float32, x.numel())
+    out_tensor(detections, "detections", x, torch.uint8, x.numel())
     shape = (ctypes.c_int * rank)(*x.shape)
     g = (ctypes.c_int * rank)(*guard)
     r = (ctypes.c_int * rank)(*ref)
@@ -282,6 +286,8 @@ def ca_cfar_windowed(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=Non
         threshold = torch.empty_like(x)
     if detections is None:
         detections = torch.empty(x.shape, dtype=torch.uint8, device=x.device)
+    out_tensor(threshold, "threshold", x, torch.float32, x.numel())
+    out_tensor(detections, "detections", x, torch.uint8, x.numel())
     code = lib().zkx_ca_cfar_window_f32(_ptr(x), x.shape[0], x.shape[1], guard_cells[0],
                                         guard_cells[1], reference_cells[0], reference_cells[1],
                                         float(pfa), _ptr(threshold), _ptr(detections),
@@ -304,6 +310,10 @@ def doppler_cfar(x, guard_cells=(2, 2), reference_cells=(8, 8), pfa=1e-6, *, win
         threshold = torch.empty(x.shape, dtype=torch.float32, device=x.device)
     if detections is None:
         detections = torch.empty(x.shape, dtype=torch.uint8, device=x.device)
+    out_tensor(threshold, "threshold", x, torch.float32, x.numel())
+    out_tensor(detections, "detections", x, torch.uint8, x.numel())
+    out_tensor(rd_map, "rd_map", x, torch.complex64, x.numel())
+    out_tensor(power, "power", x, torch.float32, x.numel())
     code = lib().zkx_doppler_cfar_c64(_ptr(x), B, P, R, WINDOWS[window], guard_cells[0], guard_cells[1],
                                       reference_cells[0], reference_cells[1], float(pfa), _ptr(threshold),
                                       _ptr(detections), _ptr(rd_map), _ptr(power), _stream(stream))
@@ -360,6 +370,9 @@ class DwellPipeline:
             threshold = torch.empty((B, P, R), dtype=torch.float32, device=x.device)
         if detections is None:
             detections = torch.empty((B, P, R), dtype=torch.uint8, device=x.device)
+        out_tensor(threshold, "threshold", x, torch.float32, x.numel())
+        out_tensor(detections, "detections", x, torch.uint8, x.numel())
+        out_tensor(rd_map, "rd_map", x, torch.complex64, x.numel())
         check(lib().zkx_dwell_batch_c64(
             _ptr(x), B, P, R, _ptr(self.spectrum), self.doppler_window, self.guard[0],
             self.guard[1], self.reference[0], self.reference[1], self.pfa, self.cfar_mode,
