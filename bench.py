@@ -41,12 +41,15 @@ def parse():
     ap.add_argument("--cfar-mode", type=int, default=1)
     ap.add_argument("--plain-dwells", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dwell", "filters", "batched", "serve", "split", "task1"],
+    ap.add_argument("--workload", choices=["dwell", "filters", "batched", "serve", "split", "task1", "probe"],
                     default="dwell")
     ap.add_argument("--dwells", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--lanes", type=int, default=2,
                     help="streams that take consecutive batches in turn (own workspace and outputs each)")
+    ap.add_argument("--no-others", action="store_true", help="skip the short filter / split-BF16 measurements")
+    ap.add_argument("--parse-counters", nargs="+", metavar="CSV_OR_OUT",
+                    help="counter CSVs of rocprofv3 runs of --workload probe, then the output JSON")
     return ap.parse_args()
 
 
@@ -249,25 +252,14 @@ def bench_serve(args):
     stats = srv.stats()
     srv.close()
 
-    # plain per-dwell drop-in on the same shapes (three operator calls each)
-    stream = torch.cuda.current_stream()
+    # plain per-dwell drop-in on the same shapes: the same baseline function
+    # as the dwell bench (three operator calls per dwell)
     plain = {}
     for si, (p, r) in enumerate(shapes):
         i0 = si * per_shape
-        x = ub.x[ub.offsets[i0]:ub.offsets[i0] + p * r].view(p, r)
+        xs = [ub.x[ub.offsets[i0 + k]:ub.offsets[i0 + k] + p * r].view(p, r) for k in range(per_shape)]
         tm = torch.from_numpy(synth.lfm_chirp(taps[0])).to(dev)
-
-        def once():
-            y = radar.pulse_compression(x, tm, normalize=True, window="hamming", widen=False)
-            pw = radar.doppler_power(y, window="hann")
-            radar.ca_cfar(pw, GUARD, REF, PFA)
-        once()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(10):
-            once()
-        torch.cuda.synchronize()
-        plain[f"{p}x{r}"] = 10 / (time.perf_counter() - t0)
+        plain[f"{p}x{r}"], _ = plain_dropin(radar, xs, tm, 40)
     plain_mixed = 1.0 / np.mean([1.0 / v for v in plain.values()])
     print(json.dumps({
         "metric": "range-Doppler dwells/s + Msamples/s (HBM roofline %) and p99 latency",
@@ -431,12 +423,272 @@ def bench_task1(args):
                       "results": res}))
 
 
+def cpu_info():
+    """CPU model and the cores this process may use (for cpu_baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail, os.cpu_count() or avail
+
+
+def _cpu_chain_worker(seeds):
+    """One process of the all-core CPU baseline: the oracle chain on the given
+    seeded dwells; returns (dwells, seconds) of pure CPU work."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as orc
+    from zkx_dev_amd import synth
+    t = 0.0
+    for sd in seeds:
+        d = synth.make_dwell(P, R, TAPS, seed=sd)
+        c0 = time.perf_counter()
+        pc = orc.pulse_compression(d.x, d.template, window="hamming", normalize=True)
+        rd = orc.pulse_doppler(pc.astype(np.complex64), window="hann")
+        orc.ca_cfar((np.abs(rd) ** 2).astype(np.float32), GUARD, REF, PFA)
+        t += time.perf_counter() - c0
+    return len(seeds), t
+
+
+def cpu_baseline(max_workers=16):
+    """The reference's CPU chain (the oracle's restatement of its *_typed_cpu
+    functions) on bounded samples of the dwell workload: one core for about
+    10 s, then every available core (up to the box's CPU share) on 2 dwells
+    each."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    model, avail, total = cpu_info()
+    done, secs = 0, 0.0
+    while done < 3 or (secs < 10.0 and done < 30):
+        n, t = _cpu_chain_worker([100 + done])
+        done += n
+        secs += t
+    workers = max(1, min(avail, max_workers))
+    seeds = [[1000 + 2 * w, 1001 + 2 * w] for w in range(workers)]
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        res = list(ex.map(_cpu_chain_worker, seeds))
+    wall = time.perf_counter() - t0
+    all_dwells = sum(r[0] for r in res)
+    return {"value": done / secs, "unit": "dwells/s", "cores": 1, "kind": "port",
+            "sample": f"{done} seeded dwells 128x4096 on one core (about 10 s): direct FP64 circular "
+                      "correlation (256 taps), direct DFT Doppler, FP32 integral-image CFAR (the "
+                      "reference *_typed_cpu chain restated in C; its window is computed once per "
+                      "call where the reference evaluates cosf per term, so the port is slightly "
+                      "faster than the reference's own code), input generation excluded",
+            "seconds": secs, "cpu_model": model, "cpus_available": avail, "cpus_total": total,
+            "all_cores": {"value": all_dwells / wall, "unit": "dwells/s", "cores": workers,
+                          "sample": f"{all_dwells} dwells, {workers} processes x 2 dwells (wall clock, "
+                                    "process start-up included)",
+                          "seconds": wall}}
+
+
+def plain_dropin(radar, x_dwells, template, n, warm=3):
+    """The plain per-dwell drop-in: the reference Task1 chain's three operator
+    calls per dwell (pulse_compression, Doppler |.|^2, ca_cfar), each call
+    allocating its outputs and workspaces as the reference's do, waiting for
+    each dwell.  Returns (dwells/s, per-dwell latencies in ms)."""
+    import torch
+
+    def once(xd):
+        y = radar.pulse_compression(xd, template, normalize=True, window="hamming", widen=False)
+        pwr = radar.doppler_power(y, window="hann")
+        radar.ca_cfar(pwr, GUARD, REF, PFA)
+    for i in range(warm):
+        once(x_dwells[i % len(x_dwells)])
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(n):
+        a = time.perf_counter()
+        once(x_dwells[i % len(x_dwells)])
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    return n / (time.perf_counter() - t0), lat
+
+
+def other_configs(reps):
+    """Short device-time measurements of the other configurations, for the
+    default line: the three polyphase calls and the split-BF16 Doppler DFT."""
+    import torch
+    from scipy import signal
+    from zkx_dev_amd import filtering, radar, synth
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    out = {}
+    x = torch.from_numpy(synth.complex_noise_tones(16, 1 << 20)).to(dev)
+    xr = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, 1 << 24).astype(np.float32)).to(dev)
+    h_k = torch.from_numpy(signal.firwin(121, 1.0 / 3.0, window=("kaiser", 5.0)).astype(np.float32)).to(dev)
+    h_lp = torch.from_numpy(signal.firwin(255, 0.25).astype(np.float32)).to(dev)
+    cases = {
+        "resample_poly_3_2_121_16x2^20_c64": (lambda: filtering.resample_poly(x, 3, 2, axis=1, coefficients=h_k),
+                                              16 * (1 << 20) * 8 + 16 * 1572864 * 8 + 121 * 4),
+        "upfirdn_1_4_255_16x2^20_c64": (lambda: filtering.upfirdn(h_lp, x, 1, 4, axis=1),
+                                        16 * (1 << 20) * 8 + 16 * 262208 * 8 + 255 * 4),
+        "firfilter_255_2^24_f32": (lambda: filtering.firfilter(h_lp, xr), (1 << 24) * 8 + 255 * 4),
+    }
+    for name, (fn, nbytes) in cases.items():
+        t = timed(fn)
+        out[name] = {"us": t * 1e6, "alg_bytes": nbytes, "hbm_frac": nbytes / t / HBM_PEAK}
+    del x, xr
+    # split-BF16 Doppler: 16 ch x 512 pulses x 8192 range, Hann, 3 bf16 products
+    C, Pp, Rr = 16, 512, 8192
+    g = torch.Generator(device=dev)
+    g.manual_seed(20261209)
+    cube = torch.complex(torch.randn((C, Pp, Rr), generator=g, device=dev),
+                         torch.randn((C, Pp, Rr), generator=g, device=dev))
+    y = torch.empty_like(cube)
+    t3 = timed(lambda: radar.doppler_dft_mfma(cube, window="hann", terms=3, out=y))
+    flops = 8.0 * Pp * Pp * Rr * C
+    out["split_bf16_doppler_16x512x8192"] = {"us": t3 * 1e6, "algorithmic_tflops": flops / t3 / 1e12,
+                                             "mfma_frac": flops / t3 / 2.5e15,
+                                             "hbm_frac": 2 * C * Pp * Rr * 8 / t3 / HBM_PEAK}
+    return out
+
+
+def counters_db():
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh).get("kernels", {})
+
+
+def probe(args):
+    """The counter workload of profiles/traffic.json (run under rocprofv3
+    --pmc, one counter set per run): a 256 MiB device copy (calibration),
+    then 3 launches of each measured kernel: the two dwell kernels at batch
+    64, the reference-identical CFAR of one batch, and the polyphase calls."""
+    import torch
+    from scipy import signal
+    from zkx_dev_amd import filtering, radar, synth
+
+    dev = torch.device("cuda:0")
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).uniform_()
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    del a, b
+    B = 64
+    cubes, tmpl, _ = synth.make_dwell_batch(2, P, R, TAPS, seed=5)
+    x = torch.from_numpy(np.concatenate([cubes] * (B // 2))).to(dev)
+    pipe = radar.DwellPipeline(torch.from_numpy(tmpl).to(dev), P, R, max_batch=B)
+    pipe0 = radar.DwellPipeline(torch.from_numpy(tmpl).to(dev), P, R, cfar_mode=0, max_batch=B)
+    pc = torch.empty_like(x)
+    for _ in range(3):
+        radar.pulse_compression_prepared(x, pipe.spectrum, out=pc)
+        radar.doppler_cfar(pc, window="hann")
+        pipe0.run(x)
+    xs = torch.from_numpy(synth.complex_noise_tones(16, 1 << 20)).to(dev)
+    xr = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, 1 << 24).astype(np.float32)).to(dev)
+    h_k = torch.from_numpy(signal.firwin(121, 1.0 / 3.0, window=("kaiser", 5.0)).astype(np.float32)).to(dev)
+    h_lp = torch.from_numpy(signal.firwin(255, 0.25).astype(np.float32)).to(dev)
+    h_1025 = torch.from_numpy(signal.firwin(1025, 0.1).astype(np.complex64)).to(dev)
+    for _ in range(3):
+        filtering.resample_poly(xs, 3, 2, axis=1, coefficients=h_k)
+        filtering.upfirdn(h_lp, xs, 1, 4, axis=1)
+        filtering.firfilter(h_lp, xr)
+        filtering.fftconvolve(xs, h_1025, "same")
+    torch.cuda.synchronize()
+    print("probe done", flush=True)
+
+
+COPY_BYTES = 256 * 1024 * 1024
+# kernels of profiles/traffic.json: name -> (substring of the traced name, algorithmic bytes per launch)
+PROBED = {
+    "pc_fused_kernel<4096>": ("pc_fused_kernel<4096>", 64 * P * R * 16 + R * 8),
+    "doppler_cfar_strip_kernel<128, 2, 2, 8, 8>": ("doppler_cfar_strip_kernel<128", 64 * P * R * 13),
+    "doppler_kernel<128, 0>": ("doppler_kernel<128, 0>", 64 * P * R * (8 + 4 + 4)),
+    "cfar_rowscan_tile_kernel": ("cfar_rowscan_tile_kernel", 64 * P * R * 8),
+    "cfar_threshold_rows_kernel": ("cfar_threshold_rows_kernel", 64 * P * R * 13),
+    "polyphase_kernel (resample_poly 3/2, 121 taps)": ("polyphase_kernel", 16 * (1 << 20) * 8 + 16 * 1572864 * 8),
+    "fir_fft_kernel (upfirdn 1/4 c64 + firfilter f32, mean)": ("fir_fft_kernel", None),
+    "ols_kernel (fftconvolve same, 1025 taps)": ("ols_kernel", 16 * (1 << 20) * 16),
+}
+
+
+def parse_counters(paths, out_json):
+    """Per kernel of PROBED: mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch,
+    scaled by what the same counters read for the calibration copy) and mean
+    SQ_INSTS_VALU per dispatch, from rocprofv3 counter_collection CSV files."""
+    import csv
+    from collections import defaultdict
+    vals = defaultdict(lambda: defaultdict(list))  # counter -> kernel name -> per-dispatch values
+    for path in paths:
+        per = defaultdict(float)
+        names = {}
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                key = (r["Counter_Name"], r["Dispatch_Id"])
+                per[key] += float(r["Counter_Value"])
+                names[key] = r["Kernel_Name"]
+        for (cnt, did), v in per.items():
+            vals[cnt][names[(cnt, did)]].append(v)
+
+    def mean_for(cnt, needle):
+        got = [v for name, vs in vals[cnt].items() if needle in name for v in vs]
+        return sum(got) / len(got) if got else None
+    # calibration: the 256 MiB copy is the largest copyBuffer dispatch (the
+    # host-to-device input copies are smaller)
+    def max_for(cnt, needle):
+        got = [v for name, vs in vals[cnt].items() if needle in name for v in vs]
+        return max(got) if got else None
+    cf, cw = max_for("FETCH_SIZE", "copyBuffer"), max_for("WRITE_SIZE", "copyBuffer")
+    kf = COPY_BYTES / (cf * 1024.0) if cf else None
+    kw = COPY_BYTES / (cw * 1024.0) if cw else None
+    out = {"calibration": {"copy_bytes": COPY_BYTES, "copy_fetch_kib": cf, "copy_write_kib": cw,
+                           "fetch_factor": kf, "write_factor": kw},
+           "kernels": {}}
+    for key, (needle, alg) in PROBED.items():
+        rec = {"batch": 64 if "dwell" in key or "cfar" in key or "pc_" in key else None}
+        fk, wk, va = mean_for("FETCH_SIZE", needle), mean_for("WRITE_SIZE", needle), mean_for("SQ_INSTS_VALU", needle)
+        if fk is not None and wk is not None and kf and kw:
+            rec.update({"fetch_kib": fk, "write_kib": wk, "hbm_bytes_per_launch": fk * 1024.0 * kf + wk * 1024.0 * kw})
+            if alg:
+                rec["alg_bytes_per_launch"] = alg
+                rec["traffic_over_alg"] = rec["hbm_bytes_per_launch"] / alg
+        if va is not None:
+            rec["valu_insts_per_launch"] = va
+        if len(rec) > 1:
+            out["kernels"][key] = rec
+    with open(out_json, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     args = parse()
+    if args.parse_counters:
+        parse_counters(args.parse_counters[:-1], args.parse_counters[-1])
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "probe":
+        probe(args)
+        return
     if args.workload == "filters":
         bench_filters(args)
         return
@@ -463,25 +715,78 @@ def main():
     x = torch.from_numpy(cubes).to(dev)
     t = torch.from_numpy(tmpl).to(dev)
     del cubes
-    pipe = radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
-                               guard=GUARD, reference=REF, pfa=PFA, cfar_mode=args.cfar_mode,
-                               max_batch=B)
-    thr = torch.empty((B, P, R), dtype=torch.float32, device=dev)
-    det = torch.empty((B, P, R), dtype=torch.uint8, device=dev)
+    lanes = max(1, args.lanes)
+    pipes = [radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
+                                 guard=GUARD, reference=REF, pfa=PFA, cfar_mode=args.cfar_mode, max_batch=B)
+             for _ in range(lanes)]
+    outs = [(torch.empty((B, P, R), dtype=torch.float32, device=dev),
+             torch.empty((B, P, R), dtype=torch.uint8, device=dev)) for _ in range(lanes)]
+    thr, det = outs[0]
     stream = torch.cuda.current_stream()
+    cells = B * P * R
 
-    # ---- batched fused pipeline ------------------------------------------
+    # ---- per-kernel timing of the stages (events on the launch stream); this
+    # also brings the clocks up before the timed steps ------------------------
+    pc_out = torch.empty((B, P, R), dtype=torch.complex64, device=dev)
+    reps = max(20, args.steps // 2)
+
+    def timed(fn, n=reps):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(n):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / n * 1e-3  # seconds per launch
+
+    for _ in range(3):
+        timed(lambda: pipes[0].run(x, threshold=thr, detections=det))
+    t_pc = timed(lambda: radar.pulse_compression_prepared(x, pipes[0].spectrum, out=pc_out))
+    stages = {"pc_fused_kernel<4096>": (t_pc, cells * 16 + R * 8)}
+    if args.cfar_mode == 1:
+        t_dc = timed(lambda: radar.doppler_cfar(pc_out, GUARD, REF, PFA, window="hann", threshold=thr,
+                                                detections=det))
+        stages["doppler_cfar_strip_kernel<128, 2, 2, 8, 8>"] = (t_dc, cells * (8 + 4 + 1))
+    pipe0 = radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
+                                guard=GUARD, reference=REF, pfa=PFA, cfar_mode=0, max_batch=B)
+    t_mode0 = timed(lambda: pipe0.run(x, threshold=thr, detections=det))
+
+    # ---- the headline CFAR against the reference's own CFAR ------------------
+    # (the FP32 integral image, bit-identical to the reference restatement) on
+    # the same power maps: masks must agree except cells whose statistic lies
+    # within 1e-6 relative of the reference threshold, which are counted
+    parity = None
+    if args.cfar_mode == 1:
+        pwm = torch.empty((B, P, R), dtype=torch.float32, device=dev)
+        radar.pulse_compression_prepared(x, pipes[0].spectrum, out=pc_out)
+        t1, d1 = radar.doppler_cfar(pc_out, GUARD, REF, PFA, window="hann", power=pwm)
+        pipes[0].run(x, threshold=thr, detections=det)
+        same = bool(torch.equal(d1, det)) and bool(torch.equal(t1.view(torch.int32), thr.view(torch.int32)))
+        differ = excused = near = 0
+        drift = 0.0
+        for b in range(B):
+            t0, d0 = radar.ca_cfar(pwm[b], GUARD, REF, PFA)
+            dm = d0 != d1[b]
+            nr = (pwm[b] - t0).abs() <= 1e-6 * t0.abs()
+            differ += int(dm.sum())
+            excused += int((dm & nr).sum())
+            near += int(nr.sum())
+            live = t0 != 0
+            drift = max(drift, float(((t1[b] - t0).abs() / t0.abs().clamp_min(1e-30))[live].max()))
+        parity = {"cells": cells, "differ": differ, "differ_within_1e-6": excused,
+                  "differ_outside_1e-6": differ - excused, "near_threshold_cells": near,
+                  "max_threshold_rel_gap": drift, "pipeline_equals_fused_kernel": same,
+                  "reference": "FP32 integral-image CA-CFAR in the reference's order (ca_cfar, bit-identical "
+                               "to the reference restatement) on the same power maps"}
+        del pwm, t1, d1
+
+    # ---- batched fused pipeline: the timed steps -----------------------------
     # Consecutive batches go to `lanes` streams in turn, each lane with its
     # own pipeline workspace and output buffers, so one batch's Doppler+CFAR
     # overlaps the next batch's pulse compression (every batch still runs
-    # the whole chain).  One lane is also timed, for reference.
-    lanes = max(1, args.lanes)
-    pipes, outs = [pipe], [(thr, det)]
-    for _ in range(lanes - 1):
-        pipes.append(radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
-                                         guard=GUARD, reference=REF, pfa=PFA, cfar_mode=args.cfar_mode,
-                                         max_batch=B))
-        outs.append((torch.empty_like(thr), torch.empty_like(det)))
+    # the whole chain).
     lane_streams = [torch.cuda.Stream() for _ in range(lanes)] if lanes > 1 else [stream]
 
     def run_steps(nlanes, steps, ev=None):
@@ -510,99 +815,40 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0, [a.elapsed_time(b) for a, b in ev]
 
-    wall1, _ = timed_steps(1)
-    one_lane_dps = B * args.steps / wall1
     wall, step_ms = timed_steps(lanes)
     dwells_per_s = B * args.steps / wall
+    wall1, _ = timed_steps(1)
+    one_lane_dps = B * args.steps / wall1
     print(f"# batched: {dwells_per_s:.1f} dwells/s over {lanes} lanes ({one_lane_dps:.1f} on one), "
           f"step p50 {percentile(step_ms, 50):.3f} ms", file=sys.stderr, flush=True)
 
-    # ---- per-kernel timing of the stages (events on the launch stream) -------
-    pc_out = torch.empty((B, P, R), dtype=torch.complex64, device=dev)
-    reps = max(5, args.steps // 2)
-
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(reps):
-            fn()
-        b.record(stream)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps * 1e-3  # seconds per launch
-
-    cells = B * P * R
-    t_pc = timed(lambda: radar.pulse_compression_prepared(x, pipe.spectrum, out=pc_out))
-    stages = {"pc_fused_kernel<4096>": (t_pc, cells * 16 + R * 8)}
-    if args.cfar_mode == 1:
-        t_dc = timed(lambda: radar.doppler_cfar(pc_out, GUARD, REF, PFA, window="hann", threshold=thr,
-                                                detections=det))
-        stages["doppler_cfar_kernel<128, 2, 2, 8, 8>"] = (t_dc, cells * (8 + 4 + 1))
-    # reference-identical CFAR pipeline beside it
-    pipe0 = radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
-                                guard=GUARD, reference=REF, pfa=PFA, cfar_mode=0, max_batch=B)
-    t_mode0 = timed(lambda: pipe0.run(x, threshold=thr, detections=det))
-    del pipe0
-
     # ---- plain per-dwell drop-in ---------------------------------------------
-    nplain = min(args.plain_dwells, B)
-    for i in range(2):
-        y = radar.pulse_compression(x[i], t, normalize=True, window="hamming", widen=False)
-        pwr = radar.doppler_power(y, window="hann")
-        radar.ca_cfar(pwr, GUARD, REF, PFA)
-    torch.cuda.synchronize()
-    lat = []
-    t0 = time.perf_counter()
-    for i in range(nplain):
-        a = time.perf_counter()
-        y = radar.pulse_compression(x[i], t, normalize=True, window="hamming", widen=False)
-        pwr = radar.doppler_power(y, window="hann")
-        radar.ca_cfar(pwr, GUARD, REF, PFA)
-        torch.cuda.synchronize()
-        lat.append((time.perf_counter() - a) * 1e3)
-    plain_wall = time.perf_counter() - t0
-    plain_dps = nplain / plain_wall
+    plain_dps, lat = plain_dropin(radar, [x[i] for i in range(min(B, 8))], t, max(5, args.plain_dwells))
 
-    # ---- dominant kernel roofline ---------------------------------------------
+    # ---- roofline of the dominant kernel ---------------------------------------
+    db = counters_db()
+    stage_info = {}
+    for name, (t_k, bytes_k) in stages.items():
+        rec = db.get(name, {})
+        valu = rec.get("valu_insts_per_launch") if rec.get("batch") == B else None
+        stage_info[name] = {"us": t_k * 1e6, "alg_bytes": bytes_k, "hbm_frac": bytes_k / t_k / HBM_PEAK,
+                            # wave64 VALU instructions x 4 cycles on 1024 SIMDs at 2.4 GHz
+                            "valu_issue_frac": valu * 4 / (1024 * 2.4e9 * t_k) if valu else None,
+                            "traffic": rec.get("hbm_bytes_per_launch") if rec.get("batch") == B else None}
     name, (t_k, bytes_k) = max(stages.items(), key=lambda kv: kv[1][0])
     achieved = bytes_k / t_k
-    traffic = None
-    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tdb = json.load(fh)
-        rec = tdb.get("kernels", {}).get(name)
-        if rec and rec.get("batch") == B:
-            traffic = rec["hbm_bytes_per_launch"]
     roofline = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                "traffic": traffic, "avg_launch_us": t_k * 1e6, "alg_bytes_per_launch": bytes_k,
-                "stage_us": {k: v[0] * 1e6 for k, v in stages.items()}}
+                "traffic": stage_info[name]["traffic"], "valu_issue_frac": stage_info[name]["valu_issue_frac"],
+                "avg_launch_us": t_k * 1e6, "alg_bytes_per_launch": bytes_k, "stages": stage_info}
     pipe_bytes = B * (13 * P * R + 8 * TAPS)
     step_s = wall / args.steps
 
-    # ---- CPU oracle on the host --------------------------------------------
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-        from oracle import oracle as orc
-        # a bounded sample: distinct seeded dwells until about 10 s of CPU work
-        dwells_done, cpu_s = 0, 0.0
-        while dwells_done < 3 or (cpu_s < 10.0 and dwells_done < 30):
-            d = synth.make_dwell(P, R, TAPS, seed=100 + dwells_done)
-            c0 = time.perf_counter()
-            pc = orc.pulse_compression(d.x, d.template, window="hamming", normalize=True)
-            rd = orc.pulse_doppler(pc.astype(np.complex64), window="hann")
-            pwr_h = (np.abs(rd) ** 2).astype(np.float32)
-            orc.ca_cfar(pwr_h, GUARD, REF, PFA)
-            cpu_s += time.perf_counter() - c0
-            dwells_done += 1
-        cpu = {"value": dwells_done / cpu_s, "unit": "dwells/s", "cores": 1, "kind": "port",
-               "sample": f"{dwells_done} seeded dwells 128x4096 (about 10 s): direct FP64 circular "
-                         "correlation (256 taps), direct DFT Doppler, FP32 integral-image CFAR "
-                         "(the reference *_typed_cpu chain), input generation excluded",
-               "seconds": cpu_s}
+    # ---- the other configurations, briefly -------------------------------------
+    others = None if args.no_others else other_configs(max(5, min(20, args.steps // 10)))
+
+    # ---- CPU baseline on the host -----------------------------------------------
+    cpu = None if args.no_cpu else cpu_baseline()
 
     out = {
         "metric": "range-Doppler dwells/s + Msamples/s (HBM roofline %) and p99 latency",
@@ -630,7 +876,9 @@ def main():
         "plain_dropin_dwells_per_s": plain_dps,
         "speedup_vs_plain": dwells_per_s / plain_dps,
         "reference_identical_cfar_dwells_per_s": B / t_mode0,
+        "cfar_vs_reference": parity,
         "roofline": roofline,
+        "other_configs": others,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

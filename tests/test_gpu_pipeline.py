@@ -119,3 +119,32 @@ def test_fused_threshold_accuracy_vs_fp64(mods, seed, snr_db):
     assert np.all(t[~live] == 0)
     near = np.abs(p - exact) <= 1e-6 * np.abs(exact)
     assert not np.any((dd != (p.astype(np.float64) > exact)) & ~near)
+
+
+def test_headline_mask_vs_reference_cfar(mods):
+    """The headline pipeline (FP64 windowed CFAR fused with the Doppler FFT)
+    against the reference's own CFAR (the FP32 integral image, reproduced bit
+    for bit by ca_cfar) on the same power maps, 16 full-size dwells: masks
+    identical except cells whose statistic lies within 1e-6 relative of the
+    reference threshold, which are counted and bounded."""
+    radar, synth = mods
+    B = 16
+    cubes, tmpl, _ = synth.make_dwell_batch(B, 128, 4096, 256, seed=424242)
+    x = torch.from_numpy(cubes).cuda()
+    pipe = radar.DwellPipeline(tmpl, 128, 4096, max_batch=B)
+    thr, det = pipe.run(x)
+    pc = radar.pulse_compression_prepared(x, pipe.spectrum)
+    pw = torch.empty((B, 128, 4096), dtype=torch.float32, device="cuda")
+    t1, d1 = radar.doppler_cfar(pc, GUARD, REF, PFA, window="hann", power=pw)
+    # the bench's pipeline and the fused kernel alone give the same bits
+    assert torch.equal(d1, det) and torch.equal(t1.view(torch.int32), thr.view(torch.int32))
+    differ = excused = 0
+    for b in range(B):
+        t0, d0 = radar.ca_cfar(pw[b], GUARD, REF, PFA)
+        dm = d0 != d1[b]
+        near = (pw[b] - t0).abs() <= 1e-6 * t0.abs()
+        assert not bool((dm & ~near).any()), f"dwell {b}: mask differs outside the 1e-6 band"
+        differ += int(dm.sum())
+        excused += int((dm & near).sum())
+    print(f"{B * 128 * 4096} cells: {differ} differ from the reference mask, all {excused} within 1e-6", flush=True)
+    assert excused <= 16

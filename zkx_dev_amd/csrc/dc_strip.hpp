@@ -3,24 +3,31 @@
 // One 256-thread block walks a strip of range bins of one dwell in chunks of
 // CW = 256 / T columns (T = N / 16 threads per Doppler column).  Each chunk is
 // transformed once: its |Y|^2 columns enter a ring in LDS that also carries
-// the last 2*cm columns of the previous chunk, so the CFAR window of every
-// output column is complete without transforming any column twice (only the
-// 2*cm columns either side of a strip are shared with its neighbours).  The
-// samples of the next chunk are loaded into registers while the current chunk
-// is transformed and its CFAR computed (barriers wait on LDS only, so the
-// loads stay in flight across them).
+// the columns of the previous chunk the next outputs still need, so the CFAR
+// window of every output column is complete without transforming any column
+// twice (only the columns either side of a strip are read by both
+// neighbours).  The samples of the next chunk are loaded into registers while
+// the current chunk is transformed and its CFAR computed (barriers wait on
+// LDS only, so the loads stay in flight across them).
 //
-// Per chunk, with the transform leading the outputs by cm columns:
-//   ring slots [0, 2cm)         columns c_k - 2cm .. c_k - 1 (carried)
-//   ring slots [2cm, 2cm + CW)  chunk k, columns c_k .. c_k + CW - 1
-//   output chunk k - 1          columns c_k - cm .. c_k - cm + CW - 1
-// with c_k = S0 + cm - CW + k*CW for a strip starting at S0 (chunk 0 only
-// fills the carry).  The CFAR statistic is the same as the tile kernel's:
-// FP64 horizontal window sums rounded to float, FP64 vertical sliding sums,
-// threshold (alpha/N)*(outer - inner) in FP64, strict x > T, T = 0 on border
-// cells.  Horizontal runs cover 16 outputs aligned at multiples of 16 and
-// vertical runs 16 rows aligned at multiples of 16, so the result of a cell
-// does not depend on the strip or batch decomposition.
+// Per chunk, with the transform leading the outputs by LEAD columns (cm
+// rounded up to 16, so chunk loads start on 128-byte lines) and
+// CARRY = LEAD + cm:
+//   ring slots [0, CARRY)           columns c_k - CARRY .. c_k - 1 (carried)
+//   ring slots [CARRY, CARRY + CW)  chunk k, columns c_k .. c_k + CW - 1
+//   output chunk k - 1              columns c_k - LEAD .. c_k - LEAD + CW - 1
+// with c_k = S0 + LEAD - CW + k*CW for a strip starting at S0 (chunk 0 only
+// fills the carry); output i's window is ring slots i .. i + 2cm.
+//
+// The statistic: FP64 prefix sums along range give each row's outer window
+// sum H_o and side bands H_s = H_o - H_inner (formed in FP64 before rounding
+// to float); FP64 prefix sums down the columns then give
+//   annulus = sum over rows g0 < |df| <= rm of H_o + sum over |df| <= g0 of H_s,
+// a sum of rounded positive terms in which no strong guard cell has to cancel;
+// T = (alpha/N) * annulus in FP64, strict x > T, T = 0 on border cells.
+// Horizontal runs cover 16 outputs aligned at multiples of 16 and vertical
+// runs 16 rows aligned at multiples of 16, so the result of a cell does not
+// depend on the strip or batch decomposition.
 #pragma once
 
 #include "fft_lds.hpp"
@@ -61,7 +68,11 @@ struct StripCfg {
     static constexpr int CM = G1 + R1;          // range half-window
     static constexpr int RM = G0 + R0;          // Doppler half-window
     static constexpr int RUN = 16;              // outputs per horizontal task, rows per vertical task
-    static constexpr int RS = CW + 2 * CM + 1;  // ring row stride (odd: rows of a wave on distinct banks)
+    // the transform leads the outputs by LEAD >= CM columns, a multiple of 16
+    // (128 B of complex64), so every chunk's loads start on a cache line
+    static constexpr int LEAD = (CM + 15) / 16 * 16;
+    static constexpr int CARRY = LEAD + CM;       // ring columns carried from one chunk to the next
+    static constexpr int RS = (CW + CARRY) | 1;   // ring row stride (odd: rows of a wave on distinct banks)
     static constexpr int HS = N + 2 * RM + 1;   // window-sum column stride, RM zero rows either side
     static constexpr int RING_FLOATS = N * RS;
     static constexpr int H_FLOATS = 2 * CW * HS;
@@ -70,7 +81,7 @@ struct StripCfg {
     static constexpr int LDS_FLOATS = RING_FLOATS + (H_FLOATS > FFT_FLOATS ? H_FLOATS : FFT_FLOATS) + TW_FLOATS;
     static constexpr size_t LDS_BYTES = sizeof(float) * LDS_FLOATS;
     static_assert(E == 16 && N % RUN == 0 && CW % RUN == 0, "strip shape");
-    static_assert(CW >= 2 * CM, "the carried columns must come from one chunk");
+    static_assert(CW >= CARRY, "the carried columns must come from one chunk");
     static_assert(N * CW == THREADS * RUN, "one horizontal task per thread");
 };
 
@@ -140,7 +151,7 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
 {
     using C = StripCfg<N, G0, G1, R0, R1>;
     constexpr int E = C::E, T = C::T, CW = C::CW, CM = C::CM, RM = C::RM, RS = C::RS, HS = C::HS;
-    constexpr int RUN = C::RUN, NT = C::THREADS;
+    constexpr int RUN = C::RUN, NT = C::THREADS, LEAD = C::LEAD, CARRY = C::CARRY;
     const int tid = threadIdx.x;
     const int R = a.R;
     const long long cells = static_cast<long long>(N) * R;
@@ -194,17 +205,17 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
     constexpr int ZPT = (2 * RM + ZTPC - 1) / ZTPC;        // pad rows per thread
     const int zpc = tid / ZTPC, zpart = tid % ZTPC;
     float* zbase = hreg + zpc * HS;  // (plane, column) zpc: plane * CW * HS + column * HS
-    // carried columns this thread copies: CPT of the 2*CM of one row
+    // carried columns this thread copies: CPT of the CARRY of one row
     constexpr int CTPR = NT / N;                           // threads per ring row
-    constexpr int CPT = (2 * CM + CTPR - 1) / CTPR;
+    constexpr int CPT = (CARRY + CTPR - 1) / CTPR;
     const int crow = tid / CTPR, cpart = tid % CTPR;
     float* cbase = ring + crow * RS + cpart * CPT;
     static_assert(NT % (2 * CW) == 0 && NT % N == 0, "pad/carry split");
 
     float2 buf[E];
-    strip_load<N, G0, G1, R0, R1>(src, R, S0 + CM - CW + s, t, buf);
+    strip_load<N, G0, G1, R0, R1>(src, R, S0 + LEAD - CW + s, t, buf);
     for (int k = 0; k <= nout; ++k) {
-        const int ck = S0 + CM - CW + k * CW;  // first column of chunk k
+        const int ck = S0 + LEAD - CW + k * CW;  // first column of chunk k
         float2 v[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = cscale(buf[e], wtab[t + e * T]);
@@ -214,9 +225,9 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
         // (after the last chunk the column is out of range: zeros, no phi)
         strip_load<N, G0, G1, R0, R1>(src, R, k < nout ? ck + CW + s : -1, t, buf);
         fft::transform<N, E, false>(v, L, TWL, t);
-        // |Y|^2 into ring slots 2CM + s
+        // |Y|^2 into ring slots CARRY + s
 #pragma unroll
-        for (int e = 0; e < E; ++e) ring[(t + e * T) * RS + 2 * CM + s] = power_of(v[e]);
+        for (int e = 0; e < E; ++e) ring[(t + e * T) * RS + CARRY + s] = power_of(v[e]);
         if (maps) {  // optional map/power outputs for the strip's own columns
             const int col = ck + s;
             const bool own = col >= S0 && col < S1;
@@ -277,7 +288,7 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
                 for (int q = 0; q < RUN + 2 * RM; ++q) qv[q + 1] = qv[q] + static_cast<double>(hoc[q]);
 #pragma unroll
                 for (int q = 0; q < RUN + 2 * G0; ++q) zv[q + 1] = zv[q] + static_cast<double>(hsc[q]);
-                const int c = ck - CM + vo;  // output column of ring slot vo (window slots vo .. vo + 2CM)
+                const int c = ck - LEAD + vo;  // output column whose window is ring slots vo .. vo + 2CM
                 const bool colin = c >= CM && c < R - CM;
                 const bool store = c < S1 && c < R;
                 const unsigned voff4 = store ? (static_cast<unsigned>(f0) * R + c) * 4u : kOff;
@@ -300,10 +311,10 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
             }
         }
         __syncthreads();
-        // carry the last 2CM columns of this chunk to ring slots [0, 2CM)
+        // carry the last CARRY columns of the ring to slots [0, CARRY)
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
-            if (cpart * CPT + j < 2 * CM) cbase[j] = cbase[CW + j];
+            if (cpart * CPT + j < CARRY) cbase[j] = cbase[CW + j];
         }
     }
 }
