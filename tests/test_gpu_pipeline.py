@@ -97,21 +97,29 @@ def test_fused_doppler_cfar_windows_vs_fp64(mods, guard, ref):
 
 
 @pytest.mark.parametrize("seed,snr_db", [(7, -10.0), (11, 10.0)])
-def test_fused_threshold_accuracy_vs_fp64(mods, seed, snr_db):
-    """Full-size dwell, configured window: every live threshold of the fused
-    Doppler+CFAR kernel lies within 5e-7 relative of the exact FP64 statistic
-    on the same power map, also beside a strong (+10 dB per sample) target,
-    whose rounded horizontal sums must cancel out of the annulus."""
+@pytest.mark.parametrize("variant,guard,ref", [(0, GUARD, REF), (1, GUARD, REF), (0, (1, 3), (4, 6))])
+def test_fused_threshold_accuracy_vs_fp64(mods, seed, snr_db, variant, guard, ref):
+    """Full-size dwell: every live threshold of the fused Doppler+CFAR kernel
+    (variant 0 the strip form, 1 the tile form the batch engine shares; a
+    window other than the configured one runs the tile form with runtime
+    cells) lies within 5e-7 relative of the exact FP64 statistic on the same
+    power map, also beside a strong (+10 dB per sample) target, whose rounded
+    horizontal sums must cancel out of the annulus."""
+    from zkx_dev_amd._lib import lib
     radar, synth = mods
     d = synth.make_dwell(128, 4096, 256, seed=seed, snr_db=snr_db)
     x = torch.from_numpy(d.x).cuda()
     pipe = radar.DwellPipeline(d.template, 128, 4096)
     pc = radar.pulse_compression_prepared(x, pipe.spectrum)
     pw = torch.empty((128, 4096), dtype=torch.float32, device="cuda")
-    thr, det = radar.doppler_cfar(pc, GUARD, REF, PFA, window="hann", power=pw)
-    torch.cuda.synchronize()
+    lib().zkx_set_kernel_variant(b"doppler_cfar", variant)
+    try:
+        thr, det = radar.doppler_cfar(pc, guard, ref, PFA, window="hann", power=pw)
+        torch.cuda.synchronize()
+    finally:
+        lib().zkx_set_kernel_variant(b"doppler_cfar", 0)
     p, t, dd = pw.cpu().numpy(), thr.cpu().numpy(), det.cpu().numpy()
-    exact = orc.ca_cfar_exact(p, GUARD, REF, PFA)
+    exact = orc.ca_cfar_exact(p, guard, ref, PFA)
     live = exact != 0
     rel = np.abs(t.astype(np.float64) - exact)[live] / np.abs(exact[live])
     print(f"seed {seed}: max threshold rel err {rel.max():.2e}", flush=True)

@@ -35,8 +35,11 @@ __host__ __device__ inline double window_value(int kind, int i, int len)
 // by the neighbouring tiles), so the power map lives only in LDS.  The CFAR
 // covers all N Doppler rows of the tile, so it needs no row halo (rows within
 // g0+r0 of the edges are border cells with T = 0).  Window sums: FP64
-// horizontal sums kept as float, FP64 vertical sliding sums, FP64 threshold
-// and comparison.
+// horizontal sums of the outer window (H_o) and of its side bands
+// (H_s = H_o - H_inner, formed in FP64) kept as float; FP64 vertical sliding
+// sums give annulus = (rows |df| <= rm of H_o) - (rows |df| <= g0 of H_o) +
+// (rows |df| <= g0 of H_s), so a strong guard cell's rounded sums cancel
+// exactly; FP64 threshold and comparison.
 template <int N>
 struct FusedCfg {
     static constexpr int E = fft::elems_for(N);
@@ -117,14 +120,14 @@ __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile,
     }
     fft::transform<N, C::E, false>(v, L, tw, t);
     // |Y|^2 tile pw[N][NCM+1] over the FFT buffers, then the horizontal sums
-    // stored column-major, ho/hi[TCO][N+1], so that every LDS stride the
+    // stored column-major, ho/hs[TCO][N+1], so that every LDS stride the
     // vertical pass uses is a compile-time constant (immediate offsets).
     // Global offsets are 32-bit from the dwell's base (P*R < 2^31).
     constexpr int PS = C::NCM + 1;
     constexpr int NS = N + 1;
     float* pw = lds;
     float* ho = pw + N * PS;
-    float* hi = ho + (C::NCM - 2 * cm) * NS;
+    float* hs = ho + (C::NCM - 2 * cm) * NS;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
@@ -157,15 +160,15 @@ __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile,
                 si += static_cast<double>(pr[j + cm + g1]) - static_cast<double>(pr[j - 1 + cm - g1]);
             }
             vo[j] = static_cast<float>(so);
-            vi[j] = static_cast<float>(si);
+            vi[j] = static_cast<float>(so - si);
         }
         float* hoc = ho + o0 * NS + f;
-        float* hic = hi + o0 * NS + f;
+        float* hsc = hs + o0 * NS + f;
 #pragma unroll
         for (int j = 0; j < RUN; ++j) {
             if (o0 + j < TCO) {
                 hoc[j * NS] = vo[j];
-                hic[j * NS] = vi[j];
+                hsc[j * NS] = vi[j];
             }
         }
     }
@@ -187,11 +190,14 @@ __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile,
         const int fi = f0 > rm ? f0 : rm;
         const int fe = (f0 + VR) < (N - rm) ? (f0 + VR) : (N - rm);
         const float* hoc = ho + o * NS;
-        const float* hic = hi + o * NS;
-        double so = 0.0, si = 0.0;
+        const float* hsc = hs + o * NS;
+        double so = 0.0, sg = 0.0, sb = 0.0;
         if (colin && fi < fe) {
             for (int d = -rm; d <= rm; ++d) so += hoc[fi + d];
-            for (int d = -g0; d <= g0; ++d) si += hic[fi + d];
+            for (int d = -g0; d <= g0; ++d) {
+                sg += hoc[fi + d];
+                sb += hsc[fi + d];
+            }
         }
         int off = f0 * R + c;
 #pragma unroll
@@ -200,9 +206,10 @@ __device__ __forceinline__ void doppler_cfar_tile_buf(const DcTile& a, int tile,
             const bool live = colin && f >= fi && f < fe;
             if (live && f > fi) {
                 so += static_cast<double>(hoc[f + rm]) - static_cast<double>(hoc[f - rm - 1]);
-                si += static_cast<double>(hic[f + g0]) - static_cast<double>(hic[f - g0 - 1]);
+                sg += static_cast<double>(hoc[f + g0]) - static_cast<double>(hoc[f - g0 - 1]);
+                sb += static_cast<double>(hsc[f + g0]) - static_cast<double>(hsc[f - g0 - 1]);
             }
-            const double T = live ? a.alpha_over_n * (so - si) : 0.0;
+            const double T = live ? a.alpha_over_n * ((so - sg) + sb) : 0.0;
             a.thr[off] = static_cast<float>(T);
             a.det[off] = static_cast<double>(xv[j]) > T ? 1 : 0;
             off += R;
