@@ -1,7 +1,8 @@
 // Owning, move-only device buffer with the reference's DeviceArray interface
 // (ZKX/cusignal_cpp/include/cusignal/runtime/device_array.h:19-121) on the HIP
 // runtime: from_host / to_host copy synchronously on the default stream,
-// reset() frees, size() counts elements.
+// reset() frees, size() counts elements; blocks are counted by the device
+// allocation tracker.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -9,6 +10,7 @@
 #include <cstddef>
 #include <vector>
 
+#include "device_allocation_tracker.h"
 #include "errors.h"
 
 namespace cusignal {
@@ -32,7 +34,7 @@ public:
     explicit DeviceArray(std::size_t count) { allocate(count); }
     ~DeviceArray() noexcept
     {
-        if (data_) (void)hipFree(data_);
+        if (data_) (void)cuda_utils::tracked_device_free(data_);
     }
     DeviceArray(const DeviceArray&) = delete;
     DeviceArray& operator=(const DeviceArray&) = delete;
@@ -44,7 +46,7 @@ public:
     DeviceArray& operator=(DeviceArray&& o) noexcept
     {
         if (this != &o) {
-            if (data_) (void)hipFree(data_);
+            if (data_) (void)cuda_utils::tracked_device_free(data_);
             data_ = o.data_;
             size_ = o.size_;
             o.data_ = nullptr;
@@ -78,7 +80,7 @@ public:
     void reset()
     {
         if (data_) {
-            const hipError_t e = hipFree(data_);
+            const hipError_t e = cuda_utils::tracked_device_free(data_);
             data_ = nullptr;
             size_ = 0;
             cuda_utils::check_hip(e, "hipFree");
@@ -96,7 +98,8 @@ private:
     {
         reset();
         if (count) {
-            cuda_utils::check_hip(hipMalloc(reinterpret_cast<void**>(&data_), sizeof(T) * count), "hipMalloc");
+            cuda_utils::check_hip(cuda_utils::tracked_device_malloc(reinterpret_cast<void**>(&data_), sizeof(T) * count),
+                                  "hipMalloc");
             size_ = count;
         }
     }

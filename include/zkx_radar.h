@@ -67,7 +67,11 @@ int zkx_version(void);
 int zkx_max_lds_fft_length(void);
 /* Kernel variant used for an operation ("pulse_compression", ...), as chosen
  * by the autotuner; variants compute identical results.  No counterpart in
- * the reference (its launches have one fixed shape). */
+ * the reference (its launches have one fixed shape).  One more name is a
+ * test switch, not a variant: "inject_fft_failure" = 1 makes the transform
+ * stage of pulse_compression / pulse_doppler return ZKX_BACKEND_ERROR
+ * before any device work (the reference's synthesized backend status for
+ * its error-path tests), 0 restores normal operation. */
 int zkx_set_kernel_variant(const char* op, int variant);
 int zkx_get_kernel_variant(const char* op);
 

@@ -40,7 +40,7 @@ def test_python_binding_covers_header():
     assert not missing, missing
 
 
-@pytest.mark.parametrize("name", ["task1_chain", "filter_chain", "typed_ops"])
+@pytest.mark.parametrize("name", ["task1_chain", "filter_chain", "typed_ops", "fft_interface"])
 def test_cpp_operator_api_compiles(name):
     """The C++ operator API headers (include/cusignal) compile for a caller
     written like the reference's Task1 steps."""
@@ -51,3 +51,19 @@ def test_cpp_operator_api_compiles(name):
                         "-I", os.path.join(root, "include"), "-I", "/opt/rocm/include", src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_error_taxonomy_helpers(tmp_path):
+    """tests/cpp/errors_check.cpp: code values, aliases, name/parse helpers,
+    business dtypes and the code an argument error carries (host only)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "errors_check")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                        "-I", os.path.join(root, "include"), "-I", "/opt/rocm/include",
+                        os.path.join(root, "tests", "cpp", "errors_check.cpp"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    run = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 0, run.stdout + run.stderr
+

@@ -141,3 +141,74 @@ def test_filter_chain(tmp_path, axis):
     # resident 1-D path on the first n samples of x
     oy1, _ = orc.firfilter(h, x[:n])
     assert orc.rel_l2(fres, oy1) < 1e-5
+
+
+@pytest.mark.parametrize("n", [384, 1024])
+def test_fft_interface_objects(tmp_path, n):
+    """tests/cpp/fft_interface.cpp drives FFTInterface / FFTInterface2D
+    (fft_interface.h:499-558 of the reference) over the fft_* ABI: host and
+    device transforms against numpy's FP64 FFT (rel-L2 <= 1e-5, FP32 data),
+    device forms of one transform bit-identical to each other, the batch
+    plan pool through ten keys and an evicted one, padding/truncation of
+    host inputs, the error behaviour and the allocation tracker."""
+    batch, nx, ny = 12, 24, 40
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))).astype(np.complex64)
+    m = (rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))).astype(np.complex64)
+    hdr = np.array([n, batch, nx, ny], np.int32)
+    out = _run("fft_interface", hdr.tobytes() + x.tobytes() + m.tobytes(), tmp_path)
+    kinds = [np.complex128, np.complex128, np.complex64, np.complex64, np.complex64, np.complex64,
+             np.complex64, np.complex128, np.complex64, np.complex128, np.complex128, np.complex128,
+             np.complex128, np.complex64, np.complex64, np.complex64, np.int32, np.int64]
+    blocks, off = [], 0
+    for dtype in kinds:
+        cnt = int(np.frombuffer(out, np.int64, 1, off)[0])
+        off += 8
+        blocks.append(np.frombuffer(out, dtype, cnt, off))
+        off += blocks[-1].nbytes
+    assert off == len(out)
+    (fft0, ifft0, dev0, raw0, inpl0, bdev, braw, hbatch, bfloat, half, back, f2, if2, f2dev, if2dev,
+     sweep, flags, tracked) = blocks
+
+    x64 = x.astype(np.complex128)
+    tol = 1e-5
+    assert orc.rel_l2(fft0, np.fft.fft(x64[0])) < tol
+    assert orc.rel_l2(ifft0, np.fft.ifft(x64[0])) < tol
+    assert orc.rel_l2(dev0, np.fft.fft(x64[0])) < tol
+    # one plan, one kernel: the DeviceArray and raw-pointer forms agree bit for bit
+    assert np.array_equal(dev0.view(np.uint64), raw0.view(np.uint64))
+    assert orc.rel_l2(inpl0, np.fft.ifft(x64[0])) < tol
+    assert orc.rel_l2(bdev, np.fft.fft(x64, axis=1).reshape(-1)) < tol
+    assert orc.rel_l2(braw, np.fft.ifft(x64, axis=1).reshape(-1)) < tol
+    cut = x64.copy()
+    cut[1, n // 2:] = 0  # row 1 was passed with n/2 values: zero padded
+    assert orc.rel_l2(hbatch, np.fft.fft(cut, axis=1).reshape(-1)) < tol
+    short = x64.reshape(-1).copy()
+    short[-3:] = 0  # the last 3 values were missing: zero padded
+    assert orc.rel_l2(bfloat, np.fft.ifft(short.reshape(batch, n), axis=1).reshape(-1)) < tol
+    xr = x64[0].real
+    assert half.size == n // 2 + 1
+    assert orc.rel_l2(half, np.fft.rfft(xr)) < tol
+    assert back.size == n and np.all(back.imag == 0)
+    assert orc.rel_l2(back.real, xr) < tol
+    m64 = m.astype(np.complex128)
+    assert orc.rel_l2(f2, np.fft.fft2(m64).reshape(-1)) < tol
+    assert orc.rel_l2(if2, np.fft.ifft2(m64).reshape(-1)) < tol
+    assert orc.rel_l2(f2dev, np.fft.fft2(m64).reshape(-1)) < tol
+    assert orc.rel_l2(if2dev, np.fft.ifft2(m64).reshape(-1)) < tol
+    # pool sweep: b = 1..10 then 1 again, each the FFT of the first b rows
+    off = 0
+    for b in list(range(1, 11)) + [1]:
+        seg = sweep[off:off + b * n]
+        off += b * n
+        assert orc.rel_l2(seg, np.fft.fft(x64[:b], axis=1).reshape(-1)) < tol, b
+    assert off == sweep.size
+    # bit-identical across pool rebuilds (same plan shape, same kernels)
+    assert np.array_equal(sweep[:n].view(np.uint64), sweep[-n:].view(np.uint64))
+    # invalid length, size mismatch, batch 0, batch mismatch, BatchOnly host
+    # call, invalid 2-D dims, 2-D size mismatch, empty host batch, batch 0
+    # float batch, unsupported length
+    assert flags.tolist() == [1] * 10, flags.tolist()
+    # 2 staging buffers of n complex64 while an FFTInterface(n) lives; n for a DeviceArray
+    before, alive, after, arr = tracked.tolist()
+    assert alive - before == 2 * n * 8 and after == before and arr - before == n * 8

@@ -261,3 +261,46 @@ def test_output_tensors_validated_before_launch(R):
     with pytest.raises(ValueError):
         R.doppler_cfar(dev(rng_c64(rng, (32, 128))), (2, 2), (8, 8), 1e-6,
                        power=torch.empty(32 * 128 - 1, device="cuda"))
+
+
+@pytest.mark.parametrize("nfft", [-1, 200])
+def test_pulse_compression_template_longer_than_pulse(R, nfft):
+    """The reference's operator accepts a template longer than the pulse and
+    correlates with its first nfft taps after windowing and normalising the
+    whole template (radartools_typed.cpp:618-640)."""
+    rng = np.random.default_rng(5)
+    P, S, T = 8, 256, 300
+    x = (rng.standard_normal((P, S)) + 1j * rng.standard_normal((P, S))).astype(np.complex64)
+    h = (rng.standard_normal(T) + 1j * rng.standard_normal(T)).astype(np.complex64)
+    y = R.pulse_compression(dev(x), dev(h), nfft=nfft, window="hann", normalize=True).cpu().numpy()
+    ref = orc.pulse_compression_fft64(x, h, nfft=nfft, window="hann", normalize=True)
+    assert y.shape == ref.shape
+    assert orc.rel_l2(y, ref) < TOL
+
+
+def test_injected_fft_failure_is_backend_error_and_writes_nothing(R):
+    """The host-forced transform failure returns BACKEND_ERROR before any
+    device work: the preallocated outputs keep their contents."""
+    import ctypes
+    from zkx_dev_amd._lib import BACKEND_ERROR, ZkxError, check, lib
+    L = lib()
+    P, S = 16, 128
+    x = torch.randn(P, S, dtype=torch.complex64, device="cuda")
+    h = torch.randn(16, dtype=torch.complex64, device="cuda")
+    out = torch.full((P, S), 3.5 + 0j, dtype=torch.complex64, device="cuda")
+    power = torch.full((P, S), -2.0, dtype=torch.float32, device="cuda")
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    check(L.zkx_set_kernel_variant(b"inject_fft_failure", 1))
+    try:
+        pc = L.zkx_pulse_compression_c64(ptr(x), P, S, ptr(h), 16, -1, 0, 1, None, ptr(out), None)
+        pd = L.zkx_pulse_doppler_c64(ptr(x), P, S, -1, 1, None, ptr(out), ptr(power), None)
+        with pytest.raises(ZkxError) as e:
+            R.pulse_doppler(x, window="hann")
+    finally:
+        check(L.zkx_set_kernel_variant(b"inject_fft_failure", 0))
+    torch.cuda.synchronize()
+    assert pc == BACKEND_ERROR and pd == BACKEND_ERROR and e.value.code == BACKEND_ERROR
+    assert bool(torch.all(out == 3.5 + 0j)) and bool(torch.all(power == -2.0))
+    # normal operation again
+    y = R.pulse_doppler(x, window="hann", widen=False)
+    assert orc.rel_l2(y.cpu().numpy(), orc.pulse_doppler_fft64(x.cpu().numpy(), window="hann")) < TOL

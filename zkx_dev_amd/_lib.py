@@ -91,6 +91,10 @@ _lib = None
 _lock = threading.Lock()
 
 
+INVALID_ARGUMENT, INVALID_SHAPE, INVALID_DTYPE = 1001, 1002, 1003
+BACKEND_ERROR = 2099
+
+
 class ZkxError(RuntimeError):
     """A non-zero return from the C ABI; .code is the reference ErrorCode."""
 
@@ -98,6 +102,13 @@ class ZkxError(RuntimeError):
         self.code = code
         self.name = ERRORS.get(code, str(code))
         super().__init__(f"{self.name} ({code}): {message}")
+
+
+class ArgumentError(ZkxError, ValueError):
+    """An invalid argument, shape or dtype (codes 1001-1003), rejected before
+    any device work.  A ValueError, as the reference throws
+    std::invalid_argument for these, that also carries the ErrorCode the C
+    ABI reports for the same problem."""
 
 
 def lib():
@@ -120,4 +131,6 @@ def lib():
 def check(code):
     if code != 0:
         msg = lib().zkx_last_error_message().decode(errors="replace")
+        if code in (INVALID_ARGUMENT, INVALID_SHAPE, INVALID_DTYPE):
+            raise ArgumentError(code, msg)
         raise ZkxError(code, msg)

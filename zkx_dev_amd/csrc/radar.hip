@@ -16,6 +16,7 @@
 #include "runtime.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <limits>
 #include <mutex>
@@ -1036,6 +1037,19 @@ int launch_pc_segments(const float2* x, long long rows, int nseg, int L, int nff
 // Kernel variant per operation, chosen by the autotuner (zkx_set_kernel_variant).
 int g_pc_variant = 0;
 
+// Host-forced transform-stage failure for error-path tests: while set, the
+// transform stage of pulse_compression / pulse_doppler reports BACKEND_ERROR
+// before any device work, as a failed FFT execution does in the reference
+// (its synthesized backend status, operator_abnormal_validation.h:93-104).
+std::atomic<int> g_inject_fft_failure{0};
+
+int injected_fft_failure(const char* op)
+{
+    if (!g_inject_fft_failure.load(std::memory_order_relaxed)) return ZKX_SUCCESS;
+    const std::string msg = std::string(op) + ": FFT execution failed (injected backend status)";
+    ZKX_FAIL(ZKX_BACKEND_ERROR, msg.c_str());
+}
+
 template <int N, int MINW>
 int launch_pc_persistent(const float2* x, long long rows, int S, const float2* H, const float2* tw, float2* y,
                          int per_cu, hipStream_t st)
@@ -1493,6 +1507,11 @@ int zkx_set_kernel_variant(const char* op, int variant)
         g_dc_strip_blocks = variant;
         return ZKX_SUCCESS;
     }
+    if (name == "inject_fft_failure") {
+        if (variant < 0 || variant > 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "inject_fft_failure must be 0 or 1");
+        g_inject_fft_failure.store(variant);
+        return ZKX_SUCCESS;
+    }
 
     ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unknown kernel variant op");
 }
@@ -1506,6 +1525,7 @@ int zkx_get_kernel_variant(const char* op)
     if (name == "fir_fft") return g_fir_fft;
     if (name == "doppler_cfar") return g_dc_variant;
     if (name == "doppler_cfar_strip_blocks") return g_dc_strip_blocks;
+    if (name == "inject_fft_failure") return g_inject_fft_failure.load();
     return -1;
 }
 
@@ -1533,6 +1553,7 @@ int zkx_pulse_compression_prepared_c64(const zkx_complex64* x, int num_pulses, i
     clear_last_error();
     if (num_pulses < 1 || samples_per_pulse < 1 || nfft < 1 || !x || !spectrum || !out)
         ZKX_FAIL(ZKX_INVALID_SHAPE, "pulse_compression shape is invalid");
+    if (int rc = injected_fft_failure("pulse_compression")) return rc;
     return pc_fused(nfft, reinterpret_cast<const float2*>(x), num_pulses, samples_per_pulse,
                     reinterpret_cast<const float2*>(spectrum), reinterpret_cast<float2*>(out), (hipStream_t)stream);
 }
@@ -1552,6 +1573,7 @@ int zkx_pulse_compression_c64(const zkx_complex64* x, int num_pulses, int sample
     if (!valid_window(window)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unsupported pulse_compression window");
     if (explicit_kind(window) != (explicit_window != nullptr))
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_compression window is invalid");
+    if (int rc = injected_fft_failure("pulse_compression")) return rc;
     // per-call spectrum buffer, kept per host thread like the reference's plan pool
     static thread_local ThreadWorkspace ws;
     if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft) {
@@ -1596,6 +1618,7 @@ int zkx_pulse_doppler_c64(const zkx_complex64* x, int num_pulses, int samples_pe
     if (!valid_window(window)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "unsupported pulse_doppler window");
     if (explicit_kind(window) != (explicit_window != nullptr))
         ZKX_FAIL(ZKX_INVALID_ARGUMENT, "pulse_doppler window is invalid");
+    if (int rc = injected_fft_failure("pulse_doppler")) return rc;
     if (!is_pow2(nfft) || nfft < 2 || nfft > kMaxLdsFft) {
         // other lengths: FP32 row transforms of any length (Bluestein beyond
         // the powers of two, as the reference's backend does)

@@ -16,7 +16,7 @@ import math
 import numpy as np
 import torch
 
-from ._lib import check, lib
+from ._lib import INVALID_ARGUMENT, INVALID_DTYPE, INVALID_SHAPE, ArgumentError, check, lib
 from .radar import _ptr, _stream
 
 
@@ -24,7 +24,7 @@ def _lines(shape, axis):
     rank = len(shape)
     ax = axis + rank if axis < 0 else axis
     if ax < 0 or ax >= rank:
-        raise ValueError("axis is outside rank")
+        raise ArgumentError(INVALID_ARGUMENT, "axis is outside rank")
     outer = int(np.prod(shape[:ax])) if ax > 0 else 1
     inner = int(np.prod(shape[ax + 1:])) if ax + 1 < rank else 1
     return outer, int(shape[ax]), inner, ax
@@ -32,9 +32,9 @@ def _lines(shape, axis):
 
 def _signal(x, name="x"):
     if not isinstance(x, torch.Tensor) or not x.is_cuda:
-        raise ValueError(f"{name} must be a device tensor")
+        raise ArgumentError(INVALID_ARGUMENT, f"{name} must be a device tensor")
     if x.dtype not in (torch.float32, torch.complex64):
-        raise ValueError(f"{name} must be float32 or complex64")
+        raise ArgumentError(INVALID_DTYPE, f"{name} must be float32 or complex64")
     return x.contiguous()
 
 
@@ -50,11 +50,11 @@ def upfirdn(h, x, up=1, down=1, axis=-1, *, stream=None):
     x = _signal(x)
     h = _taps(h)
     if h.numel() == 0:
-        raise ValueError("upfirdn coefficients")
+        raise ArgumentError(INVALID_SHAPE, "upfirdn coefficients")
     if up < 1 or down < 1:
-        raise ValueError("upfirdn factors")
+        raise ArgumentError(INVALID_ARGUMENT, "upfirdn factors")
     if x.dim() < 1 or x.dim() > 2:
-        raise ValueError("upfirdn rank")
+        raise ArgumentError(INVALID_SHAPE, "upfirdn rank")
     outer, n, inner, ax = _lines(x.shape, axis)
     out_n = lib().zkx_upfirdn_out_len(n, h.numel(), up, down)
     shape = list(x.shape)
@@ -119,14 +119,14 @@ def resample_poly(x, up, down, axis=0, *, coefficients=None, window="kaiser", be
     """
     x = _signal(x)
     if up < 1 or down < 1:
-        raise ValueError("resample_poly factors")
+        raise ArgumentError(INVALID_ARGUMENT, "resample_poly factors")
     outer, n, inner, ax = _lines(x.shape, axis)
     g = math.gcd(up, down)
     u, d = up // g, down // g
     if u == 1 and d == 1:
         return x.clone()
     if x.dim() > 2:
-        raise ValueError("resample_poly nonidentity rank")
+        raise ArgumentError(INVALID_SHAPE, "resample_poly nonidentity rank")
     if coefficients is not None and isinstance(coefficients, torch.Tensor) and coefficients.is_cuda and u < 65536:
         # device-resident taps are scaled on the device (the same float32 product)
         taps_d = (coefficients.to(torch.float32) * float(np.float32(u))).contiguous()
@@ -157,12 +157,14 @@ def firfilter(b, x, axis=-1, zi=None, *, stream=None):
     """firfilter_device (filtering_typed.h:552-565): causal FIR y = b * x along
     `axis`, first N samples; with zi (broadcast to the state shape, axis
     length K-1) also returns zf.  float32 only, as the reference."""
-    if not isinstance(x, torch.Tensor) or not x.is_cuda or x.dtype != torch.float32:
-        raise ValueError("x must be a float32 device tensor")
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise ArgumentError(INVALID_ARGUMENT, "x must be a float32 device tensor")
+    if x.dtype != torch.float32:
+        raise ArgumentError(INVALID_DTYPE, "x must be a float32 device tensor")
     x = x.contiguous()
     b = _taps(b)
     if b.numel() == 0:
-        raise ValueError("firfilter coefficients")
+        raise ArgumentError(INVALID_SHAPE, "firfilter coefficients")
     outer, n, inner, ax = _lines(x.shape, axis)
     y = torch.empty_like(x)
     zf = zi_t = None
@@ -173,7 +175,7 @@ def firfilter(b, x, axis=-1, zi=None, *, stream=None):
         try:
             zi_t = zi_t.broadcast_to(state_shape).contiguous()
         except RuntimeError as e:
-            raise ValueError("firfilter zi broadcast shape") from e
+            raise ArgumentError(INVALID_SHAPE, "firfilter zi broadcast shape") from e
         zf = torch.empty(state_shape, dtype=torch.float32, device=x.device)
     check(lib().zkx_firfilter_f32(_ptr(b), b.numel(), _ptr(x), outer, n, inner, _ptr(zi_t), _ptr(y),
                                   _ptr(zf), _stream(stream)))

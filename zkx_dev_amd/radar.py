@@ -4,9 +4,10 @@ Names, argument meaning, dtype rules and errors follow the reference's
 cusignal_cpp radartools API (ZKX/cusignal_cpp/include/cusignal/operators/
 radartools/radartools_typed.h): pulse_compression, pulse_doppler, cfar_alpha,
 ca_cfar.  Every call goes through the C ABI of libzkx_radar.so; torch only
-provides device memory and the current stream.  Shape/argument problems raise
-ValueError (the reference throws std::invalid_argument); runtime failures raise
-ZkxError carrying the reference ErrorCode.
+provides device memory and the current stream.  Shape/argument/dtype problems
+raise ArgumentError, a ValueError (the reference throws std::invalid_argument)
+carrying the ErrorCode (INVALID_SHAPE, INVALID_ARGUMENT, INVALID_DTYPE);
+runtime failures raise ZkxError carrying the reference ErrorCode.
 """
 from __future__ import annotations
 
@@ -15,7 +16,7 @@ import ctypes
 import torch
 
 from ._check import out_tensor
-from ._lib import ZkxError, check, lib
+from ._lib import INVALID_ARGUMENT, INVALID_DTYPE, INVALID_SHAPE, ArgumentError, ZkxError, check, lib
 
 WINDOWS = {"none": 0, "hann": 1, "hamming": 2, "explicit_fp32": 3, "explicit_fp64": 4}
 
@@ -31,22 +32,22 @@ def _ptr(t):
 
 def _device_tensor(t, dtype, name):
     if not isinstance(t, torch.Tensor) or not t.is_cuda:
-        raise ValueError(f"{name} must be a device tensor")
+        raise ArgumentError(INVALID_ARGUMENT, f"{name} must be a device tensor")
     if t.dtype != dtype:
-        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+        raise ArgumentError(INVALID_DTYPE, f"{name} must be {dtype}, got {t.dtype}")
     return t.contiguous()
 
 
 def _window_code(window, explicit_window, length, name):
     if window not in WINDOWS:
-        raise ValueError(f"unsupported {name} window")
+        raise ArgumentError(INVALID_ARGUMENT, f"unsupported {name} window")
     code = WINDOWS[window]
     if code >= 3:
         if explicit_window is None or explicit_window.numel() != length:
-            raise ValueError("explicit radar window size mismatch")
+            raise ArgumentError(INVALID_ARGUMENT, "explicit radar window size mismatch")
         explicit_window = _device_tensor(explicit_window, torch.float32, "explicit_window")
     elif explicit_window is not None:
-        raise ValueError("explicit radar window requires explicit_fp32 or explicit_fp64 kind")
+        raise ArgumentError(INVALID_ARGUMENT, "explicit radar window requires explicit_fp32 or explicit_fp64 kind")
     return code, explicit_window
 
 
@@ -64,14 +65,14 @@ def pulse_compression(x, template, *, normalize=False, nfft=-1, window="none",
     Returns [num_pulses, nfft]: complex128 storage when the reference would
     return ComplexFP64 (window != none) and widen=True, else complex64.
     """
+    if isinstance(x, torch.Tensor) and x.dim() != 2:
+        raise ArgumentError(INVALID_SHAPE, "pulse_compression shape is invalid")
     x = _device_tensor(x, torch.complex64, "x")
     template = _device_tensor(template, torch.complex64, "template")
-    if x.dim() != 2:
-        raise ValueError("pulse_compression shape is invalid")
     P, S = x.shape
     n = S if nfft == -1 else int(nfft)
     if n < 1 or template.numel() < 1:
-        raise ValueError("pulse_compression shape is invalid")
+        raise ArgumentError(INVALID_SHAPE, "pulse_compression shape is invalid")
     code, ew = _window_code(window, explicit_window, template.numel(), "pulse_compression")
     out = torch.empty((P, n), dtype=torch.complex64, device=x.device)
     check(lib().zkx_pulse_compression_c64(_ptr(x), P, S, _ptr(template), template.numel(), n,
@@ -120,19 +121,19 @@ def pulse_doppler(x, *, nfft=-1, window="none", explicit_window=None, out=None, 
     there and returned unwidened.  `power` (float32, same shape) optionally
     receives |Y|^2.
     """
+    if isinstance(x, torch.Tensor) and x.dim() != 2:
+        raise ArgumentError(INVALID_SHAPE, "pulse_doppler shape is invalid")
     x = _device_tensor(x, torch.complex64, "x")
-    if x.dim() != 2:
-        raise ValueError("pulse_doppler shape is invalid")
     P, R = x.shape
     n = P if nfft == -1 else int(nfft)
     if n < 1:
-        raise ValueError("pulse_doppler shape is invalid")
+        raise ArgumentError(INVALID_SHAPE, "pulse_doppler shape is invalid")
     code, ew = _window_code(window, explicit_window, P, "pulse_doppler")
     preallocated = out is not None
     if out is None:
         out = torch.empty((n, R), dtype=torch.complex64, device=x.device)
     elif out.numel() != n * R or out.dtype != torch.complex64:
-        raise ValueError("pulse_doppler preallocated output shape mismatch")
+        raise ArgumentError(INVALID_SHAPE, "pulse_doppler preallocated output shape mismatch")
     out_tensor(out, "out", x, torch.complex64, n * R)
     out_tensor(power, "power", x, torch.float32, n * R)
     check(lib().zkx_pulse_doppler_c64(_ptr(x), P, R, n, code, _ptr(ew), _ptr(out), _ptr(power),
@@ -173,13 +174,34 @@ def doppler_dft_mfma(x, *, window="none", terms=3, out=None, power=None, stream=
     return y if x.dim() == 3 else y[0]
 
 
+def _alpha_args(pfa, reference):
+    """Scalar checks of cfar_alpha's inputs: a real scalar pfa (INVALID_DTYPE
+    for complex or non-numeric input, the reference's typed entry points take
+    only real business types) and a scalar reference-cell count
+    (INVALID_SHAPE otherwise)."""
+    import numpy as np
+    for v, name in ((pfa, "pfa"), (reference, "reference")):
+        if isinstance(v, torch.Tensor):
+            if v.numel() != 1 or v.dim() > 1:
+                raise ArgumentError(INVALID_SHAPE, f"cfar_alpha {name} must be a scalar")
+            if v.is_complex() or v.dtype == torch.bool:
+                raise ArgumentError(INVALID_DTYPE, f"cfar_alpha {name} must be real")
+        elif np.ndim(v) != 0:
+            raise ArgumentError(INVALID_SHAPE, f"cfar_alpha {name} must be a scalar")
+        elif isinstance(v, (bool, np.bool_)) or not isinstance(v, (int, float, np.integer, np.floating)):
+            raise ArgumentError(INVALID_DTYPE, f"cfar_alpha {name} must be real")
+    if isinstance(reference, float) and not float(reference).is_integer():
+        raise ArgumentError(INVALID_ARGUMENT, "cfar_alpha reference must be an integer count")
+    ref = reference.item() if isinstance(reference, torch.Tensor) else reference
+    p = pfa.item() if isinstance(pfa, torch.Tensor) else pfa
+    return float(p), int(ref)
+
+
 def cfar_alpha(pfa, reference):
     """cfar_alpha_typed_cpu (radartools_typed.cpp:592-603): N*(pfa^(-1/N)-1), FP64."""
+    pfa, reference = _alpha_args(pfa, reference)
     out = ctypes.c_double()
-    code = lib().zkx_cfar_alpha(float(pfa), int(reference), ctypes.byref(out))
-    if code == 1001:
-        raise ValueError("cfar_alpha requires reference >= 1 and pfa in (0,1]")
-    check(code)
+    check(lib().zkx_cfar_alpha(pfa, reference, ctypes.byref(out)))
     return out.value
 
 
@@ -187,9 +209,10 @@ def cfar_alpha_device(pfa, reference, *, stream=None):
     """cfar_alpha_device<float> (radartools_typed.cpp:605-617): alpha computed in
     FP32 on the device (powf) from the pfa loaded to float, returned as a
     one-element float64 device tensor."""
-    p = float(torch.tensor(float(pfa), dtype=torch.float32))
-    if int(reference) < 1 or not (p > 0.0 and p <= 1.0):
-        raise ValueError("cfar_alpha requires reference >= 1 and pfa in (0,1]")
+    pfa, reference = _alpha_args(pfa, reference)
+    p = float(torch.tensor(pfa, dtype=torch.float32))
+    if reference < 1 or not (p > 0.0 and p <= 1.0):
+        raise ArgumentError(INVALID_ARGUMENT, "cfar_alpha requires reference >= 1 and pfa in (0,1]")
     out = torch.empty(1, dtype=torch.float64, device=torch.device("cuda", torch.cuda.current_device()))
     check(lib().zkx_cfar_alpha_f32_device(ctypes.c_float(p), int(reference), _ptr(out), _stream(stream)))
     return out
@@ -246,23 +269,23 @@ def ca_cfar(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=None, detect
     Returns (threshold float32, detections uint8), bit-identical to the
     reference's FP32 integral-image CFAR.
     """
+    if isinstance(x, torch.Tensor) and x.dim() > 2:
+        raise ArgumentError(INVALID_SHAPE, "ca_cfar supports only scalar, 1D, or 2D input")
     x = _device_tensor(x, torch.float32, "x")
     rank = x.dim()
-    if rank > 2:
-        raise ValueError("ca_cfar supports only scalar, 1D, or 2D input")
     if rank == 0:
         thr = torch.zeros_like(x)
         return thr, (x > thr).to(torch.uint8)
     guard = list(guard_cells)
     ref = list(reference_cells)
     if len(guard) != rank or len(ref) != rank:
-        raise ValueError("ca_cfar cell parameter rank mismatch")
+        raise ArgumentError(INVALID_ARGUMENT, "ca_cfar cell parameter rank mismatch")
     if threshold is None:
         threshold = torch.empty_like(x)
     if detections is None:
         detections = torch.empty(x.shape, dtype=torch.uint8, device=x.device)
     if threshold.numel() != x.numel() or detections.numel() != x.numel():
-        raise ValueError("ca_cfar preallocated output shape must match input")
+        raise ArgumentError(INVALID_SHAPE, "ca_cfar preallocated output shape must match input")
     out_tensor(threshold, "threshold", x, torch.float32, x.numel())
     out_tensor(detections, "detections", x, torch.uint8, x.numel())
     shape = (ctypes.c_int * rank)(*x.shape)
@@ -270,8 +293,6 @@ def ca_cfar(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=None, detect
     r = (ctypes.c_int * rank)(*ref)
     code = lib().zkx_ca_cfar_f32(_ptr(x), rank, shape, g, r, float(pfa), _ptr(threshold),
                                  _ptr(detections), None, _stream(stream))
-    if code in (1001, 1002):
-        raise ValueError(lib().zkx_last_error_message().decode())
     check(code)
     return threshold, detections
 
@@ -281,7 +302,7 @@ def ca_cfar_windowed(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=Non
     """Rank-2 CA-CFAR with FP64 windowed box sums (the fused pipeline's CFAR)."""
     x = _device_tensor(x, torch.float32, "x")
     if x.dim() != 2:
-        raise ValueError("ca_cfar_windowed needs a 2-D map")
+        raise ArgumentError(INVALID_SHAPE, "ca_cfar_windowed needs a 2-D map")
     if threshold is None:
         threshold = torch.empty_like(x)
     if detections is None:
@@ -292,8 +313,6 @@ def ca_cfar_windowed(x, guard_cells, reference_cells, pfa=1e-3, *, threshold=Non
                                         guard_cells[1], reference_cells[0], reference_cells[1],
                                         float(pfa), _ptr(threshold), _ptr(detections),
                                         _stream(stream))
-    if code in (1001, 1002):
-        raise ValueError(lib().zkx_last_error_message().decode())
     check(code)
     return threshold, detections
 
@@ -317,8 +336,6 @@ def doppler_cfar(x, guard_cells=(2, 2), reference_cells=(8, 8), pfa=1e-6, *, win
     code = lib().zkx_doppler_cfar_c64(_ptr(x), B, P, R, WINDOWS[window], guard_cells[0], guard_cells[1],
                                       reference_cells[0], reference_cells[1], float(pfa), _ptr(threshold),
                                       _ptr(detections), _ptr(rd_map), _ptr(power), _stream(stream))
-    if code in (1001, 1002):
-        raise ValueError(lib().zkx_last_error_message().decode())
     check(code)
     return threshold, detections
 
