@@ -37,10 +37,12 @@ def _dominates(s1, s2):
 @pytest.mark.parametrize("mode", ["full", "same", "valid"])
 @pytest.mark.parametrize("boundary", ["fill", "wrap", "symm"])
 def test_correlate2d_bit_identical(name, s1, s2, mode, boundary):
-    """16x256 output tiles (4 x 4 outputs per thread) with partial last tiles
-    and several column blocks, the 16x64 form for kernels whose wide window
-    exceeds its LDS budget (60 x 70), in2 larger than in1, even kernel sizes
-    ('same' origin r2/2, c2/2), a nonzero fill value."""
+    """The three dispatch forms: the wide kernel (8 x 256 outputs per block,
+    2 x 4 per thread, window in at most 48 KiB of LDS) with partial last
+    tiles and several column blocks, the 16 x 64 tile kernel for kernels
+    whose wide window exceeds that budget (60 x 70), and the per-term direct
+    form beyond both; in2 larger than in1, even kernel sizes ('same' origin
+    r2/2, c2/2), a nonzero fill value."""
     from zkx_dev_amd import convolution
     if mode == "valid" and not _dominates(s1, s2):
         pytest.skip("valid needs one input to dominate")
