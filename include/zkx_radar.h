@@ -520,11 +520,34 @@ int zkx_correlate_direct(const void* in1, long long l1, const void* in2, long lo
  * reversed, the full convolution by the overlap-save FFT, the centred crop
  * stored in the element type (half rounded to nearest; integers rounded half
  * to even and wrapped modulo 2^bits).  FP32 FFT results, not the direct sums
- * bit for bit.  The shorter input may have at most 4096 samples
- * (UNSUPPORTED_OPERATION beyond); the library holds the workspace.
+ * bit for bit.  Any lengths with l1 + l2 - 1 <= 2^26 (UNSUPPORTED_OPERATION
+ * beyond): overlap-save when one input has at most 4096 samples, one
+ * whole-length power-of-two transform pair otherwise; the library holds the
+ * workspace.
  */
 int zkx_correlate_fft(const void* in1, long long l1, const void* in2, long long l2, int dtype, int mode, void* out,
                       void* stream);
+
+/*
+ * N-D cross-correlation by FFT.  Replaces correlate_nd_output_shape and
+ * correlate_device(in1, in2, out, CorrelateNDWorkspace&)
+ * (convolution_typed.h:93,331-337; convolution_typed.cu:150-241;
+ * convolution_typed.cpp:153-199).  Row-major real inputs of one `dtype`,
+ * equal rank 0..8.  Output extents per dimension: full s1 + s2 - 1, same
+ * s1, valid |s1 - s2| + 1 ('valid' needs one input at least as large as the
+ * other in every dimension, INVALID_SHAPE otherwise); an empty input gives
+ * an empty output (elements 0, out_shape[0] = 0); rank 0 is the product of
+ * two scalars (elements 1).  Computation: every dimension padded to the
+ * power of two >= s1 + s2 - 1, in2 reversed, complex FP32 transforms along
+ * each dimension, the centred crop (start (full - out)/2) stored in the
+ * element type as zkx_correlate_fft does.  Rank 1 runs zkx_correlate_fft.
+ * Each padded operand may hold at most 2^31 - 1 elements and each padded
+ * extent at most 2^26.  out must not overlap an input.
+ */
+int zkx_correlate_nd_out_shape(const int* shape1, const int* shape2, int rank, int mode, int* out_shape,
+                               long long* elements);
+int zkx_correlate_nd(const void* in1, const int* shape1, const void* in2, const int* shape2, int rank, int dtype,
+                     int mode, void* out, void* stream);
 
 /*
  * 2-D cross-correlation, direct, row-major, in the element type `dtype`.
@@ -563,6 +586,29 @@ int zkx_firwin_lowpass_hamming(int numtaps, float cutoff, double* taps);
 long long zkx_fftconvolve_out_len(long long n1, long long n2, int mode);
 int zkx_fftconvolve_c64(const zkx_complex64* in1, int batch, int n1, const zkx_complex64* in2,
                         int n2, int mode, zkx_complex64* out, void* stream);
+
+/*
+ * fftconvolve over N-D axes (cuSignal convolve.py:159-320 with
+ * convolution_utils.py:40-100): in1 [shape1], in2 [shape2], equal rank
+ * 1..8, row-major, each float32 (complex = 0) or complex64 (complex = 1);
+ * convolved over the dimensions whose bit is set in axes_mask (all:
+ * (1 << rank) - 1); on the other dimensions the extents must be equal or 1
+ * (broadcast), INVALID_SHAPE otherwise.  Extents of the result: full
+ * s1 + s2 - 1 on a convolved axis and max(s1, s2) elsewhere; same = s1; valid
+ * = s1 - s2 + 1 on a convolved axis (operands swapped when in2 is the larger
+ * in every dimension; INVALID_SHAPE when neither is), full elsewhere; the
+ * block kept is centred, start (full - kept) / 2.  out is float32 when both
+ * inputs are real, complex64 otherwise.  Computation: convolved axes padded
+ * to the power of two >= s1 + s2 - 1, complex FP32 transforms along each,
+ * pointwise product with broadcasting, inverse transforms, crop.  Each
+ * padded operand may hold at most 2^31 - 1 elements and each padded extent
+ * at most 2^26.  zkx_fftconvolve_nd_shape writes the result's extents.
+ */
+int zkx_fftconvolve_nd_shape(const long long* shape1, const long long* shape2, int rank, unsigned axes_mask,
+                             int mode, long long* out_shape);
+int zkx_fftconvolve_nd(const void* in1, const long long* shape1, int complex1, const void* in2,
+                       const long long* shape2, int complex2, int rank, unsigned axes_mask, int mode, void* out,
+                       void* stream);
 
 #ifdef __cplusplus
 }

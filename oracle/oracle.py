@@ -787,3 +787,150 @@ def rel_l2(a, ref, floor=1e-12):
     r = np.asarray(ref).astype(np.complex128).reshape(-1)
     den = max(float(np.linalg.norm(r)), floor)
     return float(np.linalg.norm(a - r)) / den
+
+
+# ---------------------------------------------------------------------------
+# N-D convolution and correlation (FP64 ground truth)
+# ---------------------------------------------------------------------------
+def fft_axes(axes, ndim):
+    """cuSignal's axes handling for the N-D transforms
+    (cusignal/utils/fftpack_helper.py:279-360, convolve.py:257-266): None is
+    every axis, negatives count from the end, out-of-range / repeated / empty
+    given axes raise ValueError; returned sorted."""
+    if axes is None:
+        return list(range(ndim))
+    ax = np.atleast_1d(np.asarray(axes))
+    ax = [int(a) + ndim if int(a) < 0 else int(a) for a in ax]
+    if any(a < 0 or a >= ndim for a in ax):
+        raise ValueError("axes exceeds dimensionality of input")
+    if len(set(ax)) != len(ax):
+        raise ValueError("all axes must be unique")
+    if not ax:
+        raise ValueError("when provided, axes cannot be empty")
+    return sorted(ax)
+
+
+def _direct_conv_full(a, b, axes):
+    """Full linear convolution over `axes` by direct FP64 sums (tiny inputs):
+    every pair of elements whose non-axis coordinates broadcast together adds
+    a[i] * b[j] at i + j on the convolved axes."""
+    nd = a.ndim
+    shape = [a.shape[d] + b.shape[d] - 1 if d in axes else max(a.shape[d], b.shape[d]) for d in range(nd)]
+    out = np.zeros(shape, np.complex128)
+    for i in np.ndindex(*a.shape):
+        for j in np.ndindex(*b.shape):
+            k, ok = [], True
+            for d in range(nd):
+                if d in axes:
+                    k.append(i[d] + j[d])
+                else:
+                    if a.shape[d] != 1 and b.shape[d] != 1 and i[d] != j[d]:
+                        ok = False
+                        break
+                    k.append(max(i[d] if a.shape[d] > 1 else 0, j[d] if b.shape[d] > 1 else 0))
+            if ok:
+                out[tuple(k)] += complex(a[i]) * complex(b[j])
+    return out
+
+
+def fftconvolve_nd(in1, in2, mode="full", axes=None, direct=False):
+    """cuSignal fftconvolve (convolve.py:159-320, convolution_utils.py:40-100):
+    rank and broadcast checks, 'valid' operand swap decided over every
+    dimension, full extent s1 + s2 - 1 on the convolved axes and max(s1, s2)
+    elsewhere, centred crop (_centered) to the mode's extents, a real result
+    for two real inputs.  The full convolution is computed in complex128 by
+    numpy FFTs of the full length (direct=True: by direct sums)."""
+    a = np.asarray(in1)
+    b = np.asarray(in2)
+    if a.ndim == b.ndim == 0:
+        return a * b
+    if a.ndim != b.ndim:
+        raise ValueError("in1 and in2 should have the same dimensionality")
+    if a.size == 0 or b.size == 0:
+        return np.array([])
+    ax = fft_axes(axes, a.ndim)
+    s1, s2 = np.array(a.shape), np.array(b.shape)
+    other = [d for d in range(a.ndim) if d not in ax]
+    for d in other:
+        if not (s1[d] == s2[d] or s1[d] == 1 or s2[d] == 1):
+            raise ValueError("incompatible shapes for in1 and in2")
+    complex_result = np.iscomplexobj(a) or np.iscomplexobj(b)
+    shape = np.maximum(s1, s2)
+    shape[ax] = s1[ax] + s2[ax] - 1
+    if mode == "valid":
+        ok1 = bool(np.all(s1 >= s2))
+        ok2 = bool(np.all(s2 >= s1))
+        if not (ok1 or ok2):
+            raise ValueError("For 'valid' mode, one must be at least as large as the other in every dimension")
+        if not ok1:
+            a, b, s1, s2 = b, a, s2, s1
+    elif mode not in ("full", "same"):
+        raise ValueError("acceptable mode flags are 'valid', 'same', or 'full'")
+    a64 = a.astype(np.complex128)
+    b64 = b.astype(np.complex128)
+    if direct:
+        ret = _direct_conv_full(a64, b64, ax)
+    else:
+        fshape = [int(shape[d]) for d in ax]
+        ret = np.fft.ifftn(np.fft.fftn(a64, fshape, axes=ax) * np.fft.fftn(b64, fshape, axes=ax), axes=ax)
+        ret = ret[tuple(slice(int(n)) for n in shape)]
+    if not complex_result:
+        ret = ret.real
+    if mode == "full":
+        keep = shape
+    elif mode == "same":
+        keep = s1
+    else:
+        keep = shape.copy()
+        keep[ax] = s1[ax] - s2[ax] + 1
+    start = (np.array(ret.shape) - keep) // 2
+    return ret[tuple(slice(int(s), int(s + k)) for s, k in zip(start, keep))]
+
+
+def correlate_nd(in1, in2, mode="full", dtype="FP32"):
+    """correlate_typed_cpu for rank 0 / >= 2 (convolution_typed.cpp:153-199,
+    349-410) with the FFT method's storage: extents full s1 + s2 - 1, same
+    s1, valid |s1 - s2| + 1 per dimension ('valid' needs one input to
+    dominate in every dimension); the full correlation
+    z[f] = sum_k in1[f - (s2 - 1) + k] * in2[k] in FP64, the centred crop
+    start (full - out) // 2, stored in `dtype` (integers rounded half to
+    even and wrapped modulo 2^bits, half rounded to nearest)."""
+    a = np.asarray(in1).astype(np.float64)
+    b = np.asarray(in2).astype(np.float64)
+    if a.ndim != b.ndim:
+        raise ValueError("typed correlate inputs must have the same rank")
+    if a.ndim == 0:
+        full = np.asarray(np.float32(np.float32(a) * np.float32(b)), np.float64)
+        return _store_typed(full, dtype)
+    if a.size == 0 or b.size == 0:
+        return np.zeros(0, NP_DTYPE[dtype])
+    s1, s2 = np.array(a.shape), np.array(b.shape)
+    if mode == "full":
+        out = s1 + s2 - 1
+    elif mode == "same":
+        out = s1.copy()
+    elif mode == "valid":
+        if not (np.all(s1 >= s2) or np.all(s2 >= s1)):
+            raise ValueError("typed correlate valid mode requires one input to dominate in every dimension")
+        out = np.abs(s1 - s2) + 1
+    else:
+        raise ValueError("typed correlate mode must be full, same, or valid")
+    full_shape = s1 + s2 - 1
+    # correlation = convolution with in2 reversed in every dimension
+    rb = b[tuple(slice(None, None, -1) for _ in range(b.ndim))]
+    dims = list(range(a.ndim))
+    full = np.real(np.fft.ifftn(np.fft.fftn(a, full_shape, axes=dims) * np.fft.fftn(rb, full_shape, axes=dims),
+                                axes=dims))
+    start = (full_shape - out) // 2
+    return _store_typed(full[tuple(slice(int(s), int(s + k)) for s, k in zip(start, out))], dtype)
+
+
+def _store_typed(v, dtype):
+    v = np.asarray(v, np.float64)
+    if INTEGRAL[dtype]:
+        bits = _INT_BITS[dtype]
+        w = np.mod(np.rint(v.astype(np.float32).astype(np.float64)), 2.0 ** bits).astype(np.uint64)
+        return w.astype(np.dtype(f"uint{bits}")).view(NP_DTYPE[dtype])
+    if dtype == "FP16":
+        return v.astype(np.float32).astype(np.float16)
+    return v.astype(np.float32)

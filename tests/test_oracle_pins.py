@@ -406,3 +406,110 @@ def test_correlate2d_integer_wrap_and_fill():
     assert np.array_equal(got.astype(np.int64), exp)
     with pytest.raises(ValueError):
         orc.correlate2d(a, b, "same", "fill", float("nan"), "INT8")
+
+
+# ---------------------------------------------------------------------------
+# N-D fftconvolve and correlate
+# ---------------------------------------------------------------------------
+_FFTCONV_CASES = [
+    # (shape1, shape2, axes)
+    ((37,), (9,), None),
+    ((5, 7), (3, 4), None),
+    ((6, 20), (1, 5), [1]),          # kernel broadcast over rows
+    ((6, 20), (6, 5), [-1]),         # equal non-axis extents
+    ((1, 20), (4, 5), [1]),          # in1 broadcast
+    ((4, 6, 9), (2, 6, 2), [0, 2]),
+    ((4, 1, 9), (2, 6, 2), [0, 2]),
+    ((3, 4, 5), (3, 4, 5), None),
+    ((8, 3), (5, 3), [0]),
+]
+
+
+@pytest.mark.parametrize("s1,s2,axes", _FFTCONV_CASES)
+@pytest.mark.parametrize("mode", ["full", "same", "valid"])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_fftconvolve_nd_vs_scipy(s1, s2, axes, mode, cplx):
+    """scipy.signal.fftconvolve has cuSignal's semantics wherever cuSignal's
+    'valid' swap rule (every dimension, not only the convolved ones) does not
+    refuse the shapes; the oracle must match it there."""
+    rng = np.random.default_rng(sum(s1) * 7 + sum(s2))
+    a = rng.standard_normal(s1)
+    b = rng.standard_normal(s2)
+    if cplx:
+        a = a + 1j * rng.standard_normal(s1)
+    d1, d2 = np.array(s1), np.array(s2)
+    if mode == "valid" and not (np.all(d1 >= d2) or np.all(d2 >= d1)):
+        with pytest.raises(ValueError):
+            orc.fftconvolve_nd(a, b, mode, axes)
+        return
+    ref = scipy_signal.fftconvolve(a, b, mode, axes=axes)
+    got = orc.fftconvolve_nd(a, b, mode, axes)
+    assert got.shape == ref.shape
+    assert np.iscomplexobj(got) == cplx
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("s1,s2,axes", [((5,), (3,), None), ((3, 4), (2, 3), None), ((3, 5), (1, 2), [1]),
+                                        ((2, 3, 4), (2, 1, 3), [0, 2]), ((4, 2), (2, 2), [0])])
+@pytest.mark.parametrize("mode", ["full", "same", "valid"])
+def test_fftconvolve_nd_brute_force(s1, s2, axes, mode):
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(s1) + 1j * rng.standard_normal(s1)
+    b = rng.standard_normal(s2)
+    assert np.allclose(orc.fftconvolve_nd(a, b, mode, axes), orc.fftconvolve_nd(a, b, mode, axes, direct=True),
+                       rtol=1e-10, atol=1e-10)
+
+
+def test_fftconvolve_nd_cusignal_specific_rules():
+    """Where cuSignal differs from current scipy: 'valid' decides the swap
+    over every dimension (convolution_utils.py:40-70), so broadcast
+    dimensions where in1 is smaller refuse the call; scalars multiply; an
+    empty input gives an empty result; axes are validated."""
+    a, b = np.ones((1, 10)), np.ones((3, 4))
+    scipy_signal.fftconvolve(a, b, "valid", axes=[1])  # accepted by scipy
+    with pytest.raises(ValueError):
+        orc.fftconvolve_nd(a, b, "valid", [1])
+    assert orc.fftconvolve_nd(np.float64(3.0), np.float64(-2.0)) == -6.0
+    assert orc.fftconvolve_nd(np.ones((0, 3)), np.ones((2, 2))).size == 0
+    for bad in ([2], [0, 0], []):
+        with pytest.raises(ValueError):
+            orc.fftconvolve_nd(np.ones((3, 3)), np.ones((2, 2)), "full", bad)
+    with pytest.raises(ValueError):
+        orc.fftconvolve_nd(np.ones((3, 4)), np.ones((2, 3)), "full", [0])  # 4 vs 3 on a broadcast axis
+
+
+@pytest.mark.parametrize("s1,s2", [((6, 9), (3, 4)), ((3, 4), (6, 9)), ((4, 5, 3), (2, 3, 2)), ((7, 2), (2, 2))])
+@pytest.mark.parametrize("mode", ["full", "same", "valid"])
+@pytest.mark.parametrize("dtype", ["FP32", "FP16", "INT32", "INT16", "INT8"])
+def test_correlate_nd_vs_scipy(s1, s2, mode, dtype):
+    """scipy.signal.correlate (direct sums) in FP64, cast as the FFT method
+    stores; integer inputs are small so the sums are exact."""
+    rng = np.random.default_rng(len(s1) * 13 + s2[0])
+    if orc.INTEGRAL[dtype]:
+        a = rng.integers(-9, 10, s1).astype(orc.NP_DTYPE[dtype])
+        b = rng.integers(-9, 10, s2).astype(orc.NP_DTYPE[dtype])
+    else:
+        a = (rng.standard_normal(s1) / 4).astype(orc.NP_DTYPE[dtype])
+        b = (rng.standard_normal(s2) / 4).astype(orc.NP_DTYPE[dtype])
+    ref = scipy_signal.correlate(a.astype(np.float64), b.astype(np.float64), mode, method="direct")
+    got = orc.correlate_nd(a, b, mode, dtype)
+    assert got.shape == ref.shape and got.dtype == orc.NP_DTYPE[dtype]
+    if orc.INTEGRAL[dtype]:
+        bits = {"INT32": 32, "INT16": 16, "INT8": 8}[dtype]
+        wrapped = (np.rint(ref).astype(np.int64) % (1 << bits)).astype(np.dtype(f"uint{bits}")).view(got.dtype)
+        assert np.array_equal(got, wrapped)
+    else:
+        assert np.allclose(got.astype(np.float64), ref.astype(orc.NP_DTYPE[dtype]).astype(np.float64),
+                           rtol=2e-3 if dtype == "FP16" else 1e-6, atol=2e-3 if dtype == "FP16" else 1e-6)
+
+
+def test_correlate_nd_shape_rules():
+    """convolution_typed.cpp:153-199: valid needs one input to dominate;
+    scalars multiply (in FP32); an empty input gives an empty output."""
+    with pytest.raises(ValueError):
+        orc.correlate_nd(np.ones((3, 5)), np.ones((4, 2)), "valid")
+    assert orc.correlate_nd(np.float32(1.5), np.float32(-2.0)) == np.float32(-3.0)
+    assert orc.correlate_nd(np.ones((0, 3), np.float32), np.ones((2, 2), np.float32)).size == 0
+    # a 2 x 2 known answer: full correlation of [[1,2],[3,4]] with [[1,0],[0,1]]
+    got = orc.correlate_nd(np.array([[1, 2], [3, 4]], np.float32), np.array([[1, 0], [0, 1]], np.float32))
+    assert np.allclose(got, np.array([[1, 2, 0], [3, 5, 2], [0, 3, 4]], np.float32), rtol=0, atol=1e-6)

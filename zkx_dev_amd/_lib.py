@@ -58,6 +58,10 @@ SIGNATURES = {
     "zkx_decimate_f32": (I, [P, I, P, I, I, I, I, I, P, P]),
     "zkx_channelize_poly_f32": (I, [P, LL, P, LL, I, P, P]),
     "zkx_correlate_out_len": (LL, [LL, LL, I]),
+    "zkx_correlate_nd_out_shape": (I, [P, P, I, I, P, P]),
+    "zkx_correlate_nd": (I, [P, P, P, P, I, I, I, P, P]),
+    "zkx_fftconvolve_nd_shape": (I, [P, P, I, ctypes.c_uint, I, P]),
+    "zkx_fftconvolve_nd": (I, [P, P, I, P, P, I, I, ctypes.c_uint, I, P, P]),
     "zkx_correlate_direct": (I, [P, LL, P, LL, I, I, P, P]),
     "zkx_correlate_fft": (I, [P, LL, P, LL, I, I, P, P]),
     "zkx_correlate2d_out_shape": (I, [LL, LL, LL, LL, I, P, P]),

@@ -13,6 +13,7 @@
 // A thread runs the chains of 8 consecutive outputs over one sliding window of
 // in1 staged in LDS; each chain keeps the reference's order, so results are
 // bit-identical to it.
+#include "conv_nd.hpp"
 #include "fft_any.hpp"
 #include "runtime.hpp"
 
@@ -503,6 +504,140 @@ int correlate_fft(const void* in1, long long l1, const void* in2, long long l2, 
     return ZKX_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// N-D correlate by FFT (correlate_device with a CorrelateNDWorkspace,
+// ZKX/cusignal_cpp/src/operators/convolution/convolution_typed.cu:150-241;
+// shape rules convolution_typed.cpp:153-199): every dimension padded to the
+// power of two >= s1 + s2 - 1, in2 read backwards in every dimension so the
+// full linear convolution is the full correlation, complex FP32 transforms,
+// then the centred crop start (full - out)/2 stored with the 1-D FFT
+// method's rounding (integers half to even, wrapped).
+// ---------------------------------------------------------------------------
+template <class T>
+struct NdLoadTyped {
+    const T* p;
+    __device__ float2 operator()(long long i) const { return make_float2(load_f(p[i]), 0.0f); }
+};
+
+template <class T>
+struct NdStoreTyped {
+    T* p;
+    __device__ void operator()(long long i, float2 v) const { p[i] = store_rounded<T>(v.x); }
+};
+
+template <class T>
+__global__ void scalar_product_kernel(const T* a, const T* b, T* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = store_rounded<T>(load_f(a[0]) * load_f(b[0]));
+}
+
+thread_local zkx::ThreadWorkspace t_corr_nd_ws;
+
+template <class T>
+int correlate_nd_fft(const void* in1, const int* s1, const void* in2, const int* s2, int rank, const int* out_dims,
+                     void* out, hipStream_t st)
+{
+    using namespace zkx;
+    NdConvPlan p;
+    int rc = nd_conv_plan(s1, s2, rank, (1u << rank) - 1u, &p);
+    if (rc) return rc;
+    void* wsv = nullptr;
+    WorkspaceLease lease_(t_corr_nd_ws, st);
+    rc = lease_.get(p.bytes, &wsv);
+    if (rc) return rc;
+    char* ws = static_cast<char*>(wsv);
+    for (int which = 0; which < 2; ++which) {
+        NdPad pp{};
+        pp.rank = rank;
+        const int* s = which ? s2 : s1;
+        const int* g = which ? p.g2 : p.g1;
+        long long rows = 1;
+        for (int d = 0; d < rank; ++d) {
+            pp.g[d] = g[d];
+            pp.s[d] = s[d];
+            if (d + 1 < rank) rows *= g[d];
+        }
+        nd_strides(s, rank, pp.sstride);
+        pp.reverse = which;
+        float2* X = reinterpret_cast<float2*>(ws + (which ? p.off_x2 : p.off_x1));
+        hipLaunchKernelGGL(nd_pad_kernel<NdLoadTyped<T>>, nd_grid(rows, g[rank - 1]), dim3(256), 0, st,
+                           NdLoadTyped<T>{static_cast<const T*>(which ? in2 : in1)}, pp, X);
+        ZKX_LAUNCH_CHECK();
+    }
+    if ((rc = nd_conv_run(p, ws, st))) return rc;
+    NdCrop c{};
+    c.rank = rank;
+    nd_strides(p.g, rank, c.ystride);
+    long long rows = 1;
+    for (int d = 0; d < rank; ++d) {
+        c.o[d] = out_dims[d];
+        c.start[d] = (s1[d] + s2[d] - 1 - out_dims[d]) / 2;
+        if (d + 1 < rank) rows *= out_dims[d];
+    }
+    hipLaunchKernelGGL(nd_crop_kernel<NdStoreTyped<T>>, nd_grid(rows, out_dims[rank - 1]), dim3(256), 0, st,
+                       reinterpret_cast<const float2*>(ws + p.off_y), c, NdStoreTyped<T>{static_cast<T*>(out)});
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+// Output extents of the N-D correlate (convolution_typed.cpp:153-199):
+// full s1 + s2 - 1, same s1, valid |s1 - s2| + 1 per dimension; 'valid'
+// needs one input at least as large as the other in every dimension.
+// Returns the element count (0 when an input is empty), or minus the error
+// code of a failure recorded in the last error.
+long long correlate_nd_shape(const int* shape1, const int* shape2, int rank, int mode, int* out_dims)
+{
+    if (mode < 0 || mode > 2) {
+        zkx::set_last_error(ZKX_INVALID_ARGUMENT, "typed correlate mode must be full, same, or valid");
+        return -ZKX_INVALID_ARGUMENT;
+    }
+    if (rank < 0 || rank > zkx::kMaxNdRank || (rank > 0 && (!shape1 || !shape2))) {
+        zkx::set_last_error(ZKX_INVALID_SHAPE, "typed correlate rank must be 0..8 with both shapes given");
+        return -ZKX_INVALID_SHAPE;
+    }
+    long long n1 = 1, n2 = 1;
+    for (int d = 0; d < rank; ++d) {
+        if (shape1[d] < 0 || shape2[d] < 0) {
+            zkx::set_last_error(ZKX_INVALID_SHAPE, "typed correlate negative dimension");
+            return -ZKX_INVALID_SHAPE;
+        }
+        n1 *= shape1[d];
+        n2 *= shape2[d];
+        if (n1 > std::numeric_limits<int>::max() || n2 > std::numeric_limits<int>::max()) {
+            zkx::set_last_error(ZKX_INVALID_SHAPE, "typed correlate input exceeds 32-bit indexing");
+            return -ZKX_INVALID_SHAPE;
+        }
+    }
+    if (n1 == 0 || n2 == 0) {
+        if (out_dims && rank > 0) out_dims[0] = 0;
+        return 0;
+    }
+    bool dom1 = true, dom2 = true;
+    long long n = 1;
+    for (int d = 0; d < rank; ++d) {
+        const int a = shape1[d], b = shape2[d];
+        dom1 = dom1 && a >= b;
+        dom2 = dom2 && b >= a;
+        const long long e = mode == 0 ? static_cast<long long>(a) + b - 1 : mode == 1 ? a : std::abs(a - b) + 1;
+        if (e > std::numeric_limits<int>::max()) {
+            zkx::set_last_error(ZKX_INVALID_SHAPE, "typed correlate full shape exceeds 32-bit indexing");
+            return -ZKX_INVALID_SHAPE;
+        }
+        if (out_dims) out_dims[d] = static_cast<int>(e);
+        n *= e;
+    }
+    if (mode == 2 && !dom1 && !dom2) {
+        zkx::set_last_error(ZKX_INVALID_SHAPE,
+                            "typed correlate valid mode requires one input to dominate in every dimension");
+        return -ZKX_INVALID_SHAPE;
+    }
+    if (n > std::numeric_limits<int>::max()) {
+        zkx::set_last_error(ZKX_INVALID_SHAPE, "typed correlate output exceeds 32-bit indexing");
+        return -ZKX_INVALID_SHAPE;
+    }
+    return n;
+}
+
 size_t elem_size(int dtype)
 {
     switch (dtype) {
@@ -584,6 +719,69 @@ int zkx_correlate_direct(const void* in1, long long l1, const void* in2, long lo
     case ZKX_DTYPE_I32: return launch<int32_t>(in1, l1, in2, l2, start, n, out, st);
     case ZKX_DTYPE_I16: return launch<int16_t>(in1, l1, in2, l2, start, n, out, st);
     default: return launch<int8_t>(in1, l1, in2, l2, start, n, out, st);
+    }
+}
+
+int zkx_correlate_nd_out_shape(const int* shape1, const int* shape2, int rank, int mode, int* out_shape,
+                               long long* elements)
+{
+    zkx::clear_last_error();
+    if (!elements || (rank > 0 && !out_shape)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate: null output shape");
+    const long long n = correlate_nd_shape(shape1, shape2, rank, mode, out_shape);
+    if (n < 0) return static_cast<int>(-n);
+    *elements = rank == 0 ? 1 : n;
+    return ZKX_SUCCESS;
+}
+
+int zkx_correlate_nd(const void* in1, const int* shape1, const void* in2, const int* shape2, int rank, int dtype,
+                     int mode, void* out, void* stream)
+{
+    zkx::clear_last_error();
+    const size_t es = elem_size(dtype);
+    if (!es) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate element type");
+    int dims[zkx::kMaxNdRank] = {};
+    const long long n = correlate_nd_shape(shape1, shape2, rank, mode, dims);
+    if (n < 0) return static_cast<int>(-n);
+    if (rank > 0 && n == 0) return ZKX_SUCCESS;
+    if (!in1 || !in2 || !out) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "correlate: null pointer");
+    long long n1 = 1, n2 = 1;
+    for (int d = 0; d < rank; ++d) {
+        n1 *= shape1[d];
+        n2 *= shape2[d];
+    }
+    const long long nout = rank == 0 ? 1 : n;
+    if (overlap(out, es * nout, in1, es * n1) || overlap(out, es * nout, in2, es * n2))
+        ZKX_FAIL(ZKX_INVALID_ARGUMENT, "typed correlate output must not alias an input");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (rank == 0) {
+        // scalars: the product stored in the element type
+        switch (dtype) {
+        case ZKX_DTYPE_F32: hipLaunchKernelGGL(scalar_product_kernel<float>, dim3(1), dim3(64), 0, st,
+                                               static_cast<const float*>(in1), static_cast<const float*>(in2),
+                                               static_cast<float*>(out)); break;
+        case ZKX_DTYPE_F16: hipLaunchKernelGGL(scalar_product_kernel<__half>, dim3(1), dim3(64), 0, st,
+                                               static_cast<const __half*>(in1), static_cast<const __half*>(in2),
+                                               static_cast<__half*>(out)); break;
+        case ZKX_DTYPE_I32: hipLaunchKernelGGL(scalar_product_kernel<int32_t>, dim3(1), dim3(64), 0, st,
+                                               static_cast<const int32_t*>(in1), static_cast<const int32_t*>(in2),
+                                               static_cast<int32_t*>(out)); break;
+        case ZKX_DTYPE_I16: hipLaunchKernelGGL(scalar_product_kernel<int16_t>, dim3(1), dim3(64), 0, st,
+                                               static_cast<const int16_t*>(in1), static_cast<const int16_t*>(in2),
+                                               static_cast<int16_t*>(out)); break;
+        default: hipLaunchKernelGGL(scalar_product_kernel<int8_t>, dim3(1), dim3(64), 0, st,
+                                    static_cast<const int8_t*>(in1), static_cast<const int8_t*>(in2),
+                                    static_cast<int8_t*>(out));
+        }
+        ZKX_LAUNCH_CHECK();
+        return ZKX_SUCCESS;
+    }
+    if (rank == 1) return zkx_correlate_fft(in1, shape1[0], in2, shape2[0], dtype, mode, out, stream);
+    switch (dtype) {
+    case ZKX_DTYPE_F32: return correlate_nd_fft<float>(in1, shape1, in2, shape2, rank, dims, out, st);
+    case ZKX_DTYPE_F16: return correlate_nd_fft<__half>(in1, shape1, in2, shape2, rank, dims, out, st);
+    case ZKX_DTYPE_I32: return correlate_nd_fft<int32_t>(in1, shape1, in2, shape2, rank, dims, out, st);
+    case ZKX_DTYPE_I16: return correlate_nd_fft<int16_t>(in1, shape1, in2, shape2, rank, dims, out, st);
+    default: return correlate_nd_fft<int8_t>(in1, shape1, in2, shape2, rank, dims, out, st);
     }
 }
 

@@ -266,25 +266,90 @@ __all__ = ["upfirdn", "resample_poly", "firfilter", "decimate", "channelize_poly
            "design_resample_poly_coefficients", "design_decimate_filter"]
 
 
-def fftconvolve(in1, in2, mode="full", *, stream=None):
-    """fftconvolve of complex64 rows in1 [..., n1] with one 1-D in2 [n2] along
-    the last axis (cuSignal convolve.py:159-320 semantics for modes
-    'full' / 'same' / 'valid').  Real inputs are promoted to complex64."""
+def _fft_axes(axes, ndim):
+    """cuSignal's axes normalisation (fftpack_helper.py:279-360): None means
+    every axis; negative axes count from the end; out of range, repeated or
+    (when given) empty axes are errors."""
+    if axes is None:
+        return list(range(ndim))
+    ax = np.atleast_1d(np.asarray(axes))
+    if ax.ndim != 1:
+        raise ArgumentError(INVALID_ARGUMENT, "when given, axes values must be a scalar or vector")
+    if ax.size and not np.issubdtype(ax.dtype, np.integer):
+        raise ArgumentError(INVALID_ARGUMENT, "when given, axes values must be integers")
+    ax = [int(a) + ndim if int(a) < 0 else int(a) for a in ax]
+    if any(a < 0 or a >= ndim for a in ax):
+        raise ArgumentError(INVALID_ARGUMENT, "axes exceeds dimensionality of input")
+    if len(set(ax)) != len(ax):
+        raise ArgumentError(INVALID_ARGUMENT, "all axes must be unique")
+    if not ax:
+        raise ArgumentError(INVALID_ARGUMENT, "when provided, axes cannot be empty")
+    return sorted(ax)
+
+
+def fftconvolve(in1, in2, mode="full", axes=None, *, stream=None):
+    """fftconvolve (cuSignal convolve.py:159-320): N-D linear convolution of
+    two device tensors of one rank over `axes` (default all), the other axes
+    broadcast (equal extents or 1); modes 'full' / 'same' / 'valid'.  float32
+    and complex64 inputs; the result is float32 when both are real, complex64
+    otherwise.
+
+    Extension kept from the row form: a 1-D in2 against an in1 of higher
+    rank (axes not given) convolves every last-axis row of in1 with in2 and
+    returns complex64."""
     modes = {"full": 0, "same": 1, "valid": 2}
     if mode not in modes:
-        raise ValueError("acceptable mode flags are 'valid', 'same', or 'full'")
-    if not isinstance(in1, torch.Tensor) or not in1.is_cuda:
-        raise ValueError("in1 must be a device tensor")
+        raise ArgumentError(INVALID_ARGUMENT, "acceptable mode flags are 'valid', 'same', or 'full'")
+    for t, name in ((in1, "in1"), (in2, "in2")):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise ArgumentError(INVALID_ARGUMENT, f"{name} must be a device tensor")
+        if t.dtype not in (torch.float32, torch.complex64):
+            raise ArgumentError(INVALID_DTYPE, f"{name} must be float32 or complex64")
+    if in2.dim() == 1 and in1.dim() > 1 and axes is None:
+        return _fftconvolve_rows(in1, in2, modes[mode], stream)
+    if in1.dim() != in2.dim():
+        raise ArgumentError(INVALID_SHAPE, "in1 and in2 should have the same dimensionality")
+    if in1.numel() == 0 or in2.numel() == 0:
+        return torch.empty(0, dtype=torch.float32, device=in1.device)
+    scalar = in1.dim() == 0
+    a = (in1.reshape(1) if scalar else in1).contiguous()
+    b = (in2.reshape(1) if scalar else in2).contiguous()
+    rank = a.dim()
+    if rank > 8:
+        raise ArgumentError(INVALID_SHAPE, "fftconvolve supports at most 8 dimensions")
+    ax = _fft_axes(axes, rank)
+    mask = sum(1 << d for d in ax)
+    complex_out = a.is_complex() or b.is_complex()
+    s1 = (ctypes.c_longlong * rank)(*a.shape)
+    s2 = (ctypes.c_longlong * rank)(*b.shape)
+    shape = (ctypes.c_longlong * rank)()
+    check(lib().zkx_fftconvolve_nd_shape(s1, s2, rank, mask, modes[mode], shape))
+    out_shape = tuple(int(v) for v in shape)
+    y = torch.empty(out_shape, dtype=torch.complex64 if complex_out else torch.float32, device=a.device)
+    n2_last = b.shape[-1]
+    if (complex_out and a.is_complex() and b.is_complex() and ax == [rank - 1] and mode != "valid"
+            and all(v == 1 for v in b.shape[:-1]) and n2_last <= 4096):
+        # one kernel row against every row of in1: overlap-save rows
+        n1 = a.shape[-1]
+        rows = a.numel() // n1
+        check(lib().zkx_fftconvolve_c64(_ptr(a), rows, n1, _ptr(b), n2_last, modes[mode], _ptr(y),
+                                        _stream(stream)))
+    else:
+        check(lib().zkx_fftconvolve_nd(_ptr(a), s1, int(a.is_complex()), _ptr(b), s2, int(b.is_complex()), rank,
+                                       mask, modes[mode], _ptr(y), _stream(stream)))
+    return y.reshape(()) if scalar else y
+
+
+def _fftconvolve_rows(in1, in2, mode, stream):
     a = in1.to(torch.complex64).contiguous()
-    b = torch.as_tensor(in2, device=a.device).to(torch.complex64).contiguous().reshape(-1)
+    b = in2.to(torch.complex64).contiguous()
     if a.numel() == 0 or b.numel() == 0:
         return torch.empty(0, dtype=torch.complex64, device=a.device)
     n1 = a.shape[-1]
     rows = a.numel() // n1
-    out_len = lib().zkx_fftconvolve_out_len(n1, b.numel(), modes[mode])
+    out_len = lib().zkx_fftconvolve_out_len(n1, b.numel(), mode)
     y = torch.empty(tuple(a.shape[:-1]) + (out_len,), dtype=torch.complex64, device=a.device)
-    check(lib().zkx_fftconvolve_c64(_ptr(a), rows, n1, _ptr(b), b.numel(), modes[mode], _ptr(y),
-                                    _stream(stream)))
+    check(lib().zkx_fftconvolve_c64(_ptr(a), rows, n1, _ptr(b), b.numel(), mode, _ptr(y), _stream(stream)))
     return y
 
 
