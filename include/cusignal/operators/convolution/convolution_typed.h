@@ -5,7 +5,9 @@
 // chain for floating types, modular integer sums) and exceptions; and the 2-D
 // direct form correlate2d_output_shape / Correlate2DWorkspace /
 // correlate2d_device with fill, wrap and symm boundaries; the FFT method
-// through the 1-D workspace overload.  The N-D form is not provided.
+// through the 1-D workspace overload; the N-D form (CorrelateNDOptions,
+// correlate_nd_output_shape, CorrelateNDWorkspace and its correlate_device
+// overload) by FFT over every dimension.
 #pragma once
 
 #include <cstddef>
@@ -14,6 +16,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../filtering/filtering_typed.h"
 
@@ -101,6 +104,115 @@ void correlate_device(const DeviceArray<T>& in1, const DeviceArray<T>& in2, Devi
                                      detail::correlate_mode_code(workspace.mode), out.data(), nullptr);
     if (rc == ZKX_INVALID_ARGUMENT || rc == ZKX_INVALID_SHAPE) throw std::invalid_argument(zkx_last_error_message());
     detail::check_abi(rc);
+}
+
+// convolution_typed.h:70-93 (CorrelateNDOptions / CorrelateNDOutputShape /
+// correlate_nd_output_shape): any rank, shapes row-major, an empty shape is a
+// scalar.  direct is only the 1-D method (rank > 1 refused), auto picks the
+// direct method for rank <= 1 and FFT above.
+struct CorrelateNDOptions {
+    std::vector<int> shape1;
+    std::vector<int> shape2;
+    std::string mode = "full";
+    CorrelateMethod method = CorrelateMethod::auto_select;
+};
+
+struct CorrelateNDOutputShape {
+    std::vector<int> dimensions;
+    std::size_t elements = 0;
+
+    [[nodiscard]] std::size_t size() const noexcept { return elements; }
+};
+
+// convolution_typed.cpp:153-199: scalars give {} with 1 element, an empty
+// input {0} with 0, otherwise full / same / valid extents per dimension.
+inline CorrelateNDOutputShape correlate_nd_output_shape(const CorrelateNDOptions& options)
+{
+    const int code = detail::correlate_mode_code(options.mode);
+    if (options.method != CorrelateMethod::direct && options.method != CorrelateMethod::fft &&
+        options.method != CorrelateMethod::auto_select)
+        throw std::invalid_argument("typed correlate method must be direct, fft, or auto");
+    if (options.shape1.size() != options.shape2.size())
+        throw std::invalid_argument("typed correlate inputs must have the same rank");
+    if (options.method == CorrelateMethod::direct && options.shape1.size() > 1)
+        throw std::invalid_argument("typed correlate direct method is only implemented for 1D");
+    const int rank = static_cast<int>(options.shape1.size());
+    CorrelateNDOutputShape out;
+    std::vector<int> dims(rank > 0 ? rank : 1, 0);
+    long long elements = 0;
+    detail::check_abi(zkx_correlate_nd_out_shape(options.shape1.data(), options.shape2.data(), rank, code,
+                                                 dims.data(), &elements));
+    out.elements = static_cast<std::size_t>(elements);
+    if (rank == 0) return out;
+    if (elements == 0) {
+        out.dimensions = {0};
+        return out;
+    }
+    out.dimensions.assign(dims.begin(), dims.begin() + rank);
+    return out;
+}
+
+// convolution_typed.h:149-190 (CorrelateNDWorkspace): the validated shapes,
+// mode and method of one N-D correlation.  The library holds the padded
+// complex FP32 operands and plans (per host thread, ordered on the default
+// stream), so the workspace carries no device arrays.
+struct CorrelateNDWorkspace {
+    std::vector<int> shape1;
+    std::vector<int> shape2;
+    std::vector<int> output_shape;
+    CorrelateMethod requested_method = CorrelateMethod::auto_select;
+    CorrelateMethod effective_method = CorrelateMethod::fft;
+    std::string mode = "full";
+    int rank = 0;
+    std::size_t input1_elements = 0;
+    std::size_t input2_elements = 0;
+    std::size_t output_elements = 0;
+
+    CorrelateNDWorkspace() = default;
+    explicit CorrelateNDWorkspace(const CorrelateNDOptions& options) { reset(options); }
+    void reset(const CorrelateNDOptions& options)
+    {
+        const CorrelateNDOutputShape out = correlate_nd_output_shape(options);
+        auto count = [](const std::vector<int>& s) {
+            std::size_t n = 1;
+            for (int v : s) n *= static_cast<std::size_t>(v);
+            return n;
+        };
+        shape1 = options.shape1;
+        shape2 = options.shape2;
+        output_shape = out.dimensions;
+        requested_method = options.method;
+        mode = options.mode;
+        rank = static_cast<int>(shape1.size());
+        input1_elements = count(shape1);
+        input2_elements = count(shape2);
+        output_elements = out.size();
+        effective_method = requested_method == CorrelateMethod::auto_select
+                               ? (rank <= 1 ? CorrelateMethod::direct : CorrelateMethod::fft)
+                               : requested_method;
+    }
+    [[nodiscard]] std::size_t output_size() const noexcept { return output_elements; }
+};
+
+// convolution_typed.h:331-337: N-D correlate with the caller's workspace on
+// the default stream (rank 1 with the direct method runs the direct sums).
+template <typename T>
+void correlate_device(const DeviceArray<T>& in1, const DeviceArray<T>& in2, DeviceArray<T>& out,
+                      CorrelateNDWorkspace& workspace)
+{
+    if (in1.size() != workspace.input1_elements || in2.size() != workspace.input2_elements ||
+        out.size() != workspace.output_elements)
+        throw std::invalid_argument("typed correlate workspace shape mismatch");
+    if (workspace.output_elements == 0) return;
+    const int code = detail::correlate_mode_code(workspace.mode);
+    if (workspace.rank == 1 && workspace.effective_method == CorrelateMethod::direct) {
+        detail::check_abi(zkx_correlate_direct(in1.data(), static_cast<long long>(in1.size()), in2.data(),
+                                               static_cast<long long>(in2.size()), detail::dtype_code<T>(), code,
+                                               out.data(), nullptr));
+        return;
+    }
+    detail::check_abi(zkx_correlate_nd(in1.data(), workspace.shape1.data(), in2.data(), workspace.shape2.data(),
+                                       workspace.rank, detail::dtype_code<T>(), code, out.data(), nullptr));
 }
 
 // convolution_typed.h:204-218 (Correlate2DOptions / Correlate2DOutputShape)

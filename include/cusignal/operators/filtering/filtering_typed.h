@@ -731,4 +731,73 @@ void fftconvolve_device(const DeviceArray<TypedComplex<T>>& in1, const DeviceArr
                                           reinterpret_cast<zkx_complex64*>(out.out.data()), nullptr));
 }
 
+// fftconvolve over N-D axes (extension; cuSignal convolve.py:159-320): both
+// inputs of one rank (row-major shapes), convolved over `axes` (empty: all;
+// negative axes count from the end), the other dimensions broadcast (equal
+// extents or 1).  Real inputs give a real result, complex inputs a complex
+// one.
+struct FftconvolveNDOptions {
+    std::vector<int> shape1;
+    std::vector<int> shape2;
+    FftconvolveMode mode = FftconvolveMode::full;
+    std::vector<int> axes;
+};
+
+namespace detail {
+inline unsigned fftconvolve_axes_mask(const FftconvolveNDOptions& o)
+{
+    const int rank = static_cast<int>(o.shape1.size());
+    if (o.axes.empty()) return rank >= 32 ? ~0u : (1u << rank) - 1u;
+    unsigned m = 0;
+    for (int a : o.axes) {
+        const int d = a < 0 ? a + rank : a;
+        if (d < 0 || d >= rank) throw OperatorArgumentError(ErrorCode::INVALID_ARGUMENT, "axes exceeds dimensionality of input");
+        if (m >> d & 1u) throw OperatorArgumentError(ErrorCode::INVALID_ARGUMENT, "all axes must be unique");
+        m |= 1u << d;
+    }
+    return m;
+}
+}  // namespace detail
+
+// Extents of the result (zkx_fftconvolve_nd_shape).
+inline std::vector<int> fftconvolve_nd_output_shape(const FftconvolveNDOptions& o)
+{
+    if (o.shape1.size() != o.shape2.size())
+        throw OperatorArgumentError(ErrorCode::INVALID_SHAPE, "in1 and in2 should have the same dimensionality");
+    const int rank = static_cast<int>(o.shape1.size());
+    std::vector<long long> s1(o.shape1.begin(), o.shape1.end()), s2(o.shape2.begin(), o.shape2.end()), out(rank);
+    detail::check_abi(zkx_fftconvolve_nd_shape(s1.data(), s2.data(), rank, detail::fftconvolve_axes_mask(o),
+                                               static_cast<int>(o.mode), out.data()));
+    return std::vector<int>(out.begin(), out.end());
+}
+
+// A and B are float or TypedComplex<float>; O is float when both are real,
+// TypedComplex<float> otherwise; out holds fftconvolve_nd_output_shape(o)
+// elements.
+template <typename A, typename B, typename O>
+void fftconvolve_device(const DeviceArray<A>& in1, const DeviceArray<B>& in2, DeviceArray<O>& out,
+                        const FftconvolveNDOptions& o)
+{
+    constexpr bool ca = std::is_same_v<A, TypedComplex<float>>, cb = std::is_same_v<B, TypedComplex<float>>;
+    static_assert((ca || std::is_same_v<A, float>) && (cb || std::is_same_v<B, float>),
+                  "fftconvolve N-D inputs are float or TypedComplex<float>");
+    static_assert(std::is_same_v<O, std::conditional_t<ca || cb, TypedComplex<float>, float>>,
+                  "fftconvolve N-D output is float for two real inputs, TypedComplex<float> otherwise");
+    const std::vector<int> shape = fftconvolve_nd_output_shape(o);
+    auto count = [](const std::vector<int>& s) {
+        std::size_t n = 1;
+        for (int v : s) n *= static_cast<std::size_t>(v);
+        return n;
+    };
+    if (in1.size() != count(o.shape1) || in2.size() != count(o.shape2))
+        throw OperatorArgumentError(ErrorCode::INVALID_SHAPE, "fftconvolve input sizes must match their shapes");
+    if (out.size() != count(shape))
+        throw OperatorArgumentError(ErrorCode::INVALID_SHAPE, "fftconvolve output size must match the result shape");
+    const int rank = static_cast<int>(o.shape1.size());
+    std::vector<long long> s1(o.shape1.begin(), o.shape1.end()), s2(o.shape2.begin(), o.shape2.end());
+    detail::check_abi(zkx_fftconvolve_nd(in1.data(), s1.data(), ca ? 1 : 0, in2.data(), s2.data(), cb ? 1 : 0, rank,
+                                         detail::fftconvolve_axes_mask(o), static_cast<int>(o.mode), out.data(),
+                                         nullptr));
+}
+
 }  // namespace cusignal

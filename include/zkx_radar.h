@@ -600,9 +600,11 @@ int zkx_fftconvolve_c64(const zkx_complex64* in1, int batch, int n1, const zkx_c
  * block kept is centred, start (full - kept) / 2.  out is float32 when both
  * inputs are real, complex64 otherwise.  Computation: convolved axes padded
  * to the power of two >= s1 + s2 - 1, complex FP32 transforms along each,
- * pointwise product with broadcasting, inverse transforms, crop.  Each
- * padded operand may hold at most 2^31 - 1 elements and each padded extent
- * at most 2^26.  zkx_fftconvolve_nd_shape writes the result's extents.
+ * pointwise product with broadcasting, inverse transforms, crop; two
+ * complex inputs convolved along the last axis only, with in2 a single row
+ * of at most 4096 samples, run as zkx_fftconvolve_c64 over the rows of in1.
+ * Each padded operand may hold at most 2^31 - 1 elements and each padded
+ * extent at most 2^26.  zkx_fftconvolve_nd_shape writes the result's extents.
  */
 int zkx_fftconvolve_nd_shape(const long long* shape1, const long long* shape2, int rank, unsigned axes_mask,
                              int mode, long long* out_shape);

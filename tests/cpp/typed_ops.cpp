@@ -2,7 +2,8 @@
 // for one element type of the reference's instantiations (float, __half,
 // int32, int16, int8; radartools_typed.cpp:810-811, filtering_typed.cu:683+),
 // plus the complex filtering extensions, fftconvolve, decimate,
-// channelize_poly, correlate, ambgfun and correlate2d.  Built with g++ and
+// channelize_poly, correlate (1-D and N-D), ambgfun and correlate2d.  Built
+// with g++ and
 // linked to libzkx_radar.so.
 //
 // usage: typed_ops IN OUT
@@ -224,6 +225,21 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
             w.vec(3, c2.to_host());
         }
     }
+    // 26b-26d: the same pair through the N-D FFT form (CorrelateNDWorkspace),
+    // full / same / valid, in the input type
+    {
+        const auto d_k = DeviceArray<T>::from_host(std::vector<T>(h.begin(), h.begin() + 30));
+        for (const char* mode : {"full", "same", "valid"}) {
+            cusignal::CorrelateNDOptions o;
+            o.shape1 = {30, 100};
+            o.shape2 = {5, 6};
+            o.mode = mode;
+            cusignal::CorrelateNDWorkspace ws(o);
+            DeviceArray<T> cn(ws.output_size());
+            cusignal::correlate_device<T>(d_xf, d_k, cn, ws);
+            w.vec(3, cn.to_host());
+        }
+    }
     if constexpr (std::is_same_v<T, float>) {
         // 27-29: complex filtering extensions on the pulses (float taps)
         cusignal::UpfirdnComplexDeviceResult uc;
@@ -241,6 +257,27 @@ int run(std::ifstream& in, Writer& w, const int* hdr)
         fco.axis = 1;
         cusignal::firfilter_device<float>(d_h, d_x, fc, fco);
         w.vec(0, fc.y.to_host());
+        // 30-31: N-D fftconvolve: the filtering samples as 30 x 100 with the
+        // first 30 taps as 5 x 6 over both axes ('same', real result), and
+        // the pulses [P, R] with the template as [1, T] along axis 1 ('full',
+        // complex result)
+        cusignal::FftconvolveNDOptions n1;
+        n1.shape1 = {30, 100};
+        n1.shape2 = {5, 6};
+        n1.mode = cusignal::FftconvolveMode::same;
+        const auto d_k = DeviceArray<float>::from_host(std::vector<float>(h.begin(), h.begin() + 30));
+        DeviceArray<float> r1(30 * 100);
+        cusignal::fftconvolve_device(d_xf, d_k, r1, n1);
+        w.vec(0, r1.to_host());
+        cusignal::FftconvolveNDOptions n2;
+        n2.shape1 = {P, R};
+        n2.shape2 = {1, T_};
+        n2.axes = {-1};
+        std::size_t cnt = 1;
+        for (int v : cusignal::fftconvolve_nd_output_shape(n2)) cnt *= static_cast<std::size_t>(v);
+        DeviceArray<C> r2(cnt);
+        cusignal::fftconvolve_device(d_x, d_t, r2, n2);
+        w.vec(0, r2.to_host());
     }
     return 0;
 }

@@ -207,6 +207,22 @@ def test_typed_operators(name, tmp_path):
         val = next(b)
         ref = orc.correlate2d(inp["xf"].reshape(30, 100), inp["h"][:30].reshape(5, 6), mode, boundary, 2.0, name)
         assert np.array_equal(val.view(np.uint8), ref.reshape(-1).view(np.uint8)), (name, mode, boundary)
+    # N-D correlate (FFT) of the same pair, and the Python binding's result
+    for mode in ("full", "same", "valid"):
+        val = next(b)
+        k = inp["h"][:30].reshape(5, 6)
+        ref = orc.correlate_nd(inp["xf"].reshape(30, 100), k, mode, name)
+        py = convolution.correlate(torch.from_numpy(inp["xf"].reshape(30, 100)).cuda(),
+                                   torch.from_numpy(np.ascontiguousarray(k)).cuda(), mode).cpu().numpy()
+        assert np.array_equal(val.view(np.uint8), py.reshape(-1).view(np.uint8)), (name, mode)
+        if name == "INT32":
+            pass  # sums of 2^28-sized samples: FP32 transforms are not exact there; the C++ and Python paths agree
+        elif integral:
+            assert np.array_equal(val, ref.reshape(-1)), (name, mode)
+        elif name == "FP16":
+            assert np.allclose(val.astype(np.float64), ref.reshape(-1).astype(np.float64), rtol=5e-3, atol=5e-3)
+        else:
+            assert orc.rel_l2(val, ref.reshape(-1)) < 1e-5, (name, mode)
     if name == "FP32":
         # complex extensions: real taps act on each component
         uc = _c(next(b))
@@ -219,4 +235,18 @@ def test_typed_operators(name, tmp_path):
         re = filtering.firfilter(dev["h"], x.real.contiguous(), axis=1).cpu().numpy()
         im = filtering.firfilter(dev["h"], x.imag.contiguous(), axis=1).cpu().numpy()
         assert np.array_equal(fc.real, re) and np.array_equal(fc.imag, im)
+        # N-D fftconvolve through the C++ overload: bit-identical to the
+        # Python binding, within 1e-5 of the FP64 oracle
+        k = inp["h"][:30].reshape(5, 6)
+        r1 = next(b)
+        py = filtering.fftconvolve(dev["xf"].reshape(30, 100), torch.from_numpy(np.ascontiguousarray(k)).cuda(),
+                                   "same").cpu().numpy()
+        assert np.array_equal(r1.view(np.uint32), py.reshape(-1).view(np.uint32))
+        assert orc.rel_l2(r1, orc.fftconvolve_nd(inp["xf"].reshape(30, 100), k, "same").reshape(-1)) < 1e-5
+        r2 = _c(next(b))
+        tm2 = tm.reshape(1, -1)
+        py = filtering.fftconvolve(x, tm2, "full", [-1]).cpu().numpy()
+        assert np.array_equal(r2.view(np.uint64), py.reshape(-1).view(np.uint64))
+        ref = orc.fftconvolve_nd(x.cpu().numpy(), tm2.cpu().numpy(), "full", [-1])
+        assert orc.rel_l2(r2, ref.reshape(-1)) < 1e-5
     assert next(b, None) is None

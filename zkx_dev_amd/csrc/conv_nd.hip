@@ -300,6 +300,19 @@ int zkx_fftconvolve_nd(const void* in1, const long long* shape1, int complex1, c
     int rc = fftconv_shape(shape1, shape2, rank, axes_mask, mode, &q);
     if (rc) return rc;
     if (!in1 || !in2 || !out) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "fftconvolve: null pointer");
+    // one complex kernel row (leading extents 1, at most 4096 taps) against
+    // every row of a complex in1 along the last axis: overlap-save rows,
+    // without padding each row to a power of two
+    bool rows_form = complex1 && complex2 && axes_mask == (1u << (rank - 1)) && !q.swap && q.s2[rank - 1] <= 4096;
+    long long lead = 1;
+    for (int d = 0; d + 1 < rank; ++d) {
+        rows_form = rows_form && q.s2[d] == 1;
+        lead *= q.s1[d];
+    }
+    if (rows_form && lead <= 0x7fffffffLL)
+        return zkx_fftconvolve_c64(static_cast<const zkx_complex64*>(in1), static_cast<int>(lead), q.s1[rank - 1],
+                                   static_cast<const zkx_complex64*>(in2), q.s2[rank - 1], mode,
+                                   static_cast<zkx_complex64*>(out), stream);
     // convolution commutes: 'valid' takes the larger operand first
     const void* a = q.swap ? in2 : in1;
     const void* b = q.swap ? in1 : in2;

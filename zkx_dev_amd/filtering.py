@@ -326,17 +326,8 @@ def fftconvolve(in1, in2, mode="full", axes=None, *, stream=None):
     check(lib().zkx_fftconvolve_nd_shape(s1, s2, rank, mask, modes[mode], shape))
     out_shape = tuple(int(v) for v in shape)
     y = torch.empty(out_shape, dtype=torch.complex64 if complex_out else torch.float32, device=a.device)
-    n2_last = b.shape[-1]
-    if (complex_out and a.is_complex() and b.is_complex() and ax == [rank - 1] and mode != "valid"
-            and all(v == 1 for v in b.shape[:-1]) and n2_last <= 4096):
-        # one kernel row against every row of in1: overlap-save rows
-        n1 = a.shape[-1]
-        rows = a.numel() // n1
-        check(lib().zkx_fftconvolve_c64(_ptr(a), rows, n1, _ptr(b), n2_last, modes[mode], _ptr(y),
-                                        _stream(stream)))
-    else:
-        check(lib().zkx_fftconvolve_nd(_ptr(a), s1, int(a.is_complex()), _ptr(b), s2, int(b.is_complex()), rank,
-                                       mask, modes[mode], _ptr(y), _stream(stream)))
+    check(lib().zkx_fftconvolve_nd(_ptr(a), s1, int(a.is_complex()), _ptr(b), s2, int(b.is_complex()), rank,
+                                   mask, modes[mode], _ptr(y), _stream(stream)))
     return y.reshape(()) if scalar else y
 
 
