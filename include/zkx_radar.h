@@ -205,7 +205,9 @@ long long zkx_ambgfun_out_len(int nx, int ny, int cut);
  *   cut 1 (delay, at delay cut_value seconds): out double [nfreq];
  *   cut 2 (Doppler, at cut_value Hz): out double [rows].
  * fs must be nonzero for the cuts; cuts need x_count == nx and y_count == ny.
- * The 2-D and delay cuts need nfreq <= 8192 (UNSUPPORTED_OPERATION beyond).
+ * The 2-D and delay cuts take nfreq up to 2^26 (UNSUPPORTED_OPERATION
+ * beyond): one fused LDS kernel up to 8192, the global FFT beyond (the 2-D
+ * cut in chunks of about 256 MiB of delay products).
  * Zero-norm inputs give NaN, as in the reference.
  */
 int zkx_ambgfun_c64(const zkx_complex64* x, size_t x_count, int nx, const zkx_complex64* y, size_t y_count,

@@ -242,3 +242,23 @@ def test_injected_fft_failure_switch_resets():
     assert L.zkx_get_kernel_variant(b"inject_fft_failure") == 0
     assert L.zkx_set_kernel_variant(b"inject_fft_failure", 2) == _lib.INVALID_ARGUMENT
     assert L.zkx_get_kernel_variant(b"inject_fft_failure") == 0
+
+
+def test_length_limits_refused_before_device_work():
+    """Transform lengths beyond the FFT's range are refused with their codes
+    before anything touches the (here absent) device."""
+    L = _lib.lib()
+    h = Host(16)
+    big = (1 << 25) + 1  # nx + ny - 1 > 2^26 for the 2-D / delay cuts
+    for cut in (0, 1):
+        rc = L.zkx_ambgfun_c64(h.p(h.x), big, big, h.p(h.x), big, big, cut, 1.0, 0.0, 1, h.p(h.out), None)
+        assert rc == CODES["UNSUPPORTED_OPERATION"], (cut, rc)
+    s1 = (ctypes.c_longlong * 1)(1 << 26)
+    s2 = (ctypes.c_longlong * 1)(2)
+    rc = L.zkx_fftconvolve_nd(h.p(h.x), s1, 0, h.p(h.x), s2, 0, 1, 1, 0, h.p(h.out), None)
+    assert rc == CODES["INVALID_SHAPE"]
+    d1 = (ctypes.c_int * 2)(1 << 16, 1 << 16)
+    d2 = (ctypes.c_int * 2)(2, 2)
+    rc = L.zkx_correlate_nd(h.p(h.x), d1, h.p(h.x), d2, 2, 0, 0, h.p(h.out), None)
+    assert rc == CODES["INVALID_SHAPE"]
+    assert h.untouched()

@@ -106,7 +106,6 @@ def test_high_rank_2d_uses_first_extent():
 
 def test_errors_and_nan():
     from zkx_dev_amd import radar
-    from zkx_dev_amd._lib import ZkxError
     x = _dev(_rand(8, 1))
     with pytest.raises(ValueError):
         radar.ambgfun(x, 0.0, 1.0, cut="delay")
@@ -114,7 +113,37 @@ def test_errors_and_nan():
         radar.ambgfun(_dev(_rand(8, 1).reshape(4, 2)), 1.0, 1.0, cut="doppler")
     with pytest.raises(ValueError):
         radar.ambgfun(x, 1.0, 1.0, cut="range")
-    with pytest.raises(ZkxError):
-        radar.ambgfun(_dev(_rand(4097, 2)), 1.0, 1.0)  # nfreq 16384 > 8192
     z = radar.ambgfun(_dev(np.zeros(6, np.complex64)), 1.0, 1.0).cpu().numpy()
     assert np.all(np.isnan(z))
+
+
+@pytest.mark.parametrize("out_fp64", [False, True])
+def test_2d_beyond_one_lds_pass(out_fp64):
+    """nx = 6000, ny = 3000: rows 8999, nfreq 16384, through the global FFT
+    in row chunks; sampled rows (both ends, the zero-delay row, chunk
+    seams) against the FP64 FFT formulation."""
+    from zkx_dev_amd import radar
+    print(f"ambgfun 2d nx=6000 ny=3000 fp64={out_fp64}", flush=True)
+    x, y = _rand(6000, 61), _rand(3000, 62, 0.5)
+    a = radar.ambgfun(_dev(x), 1.0, 1.0, y=_dev(y), out_fp64=out_fp64)
+    assert tuple(a.shape) == (8999, 16384)
+    assert a.dtype == (torch.float64 if out_fp64 else torch.float32)
+    xn = x.astype(np.complex128) / np.linalg.norm(x.astype(np.complex128))
+    yn = y.astype(np.complex128) / np.linalg.norm(y.astype(np.complex128))
+    chunk = (256 << 20) // (8 * 16384)
+    rng = np.random.default_rng(2)
+    for row in [0, 1, 5999, 8998, chunk - 1, chunk, 2 * chunk] + rng.integers(0, 8999, 5).tolist():
+        ref = orc.ambgfun_fft64_row(xn, yn, int(row), 16384)
+        assert orc.rel_l2(a[int(row)].cpu().numpy(), ref) < 1e-5, row
+    assert float(a.max()) <= 1.0 + 1e-5
+
+
+def test_delay_cut_beyond_one_lds_pass():
+    """nx = ny = 5000 (nfreq 16384): the three whole-length transforms."""
+    from zkx_dev_amd import radar
+    x = _rand(5000, 71)
+    for tau in (0.0, 3.1e-4, -1.7e-3):
+        got = radar.ambgfun(_dev(x), 2.0e5, 1.0, cut="delay", cutValue=tau).cpu().numpy()
+        ref = orc.ambgfun_fft64(x, 2.0e5, cut="delay", cut_value=tau)
+        assert got.shape == ref.shape == (16384,)
+        assert orc.rel_l2(got, ref) < 1e-5, tau

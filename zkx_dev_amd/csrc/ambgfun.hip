@@ -16,6 +16,11 @@
 //     (the reference's circular correlation of length rows by FFTs; with
 //     rows = nx + ny - 1 it has no wrap-around, so it is evaluated directly).
 //
+// Beyond one LDS pass (nfreq > 8192, up to 2^26) the same steps run through
+// the global any-length FFT: the 2-D cut in chunks of rows (delay products
+// written to a workspace, unscaled inverse transforms, |.| stored with the
+// Doppler shift), the delay cut as three whole-length transforms.
+//
 // MI355X layout: the 2-D cut is one fused kernel -- each block builds its rows'
 // delay products in registers from the two waveforms (L2-resident), runs the
 // inverse transform in LDS (Stockham, radix 16) and writes |.| with the
@@ -23,9 +28,11 @@
 // (4 or 8 bytes per cell).  The delay cut's three transforms run in one block.
 // Norms are FP64 sums (the reference uses an FP32 scaled sum); every other
 // step is FP32, as in the reference.
+#include "fft_any.hpp"
 #include "fft_lds.hpp"
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
 
@@ -298,6 +305,142 @@ int dispatch_delay(int nfreq, const float2* x, long long x_count, const float2* 
 
 thread_local zkx::ThreadWorkspace t_amb_ws;
 
+// ---------------------------------------------------------------------------
+// nfreq > 8192
+// ---------------------------------------------------------------------------
+// D[rr][c] = yn[c] conj(xn[c - (nx-1) + row0 + rr]) for c < N (0 outside).
+__global__ void __launch_bounds__(256)
+amb_products_kernel(const float2* __restrict__ x, int nx, const float2* __restrict__ y, int ny, long long row0,
+                    long long chunk, int N, const float* __restrict__ norms, float2* __restrict__ D)
+{
+    const float inv_den = 1.0f / (norms[0] * norms[1]);
+    const long long total = chunk * N;
+    const int lg = __builtin_ctz(static_cast<unsigned>(N));  // N is a power of two
+    for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += static_cast<long long>(gridDim.x) * 256) {
+        const long long rr = e >> lg;
+        const int c = static_cast<int>(e & (N - 1));
+        const long long xc = c - (nx - 1) + row0 + rr;
+        float2 d = make_float2(0.f, 0.f);
+        if (c < ny && xc >= 0 && xc < nx) d = zkx::cscale(zkx::cmulc(y[c], x[xc]), inv_den);
+        D[e] = d;
+    }
+}
+
+template <class O>
+__global__ void __launch_bounds__(256)
+amb_store_kernel(const float2* __restrict__ D, long long chunk, int N, long long row0, O* __restrict__ out)
+{
+    const long long total = chunk * N;
+    const int lg = __builtin_ctz(static_cast<unsigned>(N));
+    for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += static_cast<long long>(gridDim.x) * 256) {
+        const long long rr = e >> lg;
+        const int f = static_cast<int>(e & (N - 1));
+        out[(row0 + rr) * N + ((f + N / 2) & (N - 1))] = out_value<O>(magnitude(D[e]));
+    }
+}
+
+// Delay cut steps on one signal of N samples.
+__global__ void __launch_bounds__(256)
+amb_delay_load_kernel(const float2* __restrict__ x, long long x_count, int N, const float* __restrict__ norms,
+                      float2* __restrict__ v)
+{
+    const float xn = norms[0];
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < N; i += static_cast<long long>(gridDim.x) * 256)
+        v[i] = i < x_count ? make_float2(x[i].x / xn, x[i].y / xn) : make_float2(0.f, 0.f);
+}
+
+__global__ void __launch_bounds__(256)
+amb_delay_phase_kernel(int N, double fs, double tau, float2* __restrict__ v)
+{
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < N; i += static_cast<long long>(gridDim.x) * 256) {
+        const double fd = -0.5 * fs + fs * static_cast<double>(i) / N;
+        double turns = fd * tau;
+        turns -= rint(turns);
+        double s, c;
+        sincospi(2.0 * turns, &s, &c);
+        v[i] = zkx::cmul(v[i], make_float2(static_cast<float>(c), static_cast<float>(s)));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+amb_delay_product_kernel(const float2* __restrict__ y, int ny, int N, const float* __restrict__ norms,
+                         float2* __restrict__ v)
+{
+    const float yn = norms[1];
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < N; i += static_cast<long long>(gridDim.x) * 256) {
+        float2 p = make_float2(0.f, 0.f);
+        if (i < ny) p = zkx::cmulc(make_float2(y[i].x / yn, y[i].y / yn), v[i]);
+        v[i] = p;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+amb_delay_store_kernel(const float2* __restrict__ v, int N, double* __restrict__ out)
+{
+    for (long long f = blockIdx.x * 256LL + threadIdx.x; f < N; f += static_cast<long long>(gridDim.x) * 256)
+        out[(f + N / 2) & (N - 1)] = static_cast<double>(hypotf(v[f].x, v[f].y));
+}
+
+unsigned amb_grid(long long n)
+{
+    const long long b = (n + 255) / 256;
+    return static_cast<unsigned>(b < 16384 ? (b > 0 ? b : 1) : 16384);
+}
+
+// Rows per chunk of the long 2-D cut: about 256 MiB of complex products.
+long long amb_chunk_rows(long long rows, int N)
+{
+    const long long c = std::max<long long>(1, (256LL << 20) / (8LL * N));
+    return std::min(rows, c);
+}
+
+size_t amb_long_bytes(int cut, long long rows, int N)
+{
+    const long long r = cut == 0 ? amb_chunk_rows(rows, N) : 1;
+    return (sizeof(float2) * static_cast<size_t>(r) * N + 255) / 256 * 256 + zkx::fft_any_workspace_bytes(r, N);
+}
+
+template <class O>
+int amb_2d_long(const float2* x, int nx, const float2* y, int ny, long long rows, int N, const float* norms,
+                char* ws, O* out, hipStream_t st)
+{
+    const long long chunk = amb_chunk_rows(rows, N);
+    float2* D = reinterpret_cast<float2*>(ws);
+    char* any = ws + (sizeof(float2) * static_cast<size_t>(chunk) * N + 255) / 256 * 256;
+    for (long long row0 = 0; row0 < rows; row0 += chunk) {
+        const long long c = std::min(chunk, rows - row0);
+        hipLaunchKernelGGL(amb_products_kernel, dim3(amb_grid(c * N)), dim3(256), 0, st, x, nx, y, ny, row0, c, N,
+                           norms, D);
+        ZKX_LAUNCH_CHECK();
+        int rc = zkx::fft_any_rows(D, c, N, true, 1.0f, D, any, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL((amb_store_kernel<O>), dim3(amb_grid(c * N)), dim3(256), 0, st, D, c, N, row0, out);
+        ZKX_LAUNCH_CHECK();
+    }
+    return ZKX_SUCCESS;
+}
+
+int amb_delay_long(const float2* x, long long x_count, const float2* y, int ny, int N, const float* norms, double fs,
+                   double tau, char* ws, double* out, hipStream_t st)
+{
+    float2* v = reinterpret_cast<float2*>(ws);
+    char* any = ws + (sizeof(float2) * static_cast<size_t>(N) + 255) / 256 * 256;
+    const unsigned g = amb_grid(N);
+    hipLaunchKernelGGL(amb_delay_load_kernel, dim3(g), dim3(256), 0, st, x, x_count, N, norms, v);
+    ZKX_LAUNCH_CHECK();
+    int rc = zkx::fft_any_rows(v, 1, N, false, 1.0f, v, any, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(amb_delay_phase_kernel, dim3(g), dim3(256), 0, st, N, fs, tau, v);
+    ZKX_LAUNCH_CHECK();
+    if ((rc = zkx::fft_any_rows(v, 1, N, true, 1.0f / static_cast<float>(N), v, any, st))) return rc;
+    hipLaunchKernelGGL(amb_delay_product_kernel, dim3(g), dim3(256), 0, st, y, ny, N, norms, v);
+    ZKX_LAUNCH_CHECK();
+    if ((rc = zkx::fft_any_rows(v, 1, N, true, 1.0f, v, any, st))) return rc;
+    hipLaunchKernelGGL(amb_delay_store_kernel, dim3(g), dim3(256), 0, st, v, N, out);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -336,8 +479,9 @@ int zkx_ambgfun_c64(const zkx_complex64* x, size_t x_count, int nx, const zkx_co
     const long long rows = static_cast<long long>(nx) + ny - 1;
     long long nfreq = 1;
     while (nfreq < rows) nfreq <<= 1;
-    if (cut != 2 && nfreq > kMaxLen)
-        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "ambgfun 2-D and delay cuts need nx + ny - 1 <= 8192");
+    if (cut != 2 && nfreq > zkx::kMaxAnyFftPow2)
+        ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "ambgfun 2-D and delay cuts need nx + ny - 1 <= 2^26");
+    const bool long_form = cut != 2 && nfreq > kMaxLen;
     if (cut != 1 && rows > std::numeric_limits<int>::max())
         ZKX_FAIL(ZKX_INVALID_SHAPE, "ambgfun input length exceeds int capacity");
     if (cut != 0 && !out_fp64) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "ambgfun one-dimensional cuts return FP64");
@@ -346,7 +490,8 @@ int zkx_ambgfun_c64(const zkx_complex64* x, size_t x_count, int nx, const zkx_co
     const float2* yd = reinterpret_cast<const float2*>(y);
 
     const size_t norm_bytes = 256;
-    const size_t ws_bytes = norm_bytes + (cut == 2 ? sizeof(float2) * static_cast<size_t>(nx + ny) : 0);
+    const size_t ws_bytes = norm_bytes + (cut == 2 ? sizeof(float2) * static_cast<size_t>(nx + ny) : 0) +
+                            (long_form ? amb_long_bytes(cut, rows, static_cast<int>(nfreq)) : 0);
     void* ws = nullptr;
     zkx::WorkspaceLease lease_(t_amb_ws, st);
     int rc = lease_.get(ws_bytes, &ws);
@@ -356,6 +501,18 @@ int zkx_ambgfun_c64(const zkx_complex64* x, size_t x_count, int nx, const zkx_co
                        yd, static_cast<long long>(y_count), norms);
     ZKX_LAUNCH_CHECK();
 
+    if (long_form) {
+        char* lw = static_cast<char*>(ws) + norm_bytes;
+        if ((rc = zkx::fft_any_prepare(static_cast<int>(nfreq)))) return rc;
+        if (cut == 1)
+            return amb_delay_long(xd, static_cast<long long>(x_count), yd, ny, static_cast<int>(nfreq), norms, fs,
+                                  cut_value, lw, static_cast<double*>(out), st);
+        if (out_fp64)
+            return amb_2d_long<double>(xd, nx, yd, ny, rows, static_cast<int>(nfreq), norms, lw,
+                                       static_cast<double*>(out), st);
+        return amb_2d_long<float>(xd, nx, yd, ny, rows, static_cast<int>(nfreq), norms, lw, static_cast<float*>(out),
+                                  st);
+    }
     if (cut == 0) {
         if (out_fp64)
             return dispatch_2d<double>(static_cast<int>(nfreq), xd, nx, yd, ny, static_cast<int>(rows), norms,
