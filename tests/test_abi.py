@@ -40,7 +40,8 @@ def test_python_binding_covers_header():
     assert not missing, missing
 
 
-@pytest.mark.parametrize("name", ["task1_chain", "filter_chain", "typed_ops", "fft_interface"])
+@pytest.mark.parametrize("name", ["task1_chain", "filter_chain", "typed_ops", "fft_interface",
+                                  "fft_backend_protocols"])
 def test_cpp_operator_api_compiles(name):
     """The C++ operator API headers (include/cusignal) compile for a caller
     written like the reference's Task1 steps."""

@@ -212,3 +212,26 @@ def test_fft_interface_objects(tmp_path, n):
     # 2 staging buffers of n complex64 while an FFTInterface(n) lives; n for a DeviceArray
     before, alive, after, arr = tracked.tolist()
     assert alive - before == 2 * n * 8 and after == before and arr - before == n * 8
+
+
+def test_fft_backend_protocols_quick():
+    """tests/cpp/fft_backend_protocols.cpp with shortened counts: the
+    reference's FFT-backend performance cases (round trip within its
+    bounds, and here within 1e-6), execute / plan-lifecycle stability (no
+    host heap growth beyond 4 KiB, no device growth) and the create /
+    allocate / execute / free memory rounds."""
+    import re
+    import subprocess
+    exe = os.path.join(BIN, "fft_backend_protocols")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (run __graft_entry__.build())")
+    print("running fft_backend_protocols quick", flush=True)
+    r = subprocess.run([exe, "quick"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    perf = [dict(kv.split("=") for kv in line.split()[1:]) for line in r.stdout.splitlines()
+            if line.startswith("[FFT-PERF]")]
+    assert len(perf) == 8
+    for p in perf:
+        assert p["correctness"] == "pass" and float(p["roundtrip_relative_l2"]) < 1e-6, p
+    assert re.search(r"\[FFT-PROTOCOLS\]\[SUMMARY\] status=passed", r.stdout)
