@@ -66,7 +66,10 @@ def bench_filters(args):
     resident custom taps); channelize_poly of them into 16 and 64 channels
     (32 taps per phase); their direct correlate with the 255 taps; and
     fftconvolve ('same') of the 16 complex channels with a 1025-tap lowpass;
-    correlate2d ('same', symm) of a 2048 x 2048 float32 image with 7 x 7 taps.
+    correlate2d ('same', symm) of a 2048 x 2048 float32 image with 7 x 7 taps;
+    the N-D fftconvolve of that image with a 63 x 63 kernel, the N-D call's
+    rows form on the channels, and the N-D correlate of the image with a
+    33 x 33 kernel.
     Reports per-call time, algorithmic bytes, the HBM fraction and the
     direct-form flop rate for each."""
     import torch
@@ -85,6 +88,9 @@ def bench_filters(args):
     h_1025 = torch.from_numpy(signal.firwin(1025, 0.1).astype(np.complex64)).to(dev)
     img = torch.from_numpy(np.random.default_rng(8).standard_normal((2048, 2048)).astype(np.float32)).to(dev)
     k7 = torch.from_numpy(np.random.default_rng(9).standard_normal((7, 7)).astype(np.float32)).to(dev)
+    k63 = torch.from_numpy(np.random.default_rng(10).standard_normal((63, 63)).astype(np.float32)).to(dev)
+    k33 = torch.from_numpy(np.random.default_rng(11).standard_normal((33, 33)).astype(np.float32)).to(dev)
+    h_1025_row = h_1025.reshape(1, -1).contiguous()
     stream = torch.cuda.current_stream()
     out = {}
     cases = {
@@ -118,6 +124,19 @@ def bench_filters(args):
         # ('same', mirrored boundary)
         "correlate2d_same_symm_2048sq_7x7": (lambda: convolution.correlate2d(img, k7, "same", "symm"),
                                              2048 * 2048 * 8 + 49 * 4, 2048 * 2048 * 49 * 2),
+        # N-D fftconvolve of the image with a 63 x 63 kernel over both axes
+        # ('same', real result; padded to 2048 + 62 -> 4096 per axis); flops
+        # counted as the direct form's MACs
+        "fftconvolve_nd_same_2048sq_63x63": (lambda: filtering.fftconvolve(img, k63, "same"),
+                                             2048 * 2048 * 8 + 63 * 63 * 4, 2048 * 2048 * 63 * 63 * 2),
+        # the 16 complex channels against one 1025-tap row along axis 1 of a
+        # [1, 1025] kernel ('full'): the overlap-save rows form of the N-D call
+        "fftconvolve_nd_rows_16x2^20_1x1025": (lambda: filtering.fftconvolve(x, h_1025_row, "full", [1]),
+                                               16 * (1 << 20) * 8 + 16 * ((1 << 20) + 1024) * 8 + 1025 * 8,
+                                               16 * (1 << 20) * 1025 * 8),
+        # N-D correlate by FFT of the image with a 33 x 33 kernel ('same')
+        "correlate_nd_same_2048sq_33x33": (lambda: convolution.correlate(img, k33, "same"),
+                                           2048 * 2048 * 8 + 33 * 33 * 4, 2048 * 2048 * 33 * 33 * 2),
     }
     for name, (fn, nbytes, flops) in cases.items():
         for _ in range(args.warmup):

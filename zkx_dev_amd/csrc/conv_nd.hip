@@ -92,7 +92,7 @@ nd_mul_kernel(const float2* X1, const float2* __restrict__ X2, NdMul p, float2* 
 {
     const int last = p.rank - 1;
     const int gl = p.g[last];
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int j0 = blockIdx.x * kNdSpan + threadIdx.x;
     const long long rows = nd_rows(p.g, p.rank);
     for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
         long long q = r, o1 = 0, o2 = 0;
@@ -102,35 +102,166 @@ nd_mul_kernel(const float2* X1, const float2* __restrict__ X2, NdMul p, float2* 
             o1 += c * p.s1[d];
             o2 += c * p.s2[d];
         }
-        if (j < gl) Y[r * gl + j] = cmul(X1[o1 + j * p.s1[last]], X2[o2 + j * p.s2[last]]);
+#pragma unroll
+        for (int k = 0; k < kNdPer; ++k) {
+            const int j = j0 + k * 256;
+            if (j >= gl) break;
+            Y[r * gl + j] = cmul(X1[o1 + j * p.s1[last]], X2[o2 + j * p.s2[last]]);
+        }
     }
 }
 
-// Transform X (extents g) along dimension d; T: scratch of X's size.
-int fft_along(float2* X, const int* g, int rank, int d, bool inverse, float2* T, char* any_ws, hipStream_t st)
+// Y = A^ B^ from Z = A^ + i B^ (A, B real): per element
+// (Z[k]^2 - conj(Z[-k])^2) / 4i, with -k taken modulo the extent along the
+// dimensions in `neg` and k itself along the others.  Y must not alias Z.
+struct NdPacked {
+    int rank;
+    int g[kMaxNdRank];
+    unsigned neg;            // bit k: position k is a transformed dimension
+    const unsigned* absmax;  // the packing's maxima (the scale s of B)
+};
+
+__global__ void __launch_bounds__(256)
+nd_mul_packed_kernel(const float2* __restrict__ Z, NdPacked p, float2* __restrict__ Y)
 {
-    const int n = g[d];
-    if (n == 1) return ZKX_SUCCESS;
-    long long outer = 1, inner = 1;
-    for (int k = 0; k < d; ++k) outer *= g[k];
-    for (int k = d + 1; k < rank; ++k) inner *= g[k];
-    const float scale = inverse ? 1.0f / static_cast<float>(n) : 1.0f;
-    if (inner == 1) return fft_any_rows(X, outer, n, inverse, scale, X, any_ws, st);
-    if (inner > 0x7fffffffLL) ZKX_FAIL(ZKX_INVALID_SHAPE, "fft convolve: extent too large");
-    int rc = transpose(X, outer, n, static_cast<int>(inner), T, st);
-    if (rc) return rc;
-    rc = fft_any_rows(T, outer * inner, n, inverse, scale, T, any_ws, st);
-    if (rc) return rc;
-    return transpose(T, outer, static_cast<int>(inner), n, X, st);
+    const int last = p.rank - 1;
+    const int gl = p.g[last];
+    const bool negl = p.neg >> last & 1u;
+    const float q4 = ldexpf(0.25f, -nd_pair_shift(p.absmax));  // 1 / (4 s)
+    const int j0 = blockIdx.x * kNdSpan + threadIdx.x;
+    const long long rows = nd_rows(p.g, p.rank);
+    for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
+        // partner row: every leading coordinate negated where transformed
+        long long q = r, pr = 0, mul = 1;
+        for (int d = last - 1; d >= 0; --d) {
+            const int c = static_cast<int>(q % p.g[d]);
+            q /= p.g[d];
+            const int cn = (p.neg >> d & 1u) ? (c == 0 ? 0 : p.g[d] - c) : c;
+            pr += static_cast<long long>(cn) * mul;
+            mul *= p.g[d];
+        }
+        const float2* zr = Z + r * gl;
+        const float2* zp = Z + pr * gl;
+        float2* yr = Y + r * gl;
+#pragma unroll
+        for (int k = 0; k < kNdPer; ++k) {
+            const int j = j0 + k * 256;
+            if (j >= gl) break;
+            const int jn = negl ? (j == 0 ? 0 : gl - j) : j;
+            const float2 a = zr[j], b = zp[jn];
+            // a^2 - conj(b)^2 = (ax^2 - ay^2 - bx^2 + by^2) + i (2 ax ay + 2 bx by)
+            const float re = (a.x * a.x - a.y * a.y) - (b.x * b.x - b.y * b.y);
+            const float im = 2.0f * (a.x * a.y + b.x * b.y);
+            // divided by 4 i s: (re + i im) / (4 i s) = (im - i re) / (4 s)
+            yr[j] = make_float2(q4 * im, -q4 * re);
+        }
+    }
 }
 
-int fft_axes(float2* X, const int* g, int rank, unsigned axes, bool inverse, float2* T, char* any_ws,
-             hipStream_t st)
+// An operand during the spectral part: its buffer, its extents in the
+// current dimension order and which original dimension each position holds.
+struct NdArr {
+    float2* p;
+    int rank;
+    int dims[kMaxNdRank];
+    int orig[kMaxNdRank];
+    long long count() const
+    {
+        long long c = 1;
+        for (int k = 0; k < rank; ++k) c *= dims[k];
+        return c;
+    }
+};
+
+NdArr nd_arr(float2* p, const int* g, int rank)
 {
-    for (int d = rank - 1; d >= 0; --d) {
-        if (!(axes >> d & 1u)) continue;
-        int rc = fft_along(X, g, rank, d, inverse, T, any_ws, st);
+    NdArr a{};
+    a.p = p;
+    a.rank = rank;
+    for (int k = 0; k < rank; ++k) {
+        a.dims[k] = g[k];
+        a.orig[k] = k;
+    }
+    return a;
+}
+
+// Move the last dimension to the front ([outer][last] -> [last][outer]),
+// or the first to the back, through the spare buffer, which then holds the
+// old storage.
+int rotate(NdArr& a, bool last_to_front, float2*& spare, hipStream_t st)
+{
+    const int R = a.rank;
+    const long long total = a.count();
+    int rc;
+    if (last_to_front) {
+        const int c = a.dims[R - 1];
+        rc = transpose(a.p, 1, static_cast<int>(total / c), c, spare, st);
+        const int od = a.orig[R - 1];
+        for (int k = R - 1; k > 0; --k) {
+            a.dims[k] = a.dims[k - 1];
+            a.orig[k] = a.orig[k - 1];
+        }
+        a.dims[0] = c;
+        a.orig[0] = od;
+    } else {
+        const int f = a.dims[0];
+        rc = transpose(a.p, 1, f, static_cast<int>(total / f), spare, st);
+        const int od = a.orig[0];
+        for (int k = 0; k + 1 < R; ++k) {
+            a.dims[k] = a.dims[k + 1];
+            a.orig[k] = a.orig[k + 1];
+        }
+        a.dims[R - 1] = f;
+        a.orig[R - 1] = od;
+    }
+    if (rc) return rc;
+    float2* t = a.p;
+    a.p = spare;
+    spare = t;
+    return ZKX_SUCCESS;
+}
+
+// Row transforms along the current last dimension when it is convolved.
+int fft_last(NdArr& a, unsigned axes, bool inverse, char* any_ws, hipStream_t st)
+{
+    const int d = a.orig[a.rank - 1];
+    const int n = a.dims[a.rank - 1];
+    if (!(axes >> d & 1u) || n == 1) return ZKX_SUCCESS;
+    const float scale = inverse ? 1.0f / static_cast<float>(n) : 1.0f;
+    return fft_any_rows(a.p, a.count() / n, n, inverse, scale, a.p, any_ws, st);
+}
+
+// Number of rotations that bring the lowest convolved dimension last.
+int nd_rotations(unsigned axes, int rank)
+{
+    int lo = 0;
+    while (!(axes >> lo & 1u)) ++lo;
+    return rank - 1 - lo;
+}
+
+// Forward: transform the last dimension, rotate it to the front, and so on
+// until every convolved dimension has been last once (S rotations); the
+// spectrum stays in the rotated order.
+int forward_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st)
+{
+    const int S = nd_rotations(axes, a.rank);
+    for (int s = 0; s <= S; ++s) {
+        int rc = fft_last(a, axes, false, any_ws, st);
         if (rc) return rc;
+        if (s < S && (rc = rotate(a, true, spare, st))) return rc;
+    }
+    return ZKX_SUCCESS;
+}
+
+// Inverse of forward_nd's order: transform the last dimension, rotate the
+// first to the back, ..., ending in the original order.
+int inverse_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st)
+{
+    const int S = nd_rotations(axes, a.rank);
+    for (int s = 0; s <= S; ++s) {
+        int rc = fft_last(a, axes, true, any_ws, st);
+        if (rc) return rc;
+        if (s < S && (rc = rotate(a, false, spare, st))) return rc;
     }
     return ZKX_SUCCESS;
 }
@@ -164,57 +295,92 @@ int nd_conv_plan(const int* s1, const int* s2, int rank, unsigned axes, NdConvPl
     if (q.n1 > kMaxNdElements || q.n2 > kMaxNdElements || q.n > kMaxNdElements)
         ZKX_FAIL(ZKX_INVALID_SHAPE, "fft convolve: padded operands exceed 2^31 elements");
     q.y_is_x1 = q.n1 == q.n;
+    const long long big = std::max(q.n1, std::max(q.n2, q.n));
     for (int d = 0; d < rank; ++d) {
         if (!(axes >> d & 1u)) continue;
-        any = std::max(any, fft_any_workspace_bytes(std::max(q.n1, std::max(q.n2, q.n)) / q.g[d], q.g[d]));
+        any = std::max(any, fft_any_workspace_bytes(big / q.g[d], q.g[d]));
     }
-    const long long big = std::max(q.n1, std::max(q.n2, q.n));
-    size_t off = 0;
-    q.off_x1 = off;
-    off += al256(sizeof(float2) * q.n1);
-    q.off_x2 = off;
-    off += al256(sizeof(float2) * q.n2);
-    q.off_y = q.y_is_x1 ? q.off_x1 : off;
-    if (!q.y_is_x1) off += al256(sizeof(float2) * q.n);
-    q.off_t = off;
-    off += al256(sizeof(float2) * big);
-    q.off_any = off;
-    off += any;
-    q.bytes = off;
+    // four buffers of the largest operand: X1, X2 and two that rotations and
+    // the product move between
+    const size_t buf = al256(sizeof(float2) * big);
+    q.off_x1 = 0;
+    q.off_x2 = buf;
+    q.off_y = 2 * buf;
+    q.off_t = 3 * buf;
+    q.off_any = 4 * buf;
+    q.off_scal = 4 * buf + al256(any);
+    q.bytes = q.off_scal + 256;
     *p = q;
     return ZKX_SUCCESS;
 }
 
-int nd_conv_run(const NdConvPlan& p, char* ws, hipStream_t st)
+int nd_conv_run(const NdConvPlan& p, char* ws, float2** y, hipStream_t st)
 {
     for (int d = 0; d < p.rank; ++d) {
         if (!(p.axes >> d & 1u)) continue;
         int rc = fft_any_prepare(p.g[d]);
         if (rc) return rc;
     }
-    float2* X1 = reinterpret_cast<float2*>(ws + p.off_x1);
-    float2* X2 = reinterpret_cast<float2*>(ws + p.off_x2);
-    float2* Y = reinterpret_cast<float2*>(ws + p.off_y);
-    float2* T = reinterpret_cast<float2*>(ws + p.off_t);
     char* any = ws + p.off_any;
-    int rc = fft_axes(X1, p.g1, p.rank, p.axes, false, T, any, st);
+    float2* spare = reinterpret_cast<float2*>(ws + p.off_t);
+    float2* free_buf = reinterpret_cast<float2*>(ws + p.off_y);
+    NdArr a = nd_arr(reinterpret_cast<float2*>(ws + p.off_x1), p.g1, p.rank);
+    NdArr b = nd_arr(reinterpret_cast<float2*>(ws + p.off_x2), p.g2, p.rank);
+    int rc = forward_nd(a, p.axes, spare, any, st);
     if (rc) return rc;
-    rc = fft_axes(X2, p.g2, p.rank, p.axes, false, T, any, st);
-    if (rc) return rc;
+    if ((rc = forward_nd(b, p.axes, spare, any, st))) return rc;
+    // product in the rotated order (the same for both operands)
+    NdArr c = a;
+    c.p = p.y_is_x1 ? a.p : free_buf;
+    for (int k = 0; k < p.rank; ++k) c.dims[k] = p.g[c.orig[k]];
     NdMul m{};
     m.rank = p.rank;
     long long st1[kMaxNdRank], st2[kMaxNdRank];
-    nd_strides(p.g1, p.rank, st1);
-    nd_strides(p.g2, p.rank, st2);
-    for (int d = 0; d < p.rank; ++d) {
-        m.g[d] = p.g[d];
-        m.s1[d] = p.g1[d] == 1 && p.g[d] > 1 ? 0 : st1[d];
-        m.s2[d] = p.g2[d] == 1 && p.g[d] > 1 ? 0 : st2[d];
+    nd_strides(a.dims, p.rank, st1);
+    nd_strides(b.dims, p.rank, st2);
+    for (int k = 0; k < p.rank; ++k) {
+        m.g[k] = c.dims[k];
+        m.s1[k] = a.dims[k] == 1 && c.dims[k] > 1 ? 0 : st1[k];
+        m.s2[k] = b.dims[k] == 1 && c.dims[k] > 1 ? 0 : st2[k];
     }
-    const long long rows = p.n / p.g[p.rank - 1];
-    hipLaunchKernelGGL(nd_mul_kernel, nd_grid(rows, p.g[p.rank - 1]), dim3(256), 0, st, X1, X2, m, Y);
+    const int last = c.dims[p.rank - 1];
+    hipLaunchKernelGGL(nd_mul_kernel, nd_grid(c.count() / last, last), dim3(256), 0, st, a.p, b.p, m, c.p);
     ZKX_LAUNCH_CHECK();
-    return fft_axes(Y, p.g, p.rank, p.axes, true, T, any, st);
+    // b's buffer is free now: it serves the inverse rotations
+    float2* spare2 = b.p;
+    if ((rc = inverse_nd(c, p.axes, spare2, any, st))) return rc;
+    *y = c.p;
+    return ZKX_SUCCESS;
+}
+
+int nd_conv_run_real_pair(const NdConvPlan& p, char* ws, float2** y, hipStream_t st)
+{
+    for (int d = 0; d < p.rank; ++d) {
+        if (!(p.axes >> d & 1u)) continue;
+        int rc = fft_any_prepare(p.g[d]);
+        if (rc) return rc;
+    }
+    char* any = ws + p.off_any;
+    float2* spare = reinterpret_cast<float2*>(ws + p.off_t);
+    NdArr z = nd_arr(reinterpret_cast<float2*>(ws + p.off_x1), p.g1, p.rank);
+    int rc = forward_nd(z, p.axes, spare, any, st);
+    if (rc) return rc;
+    NdArr c = z;
+    c.p = reinterpret_cast<float2*>(ws + p.off_y);
+    NdPacked m{};
+    m.rank = p.rank;
+    m.absmax = reinterpret_cast<const unsigned*>(ws + p.off_scal);
+    for (int k = 0; k < p.rank; ++k) {
+        m.g[k] = z.dims[k];
+        if (p.axes >> z.orig[k] & 1u) m.neg |= 1u << k;
+    }
+    const int last = z.dims[p.rank - 1];
+    hipLaunchKernelGGL(nd_mul_packed_kernel, nd_grid(z.count() / last, last), dim3(256), 0, st, z.p, m, c.p);
+    ZKX_LAUNCH_CHECK();
+    float2* spare2 = z.p;  // Z is no longer needed
+    if ((rc = inverse_nd(c, p.axes, spare2, any, st))) return rc;
+    *y = c.p;
+    return ZKX_SUCCESS;
 }
 
 }  // namespace zkx
@@ -349,9 +515,20 @@ int zkx_fftconvolve_nd(const void* in1, const long long* shape1, int complex1, c
         ZKX_LAUNCH_CHECK();
         return ZKX_SUCCESS;
     };
-    if ((rc = pad(a, ca, sa, p.g1, reinterpret_cast<float2*>(ws + p.off_x1)))) return rc;
-    if ((rc = pad(b, cb, sb, p.g2, reinterpret_cast<float2*>(ws + p.off_x2)))) return rc;
-    if ((rc = nd_conv_run(p, ws, st))) return rc;
+    float2* Yp = nullptr;
+    bool same_g = true;
+    for (int d = 0; d < rank; ++d) same_g = same_g && p.g1[d] == p.g2[d];
+    if (!ca && !cb && same_g) {
+        // two real operands: one packed transform
+        if ((rc = nd_pack_real_pair(p, ws, NdLoadF32{static_cast<const float*>(a)}, sa,
+                                    NdLoadF32{static_cast<const float*>(b)}, sb, 0, st)))
+            return rc;
+        if ((rc = nd_conv_run_real_pair(p, ws, &Yp, st))) return rc;
+    } else {
+        if ((rc = pad(a, ca, sa, p.g1, reinterpret_cast<float2*>(ws + p.off_x1)))) return rc;
+        if ((rc = pad(b, cb, sb, p.g2, reinterpret_cast<float2*>(ws + p.off_x2)))) return rc;
+        if ((rc = nd_conv_run(p, ws, &Yp, st))) return rc;
+    }
     // centred crop of the full result, which sits at the origin of Y
     NdCrop c{};
     c.rank = rank;
@@ -360,7 +537,7 @@ int zkx_fftconvolve_nd(const void* in1, const long long* shape1, int complex1, c
         c.o[d] = q.out[d];
         c.start[d] = (q.full[d] - q.out[d]) / 2;
     }
-    const float2* Y = reinterpret_cast<const float2*>(ws + p.off_y);
+    const float2* Y = Yp;
     const dim3 grid = nd_grid(count_of(q.out, rank) / q.out[rank - 1], q.out[rank - 1]);
     if (complex1 || complex2)
         hipLaunchKernelGGL(nd_crop_kernel<NdStoreC64>, grid, dim3(256), 0, st, Y, c,

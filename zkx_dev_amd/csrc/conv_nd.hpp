@@ -11,9 +11,12 @@
 // multiplies them (broadcasting) and inverse transforms the product into Y
 // (scaled by 1/F per axis), and the caller crops Y with nd_crop_kernel.
 //
-// An axis whose elements are not contiguous is brought to the rows of a
-// scratch array by a tiled transpose, transformed by the row FFT (one LDS
-// pass per row up to 8192 points, four-step beyond), and transposed back.
+// Every transform runs on rows (one LDS pass per row up to 8192 points,
+// four-step beyond): the forward part transforms the last dimension, then
+// rotates it to the front (a tiled transpose, [outer][last] -> [last][outer])
+// until the lowest convolved dimension has been last; the product is taken
+// in that rotated order and the inverse undoes the rotations, so a 2-D
+// convolution costs three transposes in all.
 // Kernels walk arrays by rows (all dimensions but the last): a block decodes
 // its row index once, with wave-uniform integer arithmetic, and its threads
 // cover the contiguous last dimension.
@@ -42,7 +45,7 @@ struct NdConvPlan {
     int g1[kMaxNdRank] = {}, g2[kMaxNdRank] = {}, g[kMaxNdRank] = {};
     long long n1 = 0, n2 = 0, n = 0;  // elements of X1, X2, Y
     bool y_is_x1 = false;             // the product overwrites X1 (no broadcast into it)
-    size_t off_x1 = 0, off_x2 = 0, off_y = 0, off_t = 0, off_any = 0, bytes = 0;
+    size_t off_x1 = 0, off_x2 = 0, off_y = 0, off_t = 0, off_any = 0, off_scal = 0, bytes = 0;  // four equal buffers, FFT scratch, scalars
 };
 
 // Plan for operands of extents s1, s2 (rank 1..8, every extent >= 1, the
@@ -51,9 +54,15 @@ struct NdConvPlan {
 int nd_conv_plan(const int* s1, const int* s2, int rank, unsigned axes, NdConvPlan* p);
 
 // Spectral part on a workspace of p.bytes bytes whose X1 / X2 regions the
-// caller has filled: Y (at ws + p.off_y) = the linear convolution, extents
-// p.g, over the convolved axes.
-int nd_conv_run(const NdConvPlan& p, char* ws, hipStream_t st);
+// caller has filled: *y (one of the workspace's buffers) = the linear
+// convolution over the convolved axes, extents p.g in the original order.
+int nd_conv_run(const NdConvPlan& p, char* ws, float2** y, hipStream_t st);
+
+// The same for two real operands packed into X1 as Z = A + i s B (requires
+// g1 == g2; nd_pack_real_pair does the packing): one forward transform, then
+// A^ B^ = (Z[k]^2 - conj(Z[-k])^2) / (4 i s), k negated along the convolved
+// axes only; *y holds the real convolution in its real parts.
+int nd_conv_run_real_pair(const NdConvPlan& p, char* ws, float2** y, hipStream_t st);
 
 // Loader for nd_pad_kernel: element i of the source as complex.
 struct NdLoadC64 {
@@ -83,12 +92,16 @@ __device__ __forceinline__ long long nd_rows(const int* ext, int rank)
     return r;
 }
 
+// A block covers kNdSpan elements of a row (kNdPer per thread, 256 apart).
+constexpr int kNdPer = 4;
+constexpr int kNdSpan = 256 * kNdPer;
+
 template <class Load>
 __global__ void __launch_bounds__(256) nd_pad_kernel(Load ld, NdPad p, float2* __restrict__ X)
 {
     const int last = p.rank - 1;
     const int gl = p.g[last], sl = p.s[last];
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int j0 = blockIdx.x * kNdSpan + threadIdx.x;
     const long long rows = nd_rows(p.g, p.rank);
     for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
         // leading coordinates of row r (uniform across the block)
@@ -100,10 +113,97 @@ __global__ void __launch_bounds__(256) nd_pad_kernel(Load ld, NdPad p, float2* _
             inside = inside && c < p.s[d];
             src += static_cast<long long>(p.reverse ? p.s[d] - 1 - c : c) * p.sstride[d];
         }
-        if (j >= gl) continue;
-        float2 v = make_float2(0.f, 0.f);
-        if (inside && j < sl) v = ld(src + (p.reverse ? sl - 1 - j : j));
-        X[r * gl + j] = v;
+        float2* row = X + r * gl;
+#pragma unroll
+        for (int k = 0; k < kNdPer; ++k) {
+            const int j = j0 + k * 256;
+            if (j >= gl) break;
+            float2 v = make_float2(0.f, 0.f);
+            if (inside && j < sl) v = ld(src + (p.reverse ? sl - 1 - j : j));
+            row[j] = v;
+        }
+    }
+}
+
+// Two real operands of one padded extent g packed into one complex array,
+// Z = A + i s B (B optionally read backwards in every dimension), so that one
+// forward transform serves both.  s is the power of two that brings max|B|
+// to the binade of max|A|: the product A^ B^ is recovered from Z by a
+// difference of squares, whose rounding error scales with |A^|^2 + |s B^|^2,
+// so balanced operands keep it at the level of a plain product; a power of
+// two scales exactly.  absmax[0..1] hold max|A|, max|B| as float bits.
+struct NdPad2 {
+    int rank;
+    int g[kMaxNdRank];
+    int sa[kMaxNdRank], sb[kMaxNdRank];
+    long long sastride[kMaxNdRank], sbstride[kMaxNdRank];
+    int reverse_b;
+    const unsigned* absmax;
+};
+
+// log2 of the balancing scale s from the two maxima (0 when either is 0).
+__device__ __forceinline__ int nd_pair_shift(const unsigned* absmax)
+{
+    const float ma = __uint_as_float(absmax[0]), mb = __uint_as_float(absmax[1]);
+    if (!(ma > 0.0f) || !(mb > 0.0f)) return 0;
+    int ea, eb;
+    (void)frexpf(ma, &ea);
+    (void)frexpf(mb, &eb);
+    const int d = ea - eb;  // kept within +-60 so that 1/(4s) stays a normal float
+    return d > 60 ? 60 : (d < -60 ? -60 : d);
+}
+
+// max |re| of two real sources into absmax[0], absmax[1] (float bits; the
+// caller zeroes them first).  Non-negative floats order as their bits.
+template <class LoadA, class LoadB>
+__global__ void __launch_bounds__(256)
+nd_absmax2_kernel(LoadA la, long long na, LoadB lb, long long nb, unsigned* absmax)
+{
+    float m[2] = {0.0f, 0.0f};
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < na; i += stride) m[0] = fmaxf(m[0], fabsf(la(i).x));
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nb; i += stride) m[1] = fmaxf(m[1], fabsf(lb(i).x));
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        float v = m[k];
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+        if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax(absmax + k, __float_as_uint(v));
+    }
+}
+
+static __global__ void nd_zero2_kernel(unsigned* p)
+{
+    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
+}
+
+template <class LoadA, class LoadB>
+__global__ void __launch_bounds__(256) nd_pad2_kernel(LoadA la, LoadB lb, NdPad2 p, float2* __restrict__ Z)
+{
+    const int last = p.rank - 1;
+    const int gl = p.g[last], sla = p.sa[last], slb = p.sb[last];
+    const int shift = nd_pair_shift(p.absmax);
+    const int j0 = blockIdx.x * kNdSpan + threadIdx.x;
+    const long long rows = nd_rows(p.g, p.rank);
+    for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
+        long long q = r, oa = 0, ob = 0;
+        bool ina = true, inb = true;
+        for (int d = last - 1; d >= 0; --d) {
+            const int c = static_cast<int>(q % p.g[d]);
+            q /= p.g[d];
+            ina = ina && c < p.sa[d];
+            inb = inb && c < p.sb[d];
+            oa += static_cast<long long>(c) * p.sastride[d];
+            ob += static_cast<long long>(p.reverse_b ? p.sb[d] - 1 - c : c) * p.sbstride[d];
+        }
+        float2* row = Z + r * gl;
+#pragma unroll
+        for (int k = 0; k < kNdPer; ++k) {
+            const int j = j0 + k * 256;
+            if (j >= gl) break;
+            const float a = (ina && j < sla) ? la(oa + j).x : 0.0f;
+            const float b = (inb && j < slb) ? lb(ob + (p.reverse_b ? slb - 1 - j : j)).x : 0.0f;
+            row[j] = make_float2(a, ldexpf(b, shift));
+        }
     }
 }
 
@@ -129,7 +229,7 @@ __global__ void __launch_bounds__(256) nd_crop_kernel(const float2* __restrict__
 {
     const int last = p.rank - 1;
     const int ol = p.o[last];
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int j0 = blockIdx.x * kNdSpan + threadIdx.x;
     const long long rows = nd_rows(p.o, p.rank);
     for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
         long long q = r, src = static_cast<long long>(p.start[last]);
@@ -138,14 +238,20 @@ __global__ void __launch_bounds__(256) nd_crop_kernel(const float2* __restrict__
             q /= p.o[d];
             src += static_cast<long long>(p.start[d] + c) * p.ystride[d];
         }
-        if (j < ol) st(r * ol + j, Y[src + j]);
+#pragma unroll
+        for (int k = 0; k < kNdPer; ++k) {
+            const int j = j0 + k * 256;
+            if (j >= ol) break;
+            st(r * ol + j, Y[src + j]);
+        }
     }
 }
 
-// Launch shape of the row kernels over `rows` rows of `last` elements.
+// Launch shape of the pad and crop kernels over `rows` rows of `last`
+// elements (kNdSpan per block along a row).
 inline dim3 nd_grid(long long rows, int last)
 {
-    const unsigned gx = static_cast<unsigned>((last + 255) / 256);
+    const unsigned gx = static_cast<unsigned>((last + kNdSpan - 1) / kNdSpan);
     const unsigned gy = static_cast<unsigned>(rows < 65535 ? (rows > 0 ? rows : 1) : 65535);
     return dim3(gx, gy);
 }
@@ -158,6 +264,41 @@ inline void nd_strides(const int* d, int rank, long long* stride)
         stride[k] = s;
         s *= d[k];
     }
+}
+
+// Pack A (extents sa) and B (extents sb, read backwards when reverse_b) into
+// X1 of the plan's workspace, with the balancing scale.
+template <class LoadA, class LoadB>
+int nd_pack_real_pair(const NdConvPlan& p, char* ws, LoadA la, const int* sa, LoadB lb, const int* sb,
+                      int reverse_b, hipStream_t st)
+{
+    const int rank = p.rank;
+    unsigned* absmax = reinterpret_cast<unsigned*>(ws + p.off_scal);
+    long long na = 1, nb = 1, rows = 1;
+    NdPad2 pp{};
+    pp.rank = rank;
+    for (int d = 0; d < rank; ++d) {
+        na *= sa[d];
+        nb *= sb[d];
+        pp.g[d] = p.g1[d];
+        pp.sa[d] = sa[d];
+        pp.sb[d] = sb[d];
+        if (d + 1 < rank) rows *= p.g1[d];
+    }
+    nd_strides(sa, rank, pp.sastride);
+    nd_strides(sb, rank, pp.sbstride);
+    pp.reverse_b = reverse_b;
+    pp.absmax = absmax;
+    hipLaunchKernelGGL(nd_zero2_kernel, dim3(1), dim3(64), 0, st, absmax);
+    ZKX_LAUNCH_CHECK();
+    const long long nmax = na > nb ? na : nb;
+    const unsigned blocks = static_cast<unsigned>((nmax + 255) / 256 < 2048 ? (nmax + 255) / 256 : 2048);
+    hipLaunchKernelGGL((nd_absmax2_kernel<LoadA, LoadB>), dim3(blocks), dim3(256), 0, st, la, na, lb, nb, absmax);
+    ZKX_LAUNCH_CHECK();
+    hipLaunchKernelGGL((nd_pad2_kernel<LoadA, LoadB>), nd_grid(rows, p.g1[rank - 1]), dim3(256), 0, st, la, lb, pp,
+                       reinterpret_cast<float2*>(ws + p.off_x1));
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
 }
 
 }  // namespace zkx

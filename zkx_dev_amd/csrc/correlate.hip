@@ -546,25 +546,12 @@ int correlate_nd_fft(const void* in1, const int* s1, const void* in2, const int*
     rc = lease_.get(p.bytes, &wsv);
     if (rc) return rc;
     char* ws = static_cast<char*>(wsv);
-    for (int which = 0; which < 2; ++which) {
-        NdPad pp{};
-        pp.rank = rank;
-        const int* s = which ? s2 : s1;
-        const int* g = which ? p.g2 : p.g1;
-        long long rows = 1;
-        for (int d = 0; d < rank; ++d) {
-            pp.g[d] = g[d];
-            pp.s[d] = s[d];
-            if (d + 1 < rank) rows *= g[d];
-        }
-        nd_strides(s, rank, pp.sstride);
-        pp.reverse = which;
-        float2* X = reinterpret_cast<float2*>(ws + (which ? p.off_x2 : p.off_x1));
-        hipLaunchKernelGGL(nd_pad_kernel<NdLoadTyped<T>>, nd_grid(rows, g[rank - 1]), dim3(256), 0, st,
-                           NdLoadTyped<T>{static_cast<const T*>(which ? in2 : in1)}, pp, X);
-        ZKX_LAUNCH_CHECK();
-    }
-    if ((rc = nd_conv_run(p, ws, st))) return rc;
+    // both operands are real: one packed transform of in1 + i s reverse(in2)
+    if ((rc = nd_pack_real_pair(p, ws, NdLoadTyped<T>{static_cast<const T*>(in1)}, s1,
+                                NdLoadTyped<T>{static_cast<const T*>(in2)}, s2, 1, st)))
+        return rc;
+    float2* Y = nullptr;
+    if ((rc = nd_conv_run_real_pair(p, ws, &Y, st))) return rc;
     NdCrop c{};
     c.rank = rank;
     nd_strides(p.g, rank, c.ystride);
@@ -575,7 +562,7 @@ int correlate_nd_fft(const void* in1, const int* s1, const void* in2, const int*
         if (d + 1 < rank) rows *= out_dims[d];
     }
     hipLaunchKernelGGL(nd_crop_kernel<NdStoreTyped<T>>, nd_grid(rows, out_dims[rank - 1]), dim3(256), 0, st,
-                       reinterpret_cast<const float2*>(ws + p.off_y), c, NdStoreTyped<T>{static_cast<T*>(out)});
+                       Y, c, NdStoreTyped<T>{static_cast<T*>(out)});
     ZKX_LAUNCH_CHECK();
     return ZKX_SUCCESS;
 }
