@@ -155,8 +155,8 @@ def bench_filters(args):
                       "workload": "filters", "results": out}))
 
 
-def bench_batched(args):
-    """The uneven batch configuration: 256 dwells with P uniform over
+def batched_summary(dwells, steps, warmup, tune=True):
+    """The uneven batch configuration: `dwells` dwells with P uniform over
     {32, 64, 128, 256}, R uniform over 1024..16384 in steps of 256, LFM
     templates of 64..1024 taps (Hamming, normalised), 1-8 targets per dwell at
     -10 dB, Doppler Hann, CA-CFAR g2x2 r8x8 pfa 1e-6 (FP64 window sums), all
@@ -166,7 +166,7 @@ def bench_batched(args):
     from zkx_dev_amd import engine, synth
 
     dev = torch.device("cuda:0")
-    specs = synth.uneven_shapes(args.dwells, seed=20261205)
+    specs = synth.uneven_shapes(dwells, seed=20261205)
     ub = synth.make_uneven_batch_device(specs, seed=20261206)
     eng = engine.DwellEngine(pc_window="hamming", normalize=True, doppler_window="hann", guard=GUARD,
                              reference=REF, pfa=PFA, lanes=1)
@@ -176,13 +176,13 @@ def bench_batched(args):
     det = torch.empty(total, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
-    if not args.no_tune:
+    if tune:
         eng.autotune(ub.x, descs, thr, det)
     tune_s = time.perf_counter() - t0
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         eng.run(ub.x, descs, thr, det)
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     for a, b in evs:
         a.record(stream)
         eng.run(ub.x, descs, thr, det)
@@ -194,7 +194,6 @@ def bench_batched(args):
     samples = sum(P * R for P, R in ub.shapes)
     alg = sum(13 * P * R + 8 * T for (P, R), T in zip(ub.shapes, ub.taps))
     # one dwell per engine call, the unbatched form of the same kernels
-    one = descs[:1].copy()
     one_lat = []
     for i in range(min(n, 64)):
         one = descs[i:i + 1].copy()
@@ -206,24 +205,32 @@ def bench_batched(args):
         one_lat.append(a.elapsed_time(b))
     one_rate = 1e3 / float(np.mean(one_lat))
     value = n / (step_ms * 1e-3)
+    out = {"value": value, "unit": "dwells/s", "steps": steps, "warmup": warmup, "ms_per_step": step_ms,
+           "msamples_per_s": samples / (step_ms * 1e-3) / 1e6,
+           "pipeline_hbm_frac": alg / (step_ms * 1e-3) / HBM_PEAK,
+           "alg_bytes_per_step": alg, "samples_per_step": samples,
+           "latency_ms": {"step_p50": percentile(lat, 50), "step_p99": percentile(lat, 99)},
+           "one_dwell_per_call_dwells_per_s": one_rate, "speedup_vs_one_per_call": value / one_rate,
+           "autotune_s": tune_s, "tuning_lines": len(eng.tuning().splitlines()),
+           "workload": f"{n} uneven dwells, P in {{32..256}}, R in 1024..16384 step 256, "
+                       "T 64..1024, Doppler Hann, CA-CFAR g2x2 r8x8 pfa 1e-6; HIP graph of two persistent kernels"}
+    eng.close()
+    del ub, thr, det
+    return out
+
+
+def bench_batched(args):
+    """The uneven batch configuration as its own JSON line (batched_summary)."""
+    r = batched_summary(args.dwells, args.steps, args.warmup, tune=not args.no_tune)
     print(json.dumps({
         "metric": "range-Doppler dwells/s + Msamples/s (HBM roofline %) and p99 latency",
-        "workload": "batched uneven dwells", "value": value, "unit": "dwells/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_ms,
-        "msamples_per_s": samples / (step_ms * 1e-3) / 1e6,
-        "pipeline_hbm_frac": alg / (step_ms * 1e-3) / HBM_PEAK,
-        "alg_bytes_per_step": alg, "samples_per_step": samples,
-        "latency_ms": {"step_p50": percentile(lat, 50), "step_p99": percentile(lat, 99)},
-        "one_dwell_per_call_dwells_per_s": one_rate, "speedup_vs_one_per_call": value / one_rate,
-        "autotune_s": tune_s, "tuning_lines": len(eng.tuning().splitlines()),
+        "workload": "batched uneven dwells", "n_gpus": 1, **{k: v for k, v in r.items() if k != "workload"},
         "dtype": "fp32", "data": "synthetic (seeded shapes, targets and noise, generated on the device)",
-        "config": {"workload": f"{n} uneven dwells, P in {{32..256}}, R in 1024..16384 step 256, "
-                               "T 64..1024, Doppler Hann, CA-CFAR g2x2 r8x8 pfa 1e-6",
-                   "launch": "HIP graph of two persistent kernels"}}))
-    eng.close()
+        "config": {"workload": r["workload"], "launch": "HIP graph of two persistent kernels"}}))
 
 
-def bench_serve(args):
+def serve_summary(rates=(1000, 5000, 20000), seconds=1.0,
+                  sweep_rates=(20000, 50000, 80000, 100000, 120000), sweep_seconds=0.5, plain_n=40):
     """The serving configuration: Poisson arrivals at 1k, 5k and 20k
     requests/s of dwells drawn equiprobably from 64x2048, 128x4096 and
     256x8192 complex64 (resident in a device pool), 4 resident chirp templates
@@ -263,8 +270,8 @@ def bench_serve(args):
                 "msamples_per_s": sum(p_ * r_ for p_, r_ in zip(pp, rr)) / makespan / 1e6}
 
     run(5000, 0.1, [0, 1, 2])  # warm-up: graphs and workspaces on every lane
-    mixed = {f"{r // 1000}k": run(r, 1.0, [0, 1, 2]) for r in (1000, 5000, 20000)}
-    sweep = {f"{r // 1000}k": run(r, 0.5, [1]) for r in (20000, 50000, 80000, 100000, 120000)}
+    mixed = {f"{r // 1000}k": run(r, seconds, [0, 1, 2]) for r in rates}
+    sweep = {f"{r // 1000}k": run(r, sweep_seconds, [1]) for r in sweep_rates}
     ok = [v["achieved_per_s"] for v in sweep.values() if v["p99_ms"] <= 2.0
           and v["achieved_per_s"] >= 0.95 * v["offered_per_s"]]
     sustained = max(ok) if ok else 0.0
@@ -278,20 +285,27 @@ def bench_serve(args):
         i0 = si * per_shape
         xs = [ub.x[ub.offsets[i0 + k]:ub.offsets[i0 + k] + p * r].view(p, r) for k in range(per_shape)]
         tm = torch.from_numpy(synth.lfm_chirp(taps[0])).to(dev)
-        plain[f"{p}x{r}"], _ = plain_dropin(radar, xs, tm, 40)
+        plain[f"{p}x{r}"], _ = plain_dropin(radar, xs, tm, plain_n)
     plain_mixed = 1.0 / np.mean([1.0 / v for v in plain.values()])
+    out = {"mixed_shapes": mixed, "sweep_128x4096": sweep,
+           "sustained_128x4096_at_p99_le_2ms": sustained,
+           "plain_dropin_dwells_per_s": plain, "plain_dropin_mixed_dwells_per_s": plain_mixed,
+           "sustained_vs_plain_128x4096": sustained / plain["128x4096"] if plain.get("128x4096") else None,
+           "server_stats": stats}
+    eng.close()
+    del ub
+    return out
+
+
+def bench_serve(args):
+    """The serving configuration as its own JSON line (serve_summary)."""
+    r = serve_summary()
     print(json.dumps({
         "metric": "range-Doppler dwells/s + Msamples/s (HBM roofline %) and p99 latency",
-        "workload": "serving", "unit": "dwells/s", "n_gpus": 1,
-        "mixed_shapes": mixed, "sweep_128x4096": sweep,
-        "sustained_128x4096_at_p99_le_2ms": sustained,
-        "plain_dropin_dwells_per_s": plain, "plain_dropin_mixed_dwells_per_s": plain_mixed,
-        "sustained_vs_plain_128x4096": sustained / plain["128x4096"] if plain.get("128x4096") else None,
-        "server_stats": stats, "dtype": "fp32",
+        "workload": "serving", "unit": "dwells/s", "n_gpus": 1, **r, "dtype": "fp32",
         "data": "synthetic dwells resident in a device pool (seeded targets and noise)",
         "config": {"workload": "Poisson dwell requests, shapes 64x2048 / 128x4096 / 256x8192, 4 templates "
                                "128..512 taps, 64 GB admission budget, 3 lanes, batches <= 64 dwells"}}))
-    eng.close()
 
 
 def bench_split(args):
@@ -535,8 +549,9 @@ def plain_dropin(radar, x_dwells, template, n, warm=3):
 
 
 def other_configs(reps):
-    """Short device-time measurements of the other configurations, for the
-    default line: the three polyphase calls and the split-BF16 Doppler DFT."""
+    """Short measurements of the other configurations, for the default line:
+    the three polyphase calls, the split-BF16 Doppler DFT, the uneven batch
+    through the engine and a short serving replay."""
     import torch
     from scipy import signal
     from zkx_dev_amd import filtering, radar, synth
@@ -583,6 +598,21 @@ def other_configs(reps):
     out["split_bf16_doppler_16x512x8192"] = {"us": t3 * 1e6, "algorithmic_tflops": flops / t3 / 1e12,
                                              "mfma_frac": flops / t3 / 2.5e15,
                                              "hbm_frac": 2 * C * Pp * Rr * 8 / t3 / HBM_PEAK}
+    del cube, y
+    torch.cuda.synchronize()
+    # batching: the 256 uneven dwells through the engine (autotuned, HIP graph)
+    b = batched_summary(256, steps=30, warmup=30, tune=True)
+    out["batched_uneven_256"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "msamples_per_s",
+                                                    "pipeline_hbm_frac", "latency_ms",
+                                                    "one_dwell_per_call_dwells_per_s", "autotune_s", "workload")}
+    # serving: Poisson arrivals at 20k requests/s of mixed shapes, and the
+    # 128x4096-only rate sustained at p99 <= 2 ms from a short sweep
+    sv = serve_summary(rates=(20000,), seconds=0.5, sweep_rates=(50000, 80000, 100000), sweep_seconds=0.3,
+                       plain_n=20)
+    out["serving_poisson"] = {"mixed_20k": sv["mixed_shapes"]["20k"],
+                              "sustained_128x4096_at_p99_le_2ms": sv["sustained_128x4096_at_p99_le_2ms"],
+                              "sustained_vs_plain_128x4096": sv["sustained_vs_plain_128x4096"],
+                              "plain_dropin_128x4096": sv["plain_dropin_dwells_per_s"].get("128x4096")}
     return out
 
 
