@@ -163,11 +163,19 @@ nd_absmax2_kernel(LoadA la, long long na, LoadB lb, long long nb, unsigned* absm
     const long long stride = static_cast<long long>(gridDim.x) * 256;
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < na; i += stride) m[0] = fmaxf(m[0], fabsf(la(i).x));
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nb; i += stride) m[1] = fmaxf(m[1], fabsf(lb(i).x));
+    // one atomic per block and operand: the waves meet in LDS first
+    __shared__ float part[2][4];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         float v = m[k];
         for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-        if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax(absmax + k, __float_as_uint(v));
+        if ((threadIdx.x & 63) == 0) part[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const float v = fmaxf(fmaxf(part[threadIdx.x][0], part[threadIdx.x][1]),
+                              fmaxf(part[threadIdx.x][2], part[threadIdx.x][3]));
+        if (v > 0.0f) atomicMax(absmax + threadIdx.x, __float_as_uint(v));
     }
 }
 
@@ -292,7 +300,8 @@ int nd_pack_real_pair(const NdConvPlan& p, char* ws, LoadA la, const int* sa, Lo
     hipLaunchKernelGGL(nd_zero2_kernel, dim3(1), dim3(64), 0, st, absmax);
     ZKX_LAUNCH_CHECK();
     const long long nmax = na > nb ? na : nb;
-    const unsigned blocks = static_cast<unsigned>((nmax + 255) / 256 < 2048 ? (nmax + 255) / 256 : 2048);
+    // a few blocks per CU, each striding over the operands
+    const unsigned blocks = static_cast<unsigned>((nmax + 255) / 256 < 512 ? (nmax + 255) / 256 : 512);
     hipLaunchKernelGGL((nd_absmax2_kernel<LoadA, LoadB>), dim3(blocks), dim3(256), 0, st, la, na, lb, nb, absmax);
     ZKX_LAUNCH_CHECK();
     hipLaunchKernelGGL((nd_pad2_kernel<LoadA, LoadB>), nd_grid(rows, p.g1[rank - 1]), dim3(256), 0, st, la, lb, pp,
