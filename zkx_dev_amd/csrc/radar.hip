@@ -393,77 +393,96 @@ cfar_colscan_stream_kernel(const float* __restrict__ x, int rows, int cols, floa
     }
 }
 
-// Row prefix (axis 1) in place with coalesced traffic: a block owns 64 rows;
-// its 256 threads move 64-column chunks between HBM and an LDS tile (one
-// float4 per thread per 16 columns, RD chunks in flight in registers), and the
-// first wave scans each tile row left to right.  Rows past the end and the
-// column tail are clamped so that every thread issues the same loads and
-// stores in every iteration of the main loop (exact waits); clamped lanes
-// store the values the owning lane stores.  Requires cols % 4 == 0.
-constexpr int RT_CW = 64, RT_TH = 256, RT_RD = 4;
+// Row prefix (axis 1) in place with coalesced traffic.  A block owns RT_TR
+// rows of one dwell and walks them in RT_TC-column chunks: its 256 threads
+// move each chunk between HBM and an LDS tile with float4 accesses (RT_RD
+// chunks in flight in registers), and the first RT_TR lanes of wave 0 scan the
+// tile's rows left to right, one row per lane, four columns per LDS access.
+// The tile row stride RT_TC+4 floats (4 banks past a multiple of the bank
+// count) keeps the loaders' row-major float4 writes and the scanning lanes'
+// float4 reads down the rows free of bank conflicts.  Rows past the end are
+// clamped to the last row and, in the last chunk, columns past the end to the
+// last float4 of the row: a clamped thread loads and stores the address its
+// owner does and stores the value its owner stores, so every thread issues the
+// same loads and stores in every chunk and the waits stay exact.  The columns
+// past the end are scanned after the row's last value and never stored.
+// Small row blocks spread one dwell's rows over many CUs; a single dwell is
+// then bound by the scanning lanes (one dependent add per element, one LDS
+// read and one write per four).
+// Requires cols % 4 == 0.
+constexpr int RT_TR = 16, RT_TC = 256, RT_TH = 256, RT_RD = 4;
+typedef float rt_f4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(RT_TH)
 cfar_rowscan_tile_kernel(float* I, int rows, int cols)
 {
-    __shared__ float tile[2][64][RT_CW + 1];
-    constexpr int PER = 64 * RT_CW / 4 / RT_TH;  // float4 per thread per chunk
+    constexpr int Q = RT_TC / 4;            // rt_f4 per tile row
+    constexpr int PER = RT_TR * Q / RT_TH;  // rt_f4 per thread per chunk
+    constexpr int LS = RT_TC + 4;           // tile row stride (floats)
+    constexpr int G = 16;                   // rt_f4 a scanning lane holds at once
+    static_assert(RT_TR * Q % RT_TH == 0 && RT_TR <= 64 && Q % G == 0, "tile shape");
+    __shared__ __align__(16) float tile[2][RT_TR * LS];
     const int tid = threadIdx.x;
     float* dw = I + static_cast<long long>(blockIdx.y) * rows * cols;
-    const int r0 = blockIdx.x * 64;
-    const int chunks = cols / RT_CW;  // full chunks; the tail is handled after
-    // element e of a chunk: row e / 16, float4 column e % 16
-    long long off[PER];
+    const int r0 = blockIdx.x * RT_TR;
+    const int chunks = (cols + RT_TC - 1) / RT_TC;
+    // element e of a chunk: tile row e / Q, rt_f4 column e % Q
+    long long roff[PER];
+    int q4[PER], tpos[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
         const int e = tid + i * RT_TH;
-        const int r = min(r0 + e / (RT_CW / 4), rows - 1);
-        off[i] = static_cast<long long>(r) * cols + 4 * (e % (RT_CW / 4));
+        roff[i] = static_cast<long long>(min(r0 + e / Q, rows - 1)) * cols;
+        q4[i] = 4 * (e % Q);
+        tpos[i] = (e / Q) * LS;
     }
-    float4 buf[RT_RD][PER];
-    if (chunks > 0) {
+    auto col = [&](int i, int k) { return min(k * RT_TC + q4[i], cols - 4); };
+    rt_f4 buf[RT_RD][PER];
 #pragma unroll
-        for (int d = 0; d < RT_RD; ++d)
-#pragma unroll
-            for (int i = 0; i < PER; ++i)
-                buf[d][i] = *reinterpret_cast<const float4*>(dw + off[i] + static_cast<long long>(min(d, chunks - 1)) * RT_CW);
-    }
-    float run = 0.0f;
-    const int lane = tid;  // scanning lane (first wave): row r0 + lane
-    auto step = [&](float4 (&cur)[PER], int k) {
-        float (*t)[RT_CW + 1] = tile[k & 1];
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int e = tid + i * RT_TH;
-            float* q = &t[e / (RT_CW / 4)][4 * (e % (RT_CW / 4))];
-            q[0] = cur[i].x;
-            q[1] = cur[i].y;
-            q[2] = cur[i].z;
-            q[3] = cur[i].w;
-        }
-        const int kn = min(k + RT_RD, chunks - 1);
+    for (int d = 0; d < RT_RD; ++d)
 #pragma unroll
         for (int i = 0; i < PER; ++i)
-            cur[i] = *reinterpret_cast<const float4*>(dw + off[i] + static_cast<long long>(kn) * RT_CW);
+            buf[d][i] = *reinterpret_cast<const rt_f4*>(dw + roff[i] + col(i, min(d, chunks - 1)));
+    float run = 0.0f;
+    auto step = [&](rt_f4 (&cur)[PER], int k) __attribute__((always_inline)) {
+        float* t = tile[k & 1];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) *reinterpret_cast<rt_f4*>(t + tpos[i] + q4[i]) = cur[i];
+        const int kn = min(k + RT_RD, chunks - 1);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) cur[i] = *reinterpret_cast<const rt_f4*>(dw + roff[i] + col(i, kn));
         __syncthreads();
-        if (tid < 64) {
-            float* row = t[lane];
-            float v[RT_CW];
+        if (tid < RT_TR) {
+            // groups of G float4: the next group's reads are issued before
+            // the current group's adds, so LDS latency hides behind the chain
+            rt_f4* row = reinterpret_cast<rt_f4*>(t + tid * LS);
+            rt_f4 v[2][G];
 #pragma unroll
-            for (int j = 0; j < RT_CW; ++j) v[j] = row[j];
+            for (int j = 0; j < G; ++j) v[0][j] = row[j];
 #pragma unroll
-            for (int j = 0; j < RT_CW; ++j) {
-                run = __fadd_rn(run, v[j]);
-                v[j] = run;
+            for (int g = 0; g < Q / G; ++g) {
+                rt_f4 (&a)[G] = v[g & 1];
+                if (g + 1 < Q / G) {
+#pragma unroll
+                    for (int j = 0; j < G; ++j) v[(g + 1) & 1][j] = row[(g + 1) * G + j];
+                }
+#pragma unroll
+                for (int j = 0; j < G; ++j) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        run = __fadd_rn(run, a[j][u]);
+                        a[j][u] = run;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < G; ++j) row[g * G + j] = a[j];
             }
-#pragma unroll
-            for (int j = 0; j < RT_CW; ++j) row[j] = v[j];
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int e = tid + i * RT_TH;
-            const float* q = &t[e / (RT_CW / 4)][4 * (e % (RT_CW / 4))];
-            *reinterpret_cast<float4*>(dw + off[i] + static_cast<long long>(k) * RT_CW) =
-                make_float4(q[0], q[1], q[2], q[3]);
+            const int c = col(i, k);
+            *reinterpret_cast<rt_f4*>(dw + roff[i] + c) =
+                *reinterpret_cast<const rt_f4*>(t + tpos[i] + (c - k * RT_TC));
         }
     };
     // whole groups of RT_RD chunks: straight-line bodies, so every wait is exact
@@ -475,15 +494,6 @@ cfar_rowscan_tile_kernel(float* I, int rows, int cols)
 #pragma unroll
     for (int d = 0; d < RT_RD; ++d)
         if (kmain + d < chunks) step(buf[d], kmain + d);
-    // column tail (cols % 64 values), in place, by the lanes that own a row
-    const int c0 = chunks * RT_CW;
-    if (c0 < cols && tid < 64 && r0 + lane < rows) {
-        float* row = dw + static_cast<long long>(r0 + lane) * cols;
-        for (int c = c0; c < cols; ++c) {
-            run = __fadd_rn(run, row[c]);
-            row[c] = run;
-        }
-    }
 }
 
 // Threshold and detection per cell from the integral image, in the reference's
@@ -1215,7 +1225,8 @@ int cfar_faithful_2d(const float* x, const float* colsum, int batch, int rows, i
                                rows, cols, I);
             ZKX_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(cfar_rowscan_tile_kernel, dim3((rows + 63) / 64, batch), dim3(RT_TH), 0, st, I, rows, cols);
+        hipLaunchKernelGGL(cfar_rowscan_tile_kernel, dim3((rows + RT_TR - 1) / RT_TR, batch), dim3(RT_TH), 0, st, I,
+                           rows, cols);
         ZKX_LAUNCH_CHECK();
         hipLaunchKernelGGL(cfar_threshold_rows_kernel, dim3((cols + 255) / 256, (rows + TK - 1) / TK, batch),
                            dim3(256), 0, st, x, I, rows, cols, g0, g1, r0, r1, alpha, count, thr, det);
