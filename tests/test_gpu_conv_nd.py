@@ -84,6 +84,25 @@ def test_fftconvolve_nd_long_axes(F, s1, s2, axes):
         assert orc.rel_l2(y.cpu().numpy(), orc.fftconvolve_nd(a, b, mode, axes)) < TOL
 
 
+@pytest.mark.parametrize("s1,s2,axes", [((1000,), (77,), None),             # one self-paired row
+                                        ((300, 40), (40, 9), None),        # 512-point last pass, 64 row pairs
+                                        ((130, 7), (127, 3), [0, 1]),      # 256-point last pass
+                                        ((3, 700, 20), (3, 50, 5), [1, 2]),  # leading axis not negated
+                                        ((5, 2000), (5, 100), [1]),        # every row its own partner
+                                        ((2100, 6), (30, 6), [0])])        # 4096-point last pass
+def test_fftconvolve_nd_fused_product(F, s1, s2, axes):
+    """Two real operands whose lowest convolved axis pads to 256..4096 points
+    take the fused last-axis transform + packed product + inverse kernel."""
+    rng = np.random.default_rng(sum(s1) + 7 * sum(s2))
+    a, b = _inputs(rng, s1, s2, False, False)
+    print(f"fftconvolve fused {s1} {s2} axes={axes}", flush=True)
+    for mode in ("full", "same", "valid"):
+        y = F.fftconvolve(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), mode, axes)
+        ref = orc.fftconvolve_nd(a, b, mode, axes)
+        assert tuple(y.shape) == ref.shape and y.dtype == torch.float32
+        assert orc.rel_l2(y.cpu().numpy(), ref) < TOL
+
+
 def test_fftconvolve_nd_rows_route_matches_general(F):
     """A complex kernel row against every row of in1 takes the overlap-save
     route; its result agrees with the general engine (real part of a real
@@ -122,7 +141,8 @@ DTYPES = {"FP32": torch.float32, "FP16": torch.float16, "INT32": torch.int32, "I
 
 
 @pytest.mark.parametrize("s1,s2", [((6, 9), (3, 4)), ((3, 4), (6, 9)), ((4, 5, 3), (2, 3, 2)), ((7, 2), (2, 2)),
-                                   ((64, 200), (9, 33)), ((2, 3, 4, 5), (2, 2, 2, 2))])
+                                   ((64, 200), (9, 33)), ((2, 3, 4, 5), (2, 2, 2, 2)),
+                                   ((300, 20), (30, 5))])  # fused 512-point last pass
 @pytest.mark.parametrize("mode", ["full", "same", "valid"])
 @pytest.mark.parametrize("dtype", list(DTYPES))
 def test_correlate_nd(s1, s2, mode, dtype):

@@ -16,6 +16,7 @@
 // to a power of two, which changes only the rounding of the result.
 #include "conv_nd.hpp"
 #include "fft_any.hpp"
+#include "fft_lds.hpp"
 #include "runtime.hpp"
 
 #include <algorithm>
@@ -158,6 +159,88 @@ nd_mul_packed_kernel(const float2* __restrict__ Z, NdPacked p, float2* __restric
     }
 }
 
+// The same product with the last dimension's forward and inverse transforms
+// fused around it (power-of-two N = that extent, always a transformed
+// dimension here): a block owns row r of Z and its partner row (every leading
+// coordinate negated where transformed), one LDS slot each.  Each slot
+// transforms its row, stores the spectrum in its slot, reads the partner's
+// bin -j, forms the product in registers and inverse transforms it (scaled by
+// 1/N) into the same row of Y.  A block whose partner row precedes its own
+// leaves both rows to the partner's block; a self-paired row runs in both
+// slots and slot 0 stores it.  Y may be Z: a block reads and writes only its
+// own two rows.
+template <int N>
+__global__ void __launch_bounds__(2 * (N / fft::elems_for(N)), N == 4096 ? 4 : 1)
+nd_packed_rows_kernel(const float2* Z, NdPacked p, const float2* __restrict__ tw, float2* Y)
+{
+    constexpr int E = fft::elems_for(N), T = N / E, SL = fft::RowLds::floats_for(N);
+    extern __shared__ float lds[];
+    const int slot = threadIdx.x / T, t = threadIdx.x % T;
+    const int last = p.rank - 1;
+    const long long r = blockIdx.x;
+    long long q = r, pr = 0, mul = 1;
+    for (int d = last - 1; d >= 0; --d) {
+        const int c = static_cast<int>(q % p.g[d]);
+        q /= p.g[d];
+        const int cn = (p.neg >> d & 1u) ? (c == 0 ? 0 : p.g[d] - c) : c;
+        pr += static_cast<long long>(cn) * mul;
+        mul *= p.g[d];
+    }
+    if (pr < r) return;  // the whole block: the partner's block does both rows
+    const long long row = slot == 0 ? r : pr;
+    const int other = pr == r ? slot : 1 - slot;
+    const fft::RowLds L{lds + slot * 2 * SL, lds + slot * 2 * SL + SL};
+    const fft::RowLds P{lds + other * 2 * SL, lds + other * 2 * SL + SL};
+    const float2* src = Z + row * N;
+    float2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) v[k] = src[t + k * T];
+    fft::transform<N, E, false>(v, L, tw, t);
+    __syncthreads();  // the transform's last LDS reads are done
+#pragma unroll
+    for (int k = 0; k < E; ++k) L.store(t + k * T, v[k]);
+    __syncthreads();
+    const float q4 = ldexpf(0.25f, -nd_pair_shift(p.absmax));  // 1 / (4 s)
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int j = t + k * T;
+        const float2 a = v[k], b = P.load((N - j) & (N - 1));
+        const float re = (a.x * a.x - a.y * a.y) - (b.x * b.x - b.y * b.y);
+        const float im = 2.0f * (a.x * a.y + b.x * b.y);
+        v[k] = make_float2(q4 * im, -q4 * re);
+    }
+    __syncthreads();  // the partner's reads of this slot are done
+    fft::transform<N, E, true>(v, L, tw, t);
+    if (slot == 0 || pr != r) {
+        float2* dst = Y + row * N;
+        const float sc = 1.0f / static_cast<float>(N);
+#pragma unroll
+        for (int k = 0; k < E; ++k) dst[t + k * T] = cscale(v[k], sc);
+    }
+}
+
+template <int N>
+int launch_packed_rows(const float2* Z, const NdPacked& m, long long rows, float2* Y, hipStream_t st)
+{
+    constexpr int T = N / fft::elems_for(N);
+    constexpr size_t bytes = sizeof(float) * 2 * 2 * fft::RowLds::floats_for(N);
+    const float2* tw = nullptr;
+    int rc = twiddles(N, &tw);
+    if (rc) return rc;
+    if (rows > 0x7fffffffLL) ZKX_FAIL(ZKX_INVALID_SHAPE, "fft convolve: too many rows");
+    if (bytes > 64 * 1024)
+        ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(nd_packed_rows_kernel<N>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    hipLaunchKernelGGL(nd_packed_rows_kernel<N>, dim3(static_cast<unsigned>(rows)), dim3(2 * T), bytes, st, Z, m,
+                       tw, Y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+// Last extents the fused product handles (one LDS pass, at least a wave per
+// block, both rows' registers within a 512-thread block's budget).
+bool packed_rows_fused(int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0; }
+
 // An operand during the spectral part: its buffer, its extents in the
 // current dimension order and which original dimension each position holds.
 struct NdArr {
@@ -241,12 +324,13 @@ int nd_rotations(unsigned axes, int rank)
 
 // Forward: transform the last dimension, rotate it to the front, and so on
 // until every convolved dimension has been last once (S rotations); the
-// spectrum stays in the rotated order.
-int forward_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st)
+// spectrum stays in the rotated order.  skip_final leaves the last
+// dimension untransformed (a fused product kernel does it).
+int forward_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st, bool skip_final = false)
 {
     const int S = nd_rotations(axes, a.rank);
     for (int s = 0; s <= S; ++s) {
-        int rc = fft_last(a, axes, false, any_ws, st);
+        int rc = s == S && skip_final ? ZKX_SUCCESS : fft_last(a, axes, false, any_ws, st);
         if (rc) return rc;
         if (s < S && (rc = rotate(a, true, spare, st))) return rc;
     }
@@ -254,12 +338,13 @@ int forward_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_
 }
 
 // Inverse of forward_nd's order: transform the last dimension, rotate the
-// first to the back, ..., ending in the original order.
-int inverse_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st)
+// first to the back, ..., ending in the original order.  skip_first: the
+// last dimension is already inverse transformed.
+int inverse_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st, bool skip_first = false)
 {
     const int S = nd_rotations(axes, a.rank);
     for (int s = 0; s <= S; ++s) {
-        int rc = fft_last(a, axes, true, any_ws, st);
+        int rc = s == 0 && skip_first ? ZKX_SUCCESS : fft_last(a, axes, true, any_ws, st);
         if (rc) return rc;
         if (s < S && (rc = rotate(a, false, spare, st))) return rc;
     }
@@ -363,7 +448,10 @@ int nd_conv_run_real_pair(const NdConvPlan& p, char* ws, float2** y, hipStream_t
     char* any = ws + p.off_any;
     float2* spare = reinterpret_cast<float2*>(ws + p.off_t);
     NdArr z = nd_arr(reinterpret_cast<float2*>(ws + p.off_x1), p.g1, p.rank);
-    int rc = forward_nd(z, p.axes, spare, any, st);
+    // after the rotations the last dimension is the lowest convolved one
+    const int lo = [&] { int d = 0; while (!(p.axes >> d & 1u)) ++d; return d; }();
+    const bool fused = packed_rows_fused(p.g[lo]);
+    int rc = forward_nd(z, p.axes, spare, any, st, fused);
     if (rc) return rc;
     NdArr c = z;
     c.p = reinterpret_cast<float2*>(ws + p.off_y);
@@ -375,10 +463,22 @@ int nd_conv_run_real_pair(const NdConvPlan& p, char* ws, float2** y, hipStream_t
         if (p.axes >> z.orig[k] & 1u) m.neg |= 1u << k;
     }
     const int last = z.dims[p.rank - 1];
-    hipLaunchKernelGGL(nd_mul_packed_kernel, nd_grid(z.count() / last, last), dim3(256), 0, st, z.p, m, c.p);
-    ZKX_LAUNCH_CHECK();
+    if (fused) {
+        const long long rows = z.count() / last;
+        switch (last) {
+        case 256: rc = launch_packed_rows<256>(z.p, m, rows, c.p, st); break;
+        case 512: rc = launch_packed_rows<512>(z.p, m, rows, c.p, st); break;
+        case 1024: rc = launch_packed_rows<1024>(z.p, m, rows, c.p, st); break;
+        case 2048: rc = launch_packed_rows<2048>(z.p, m, rows, c.p, st); break;
+        default: rc = launch_packed_rows<4096>(z.p, m, rows, c.p, st); break;
+        }
+        if (rc) return rc;
+    } else {
+        hipLaunchKernelGGL(nd_mul_packed_kernel, nd_grid(z.count() / last, last), dim3(256), 0, st, z.p, m, c.p);
+        ZKX_LAUNCH_CHECK();
+    }
     float2* spare2 = z.p;  // Z is no longer needed
-    if ((rc = inverse_nd(c, p.axes, spare2, any, st))) return rc;
+    if ((rc = inverse_nd(c, p.axes, spare2, any, st, fused))) return rc;
     *y = c.p;
     return ZKX_SUCCESS;
 }
