@@ -40,17 +40,21 @@ long long count_of(const int* d, int rank)
     return c;
 }
 
-// out[b][c][r] = in[b][r][c] for b < B, r < R, c < C; 32 x 32 tiles through
-// LDS (one padding column keeps the column reads on distinct banks).  The
-// tile index runs over blockIdx.x (either extent may be large), the batch
-// over blockIdx.y.
+// out[b][c][r] = in[b][r][c] for b < B, r < R and c in [c_lo, c_hi) of C
+// columns, with rows r >= r_live read as zeros (not read at all); output
+// rows outside [c_lo, c_hi) are left as they were.  32 x 32 tiles through LDS
+// (one padding column keeps the column reads on distinct banks).  The tile
+// index runs over blockIdx.x (either extent may be large), the batch over
+// blockIdx.y.
 __global__ void __launch_bounds__(256)
-nd_transpose_kernel(const float2* __restrict__ in, long long B, int R, int C, float2* __restrict__ out)
+nd_transpose_kernel(const float2* __restrict__ in, long long B, int R, int C, int r_live, int c_lo, int c_hi,
+                    float2* __restrict__ out)
 {
     __shared__ float2 tile[32][33];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    const int tiles_c = (C + 31) / 32;
-    const int r0 = static_cast<int>(blockIdx.x / tiles_c) * 32, c0 = static_cast<int>(blockIdx.x % tiles_c) * 32;
+    const int tiles_c = (c_hi - c_lo + 31) / 32;
+    const int r0 = static_cast<int>(blockIdx.x / tiles_c) * 32;
+    const int c0 = c_lo + static_cast<int>(blockIdx.x % tiles_c) * 32;
     const long long plane = static_cast<long long>(R) * C;
     for (long long b = blockIdx.y; b < B; b += gridDim.y) {
         const float2* src = in + b * plane;
@@ -58,24 +62,29 @@ nd_transpose_kernel(const float2* __restrict__ in, long long B, int R, int C, fl
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int r = r0 + ty + 8 * k, c = c0 + tx;
-            if (r < R && c < C) tile[ty + 8 * k][tx] = src[static_cast<long long>(r) * C + c];
+            if (r < R && c < c_hi)
+                tile[ty + 8 * k][tx] = r < r_live ? src[static_cast<long long>(r) * C + c] : make_float2(0.f, 0.f);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int c = c0 + ty + 8 * k, r = r0 + tx;
-            if (r < R && c < C) dst[static_cast<long long>(c) * R + r] = tile[tx][ty + 8 * k];
+            if (r < R && c < c_hi) dst[static_cast<long long>(c) * R + r] = tile[tx][ty + 8 * k];
         }
         __syncthreads();
     }
 }
 
-int transpose(const float2* in, long long B, int R, int C, float2* out, hipStream_t st)
+int transpose(const float2* in, long long B, int R, int C, float2* out, hipStream_t st, int r_live = -1,
+              int c_lo = 0, int c_hi = -1)
 {
-    const long long tiles = static_cast<long long>((R + 31) / 32) * ((C + 31) / 32);
+    if (r_live < 0) r_live = R;
+    if (c_hi < 0) c_hi = C;
+    if (c_hi <= c_lo) return ZKX_SUCCESS;
+    const long long tiles = static_cast<long long>((R + 31) / 32) * ((c_hi - c_lo + 31) / 32);
     if (tiles > 0x7fffffffLL) ZKX_FAIL(ZKX_INVALID_SHAPE, "fft convolve: transpose extent too large");
     const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(std::min<long long>(B, 65535)));
-    hipLaunchKernelGGL(nd_transpose_kernel, grid, dim3(256), 0, st, in, B, R, C, out);
+    hipLaunchKernelGGL(nd_transpose_kernel, grid, dim3(256), 0, st, in, B, R, C, r_live, c_lo, c_hi, out);
     ZKX_LAUNCH_CHECK();
     return ZKX_SUCCESS;
 }
@@ -270,15 +279,20 @@ NdArr nd_arr(float2* p, const int* g, int rank)
 
 // Move the last dimension to the front ([outer][last] -> [last][outer]),
 // or the first to the back, through the spare buffer, which then holds the
-// old storage.
-int rotate(NdArr& a, bool last_to_front, float2*& spare, hipStream_t st)
+// old storage.  last_to_front: outer rows from `live` on are zeros (read as
+// such); first_to_back: only the new leading-dimension rows [keep_lo,
+// keep_hi) are produced (all when keep_hi is 0).
+int rotate(NdArr& a, bool last_to_front, float2*& spare, hipStream_t st, long long live = -1, int keep_lo = 0,
+           int keep_hi = 0)
 {
     const int R = a.rank;
     const long long total = a.count();
     int rc;
     if (last_to_front) {
         const int c = a.dims[R - 1];
-        rc = transpose(a.p, 1, static_cast<int>(total / c), c, spare, st);
+        const long long outer = total / c;
+        rc = transpose(a.p, 1, static_cast<int>(outer), c, spare, st,
+                       static_cast<int>(live < 0 || live > outer ? outer : live));
         const int od = a.orig[R - 1];
         for (int k = R - 1; k > 0; --k) {
             a.dims[k] = a.dims[k - 1];
@@ -288,7 +302,12 @@ int rotate(NdArr& a, bool last_to_front, float2*& spare, hipStream_t st)
         a.orig[0] = od;
     } else {
         const int f = a.dims[0];
-        rc = transpose(a.p, 1, f, static_cast<int>(total / f), spare, st);
+        const long long C = total / f;
+        // output row c = (coordinate of the new leading dimension) * inner + ...
+        const long long inner = R >= 2 ? C / a.dims[1] : C;
+        const bool win = keep_hi > keep_lo && R >= 2;
+        rc = transpose(a.p, 1, f, static_cast<int>(C), spare, st, f, win ? static_cast<int>(keep_lo * inner) : 0,
+                       win ? static_cast<int>(keep_hi * inner) : -1);
         const int od = a.orig[0];
         for (int k = 0; k + 1 < R; ++k) {
             a.dims[k] = a.dims[k + 1];
@@ -304,14 +323,20 @@ int rotate(NdArr& a, bool last_to_front, float2*& spare, hipStream_t st)
     return ZKX_SUCCESS;
 }
 
-// Row transforms along the current last dimension when it is convolved.
-int fft_last(NdArr& a, unsigned axes, bool inverse, char* any_ws, hipStream_t st)
+// Row transforms along the current last dimension when it is convolved,
+// over rows [r_lo, r_hi) (r_hi < 0: to the end).
+int fft_last(NdArr& a, unsigned axes, bool inverse, char* any_ws, hipStream_t st, long long r_lo = 0,
+             long long r_hi = -1)
 {
     const int d = a.orig[a.rank - 1];
     const int n = a.dims[a.rank - 1];
     if (!(axes >> d & 1u) || n == 1) return ZKX_SUCCESS;
+    const long long rows = a.count() / n;
+    if (r_hi < 0 || r_hi > rows) r_hi = rows;
+    if (r_hi <= r_lo) return ZKX_SUCCESS;
     const float scale = inverse ? 1.0f / static_cast<float>(n) : 1.0f;
-    return fft_any_rows(a.p, a.count() / n, n, inverse, scale, a.p, any_ws, st);
+    float2* x = a.p + r_lo * n;
+    return fft_any_rows(x, r_hi - r_lo, n, inverse, scale, x, any_ws, st);
 }
 
 // Number of rotations that bring the lowest convolved dimension last.
@@ -325,28 +350,46 @@ int nd_rotations(unsigned axes, int rank)
 // Forward: transform the last dimension, rotate it to the front, and so on
 // until every convolved dimension has been last once (S rotations); the
 // spectrum stays in the rotated order.  skip_final leaves the last
-// dimension untransformed (a fused product kernel does it).
-int forward_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st, bool skip_final = false)
+// dimension untransformed (a fused product kernel does it).  Rows from
+// `live` on (original order) are zeros: the first transform and rotation
+// leave them out.
+int forward_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st, bool skip_final,
+               long long live)
 {
     const int S = nd_rotations(axes, a.rank);
     for (int s = 0; s <= S; ++s) {
-        int rc = s == S && skip_final ? ZKX_SUCCESS : fft_last(a, axes, false, any_ws, st);
+        int rc = s == S && skip_final ? ZKX_SUCCESS : fft_last(a, axes, false, any_ws, st, 0, s == 0 ? live : -1);
         if (rc) return rc;
-        if (s < S && (rc = rotate(a, true, spare, st))) return rc;
+        if (s < S && (rc = rotate(a, true, spare, st, s == 0 ? live : -1))) return rc;
     }
     return ZKX_SUCCESS;
 }
 
 // Inverse of forward_nd's order: transform the last dimension, rotate the
 // first to the back, ..., ending in the original order.  skip_first: the
-// last dimension is already inverse transformed.
-int inverse_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st, bool skip_first = false)
+// last dimension is already inverse transformed.  With keep_hi > keep_lo
+// (rank >= 2) the last rotation and transform produce only the rows whose
+// dimension-0 coordinate is in [keep_lo, keep_hi).
+int inverse_nd(NdArr& a, unsigned axes, float2*& spare, char* any_ws, hipStream_t st, bool skip_first, int keep_lo,
+               int keep_hi)
 {
     const int S = nd_rotations(axes, a.rank);
+    const bool win = keep_hi > keep_lo && a.rank >= 2;
     for (int s = 0; s <= S; ++s) {
-        int rc = s == 0 && skip_first ? ZKX_SUCCESS : fft_last(a, axes, true, any_ws, st);
+        int rc;
+        if (s == 0 && skip_first) {
+            rc = ZKX_SUCCESS;
+        } else if (s == S && win) {
+            // original order: rows of dimension-0 coordinate c are c*inner ..
+            const long long inner = a.count() / a.dims[a.rank - 1] / a.dims[0];
+            rc = fft_last(a, axes, true, any_ws, st, keep_lo * inner, keep_hi * inner);
+        } else {
+            rc = fft_last(a, axes, true, any_ws, st);
+        }
         if (rc) return rc;
-        if (s < S && (rc = rotate(a, false, spare, st))) return rc;
+        if (s < S && (rc = s == S - 1 && win ? rotate(a, false, spare, st, -1, keep_lo, keep_hi)
+                                             : rotate(a, false, spare, st)))
+            return rc;
     }
     return ZKX_SUCCESS;
 }
@@ -362,6 +405,8 @@ int nd_conv_plan(const int* s1, const int* s2, int rank, unsigned axes, NdConvPl
     size_t any = 0;
     for (int d = 0; d < rank; ++d) {
         if (s1[d] < 1 || s2[d] < 1) ZKX_FAIL(ZKX_INVALID_SHAPE, "fft convolve: empty extent");
+        q.s1[d] = s1[d];
+        q.s2[d] = s2[d];
         if (axes >> d & 1u) {
             const long long f = next_pow2_ll(static_cast<long long>(s1[d]) + s2[d] - 1);
             if (f > kMaxAnyFftPow2) ZKX_FAIL(ZKX_INVALID_SHAPE, "fft convolve: axis too long");
@@ -411,9 +456,9 @@ int nd_conv_run(const NdConvPlan& p, char* ws, float2** y, hipStream_t st)
     float2* free_buf = reinterpret_cast<float2*>(ws + p.off_y);
     NdArr a = nd_arr(reinterpret_cast<float2*>(ws + p.off_x1), p.g1, p.rank);
     NdArr b = nd_arr(reinterpret_cast<float2*>(ws + p.off_x2), p.g2, p.rank);
-    int rc = forward_nd(a, p.axes, spare, any, st);
+    int rc = forward_nd(a, p.axes, spare, any, st, false, nd_live_rows(p.s1, p.g1, p.rank));
     if (rc) return rc;
-    if ((rc = forward_nd(b, p.axes, spare, any, st))) return rc;
+    if ((rc = forward_nd(b, p.axes, spare, any, st, false, nd_live_rows(p.s2, p.g2, p.rank)))) return rc;
     // product in the rotated order (the same for both operands)
     NdArr c = a;
     c.p = p.y_is_x1 ? a.p : free_buf;
@@ -433,7 +478,7 @@ int nd_conv_run(const NdConvPlan& p, char* ws, float2** y, hipStream_t st)
     ZKX_LAUNCH_CHECK();
     // b's buffer is free now: it serves the inverse rotations
     float2* spare2 = b.p;
-    if ((rc = inverse_nd(c, p.axes, spare2, any, st))) return rc;
+    if ((rc = inverse_nd(c, p.axes, spare2, any, st, false, p.keep_lo0, p.keep_hi0))) return rc;
     *y = c.p;
     return ZKX_SUCCESS;
 }
@@ -451,7 +496,9 @@ int nd_conv_run_real_pair(const NdConvPlan& p, char* ws, float2** y, hipStream_t
     // after the rotations the last dimension is the lowest convolved one
     const int lo = [&] { int d = 0; while (!(p.axes >> d & 1u)) ++d; return d; }();
     const bool fused = packed_rows_fused(p.g[lo]);
-    int rc = forward_nd(z, p.axes, spare, any, st, fused);
+    int smax[kMaxNdRank];
+    for (int d = 0; d < p.rank; ++d) smax[d] = std::max(p.s1[d], p.s2[d]);
+    int rc = forward_nd(z, p.axes, spare, any, st, fused, nd_live_rows(smax, p.g1, p.rank));
     if (rc) return rc;
     NdArr c = z;
     c.p = reinterpret_cast<float2*>(ws + p.off_y);
@@ -478,7 +525,7 @@ int nd_conv_run_real_pair(const NdConvPlan& p, char* ws, float2** y, hipStream_t
         ZKX_LAUNCH_CHECK();
     }
     float2* spare2 = z.p;  // Z is no longer needed
-    if ((rc = inverse_nd(c, p.axes, spare2, any, st, fused))) return rc;
+    if ((rc = inverse_nd(c, p.axes, spare2, any, st, fused, p.keep_lo0, p.keep_hi0))) return rc;
     *y = c.p;
     return ZKX_SUCCESS;
 }
@@ -605,7 +652,8 @@ int zkx_fftconvolve_nd(const void* in1, const long long* shape1, int complex1, c
         }
         nd_strides(s, rank, pp.sstride);
         pp.reverse = 0;
-        const dim3 grid = nd_grid(count_of(g, rank) / g[rank - 1], g[rank - 1]);
+        pp.rows = nd_pad_rows(p, s, g);
+        const dim3 grid = nd_grid(pp.rows, g[rank - 1]);
         if (cplx)
             hipLaunchKernelGGL(nd_pad_kernel<NdLoadC64>, grid, dim3(256), 0, st,
                                NdLoadC64{static_cast<const float2*>(src)}, pp, X);
@@ -615,6 +663,9 @@ int zkx_fftconvolve_nd(const void* in1, const long long* shape1, int complex1, c
         ZKX_LAUNCH_CHECK();
         return ZKX_SUCCESS;
     };
+    // the crop reads rows [start, start + out) of dimension 0
+    p.keep_lo0 = (q.full[0] - q.out[0]) / 2;
+    p.keep_hi0 = p.keep_lo0 + q.out[0];
     float2* Yp = nullptr;
     bool same_g = true;
     for (int d = 0; d < rank; ++d) same_g = same_g && p.g1[d] == p.g2[d];

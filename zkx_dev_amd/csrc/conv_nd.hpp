@@ -42,11 +42,39 @@ struct NdDims {
 struct NdConvPlan {
     int rank = 0;
     unsigned axes = 0;  // bit d set: dimension d is convolved
+    int s1[kMaxNdRank] = {}, s2[kMaxNdRank] = {};  // the operands' own extents
     int g1[kMaxNdRank] = {}, g2[kMaxNdRank] = {}, g[kMaxNdRank] = {};
     long long n1 = 0, n2 = 0, n = 0;  // elements of X1, X2, Y
     bool y_is_x1 = false;             // the product overwrites X1 (no broadcast into it)
+    // rows of dimension 0 the caller reads from Y, [keep_lo0, keep_hi0) (both
+    // 0: all); with rank >= 2 the last inverse rotation and transform skip
+    // the others, so Y holds only those rows
+    int keep_lo0 = 0, keep_hi0 = 0;
     size_t off_x1 = 0, off_x2 = 0, off_y = 0, off_t = 0, off_any = 0, off_scal = 0, bytes = 0;  // four equal buffers, FFT scratch, scalars
 };
+
+// Rows (of the last dimension, row-major over the others) of a padded
+// operand of extents g that can hold data: its dimension-0 coordinate is
+// below s[0], the operand's own extent (rank 1: the one row).
+inline long long nd_live_rows(const int* s, const int* g, int rank)
+{
+    if (rank < 2) return 1;
+    long long r = s[0];
+    for (int d = 1; d + 1 < rank; ++d) r *= g[d];
+    return r;
+}
+
+// Rows of the padded operand (extents g) the pad kernels must write: the
+// forward part reads only the live rows when a rotation follows the first
+// transform (rows past them are read as zeros), else all rows.
+inline long long nd_pad_rows(const NdConvPlan& p, const int* s, const int* g)
+{
+    int lo = 0;
+    while (!(p.axes >> lo & 1u)) ++lo;
+    long long all = 1;
+    for (int d = 0; d + 1 < p.rank; ++d) all *= g[d];
+    return p.rank - 1 - lo >= 1 ? nd_live_rows(s, g, p.rank) : all;
+}
 
 // Plan for operands of extents s1, s2 (rank 1..8, every extent >= 1, the
 // non-convolved extents equal or 1).  Fails with INVALID_SHAPE when an
@@ -83,6 +111,7 @@ struct NdPad {
     int s[kMaxNdRank];
     long long sstride[kMaxNdRank];
     int reverse;
+    long long rows;  // leading rows of X written (nd_pad_rows)
 };
 
 __device__ __forceinline__ long long nd_rows(const int* ext, int rank)
@@ -102,7 +131,7 @@ __global__ void __launch_bounds__(256) nd_pad_kernel(Load ld, NdPad p, float2* _
     const int last = p.rank - 1;
     const int gl = p.g[last], sl = p.s[last];
     const int j0 = blockIdx.x * kNdSpan + threadIdx.x;
-    const long long rows = nd_rows(p.g, p.rank);
+    const long long rows = p.rows;
     for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
         // leading coordinates of row r (uniform across the block)
         long long q = r, src = 0;
@@ -139,6 +168,7 @@ struct NdPad2 {
     long long sastride[kMaxNdRank], sbstride[kMaxNdRank];
     int reverse_b;
     const unsigned* absmax;
+    long long rows;  // leading rows of Z written (nd_pad_rows)
 };
 
 // log2 of the balancing scale s from the two maxima (0 when either is 0).
@@ -191,7 +221,7 @@ __global__ void __launch_bounds__(256) nd_pad2_kernel(LoadA la, LoadB lb, NdPad2
     const int gl = p.g[last], sla = p.sa[last], slb = p.sb[last];
     const int shift = nd_pair_shift(p.absmax);
     const int j0 = blockIdx.x * kNdSpan + threadIdx.x;
-    const long long rows = nd_rows(p.g, p.rank);
+    const long long rows = p.rows;
     for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
         long long q = r, oa = 0, ob = 0;
         bool ina = true, inb = true;
@@ -282,17 +312,20 @@ int nd_pack_real_pair(const NdConvPlan& p, char* ws, LoadA la, const int* sa, Lo
 {
     const int rank = p.rank;
     unsigned* absmax = reinterpret_cast<unsigned*>(ws + p.off_scal);
-    long long na = 1, nb = 1, rows = 1;
+    long long na = 1, nb = 1;
     NdPad2 pp{};
     pp.rank = rank;
+    int smax[kMaxNdRank];
     for (int d = 0; d < rank; ++d) {
         na *= sa[d];
         nb *= sb[d];
         pp.g[d] = p.g1[d];
         pp.sa[d] = sa[d];
         pp.sb[d] = sb[d];
-        if (d + 1 < rank) rows *= p.g1[d];
+        smax[d] = sa[d] > sb[d] ? sa[d] : sb[d];
     }
+    const long long rows = nd_pad_rows(p, smax, p.g1);
+    pp.rows = rows;
     nd_strides(sa, rank, pp.sastride);
     nd_strides(sb, rank, pp.sbstride);
     pp.reverse_b = reverse_b;

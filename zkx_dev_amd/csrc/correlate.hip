@@ -551,6 +551,8 @@ int correlate_nd_fft(const void* in1, const int* s1, const void* in2, const int*
                                 NdLoadTyped<T>{static_cast<const T*>(in2)}, s2, 1, st)))
         return rc;
     float2* Y = nullptr;
+    p.keep_lo0 = (s1[0] + s2[0] - 1 - out_dims[0]) / 2;  // the crop's rows of dimension 0
+    p.keep_hi0 = p.keep_lo0 + out_dims[0];
     if ((rc = nd_conv_run_real_pair(p, ws, &Y, st))) return rc;
     NdCrop c{};
     c.rank = rank;
