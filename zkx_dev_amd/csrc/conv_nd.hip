@@ -439,7 +439,7 @@ int nd_conv_plan(const int* s1, const int* s2, int rank, unsigned axes, NdConvPl
     q.off_t = 3 * buf;
     q.off_any = 4 * buf;
     q.off_scal = 4 * buf + al256(any);
-    q.bytes = q.off_scal + 256;
+    q.bytes = q.off_scal + 256 + sizeof(float) * 2 * kNdAbsBlocks;
     *p = q;
     return ZKX_SUCCESS;
 }

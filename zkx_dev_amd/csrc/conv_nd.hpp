@@ -50,7 +50,7 @@ struct NdConvPlan {
     // 0: all); with rank >= 2 the last inverse rotation and transform skip
     // the others, so Y holds only those rows
     int keep_lo0 = 0, keep_hi0 = 0;
-    size_t off_x1 = 0, off_x2 = 0, off_y = 0, off_t = 0, off_any = 0, off_scal = 0, bytes = 0;  // four equal buffers, FFT scratch, scalars
+    size_t off_x1 = 0, off_x2 = 0, off_y = 0, off_t = 0, off_any = 0, off_scal = 0, bytes = 0;  // four equal buffers, FFT scratch, scalars (2 words + block maxima)
 };
 
 // Rows (of the last dimension, row-major over the others) of a padded
@@ -183,35 +183,73 @@ __device__ __forceinline__ int nd_pair_shift(const unsigned* absmax)
     return d > 60 ? 60 : (d < -60 ? -60 : d);
 }
 
-// max |re| of two real sources into absmax[0], absmax[1] (float bits; the
-// caller zeroes them first).  Non-negative floats order as their bits.
+// Block maxima of |re| of two real sources: part[2 b + k] for block b and
+// operand k (A: 0, B: 1).  Same-address atomics serialize at tens of ns
+// each, so the blocks store partial maxima and nd_absmax2_reduce_kernel
+// combines them.
+constexpr int kNdAbsBlocks = 512;
 template <class LoadA, class LoadB>
 __global__ void __launch_bounds__(256)
-nd_absmax2_kernel(LoadA la, long long na, LoadB lb, long long nb, unsigned* absmax)
+nd_absmax2_kernel(LoadA la, long long na, LoadB lb, long long nb, float* __restrict__ part)
 {
     float m[2] = {0.0f, 0.0f};
-    const long long stride = static_cast<long long>(gridDim.x) * 256;
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < na; i += stride) m[0] = fmaxf(m[0], fabsf(la(i).x));
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nb; i += stride) m[1] = fmaxf(m[1], fabsf(lb(i).x));
-    // one atomic per block and operand: the waves meet in LDS first
-    __shared__ float part[2][4];
+    // kAbsU independent loads per thread and step keep enough reads in flight
+    constexpr int kAbsU = 8;
+    const long long stride = static_cast<long long>(gridDim.x) * 256 * kAbsU;
+    for (long long i0 = blockIdx.x * 256LL * kAbsU + threadIdx.x; i0 < na; i0 += stride) {
+        float v[kAbsU];
+#pragma unroll
+        for (int u = 0; u < kAbsU; ++u) {
+            const long long i = i0 + u * 256LL;
+            v[u] = i < na ? fabsf(la(i).x) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kAbsU; ++u) m[0] = fmaxf(m[0], v[u]);
+    }
+    for (long long i0 = blockIdx.x * 256LL * kAbsU + threadIdx.x; i0 < nb; i0 += stride) {
+        float v[kAbsU];
+#pragma unroll
+        for (int u = 0; u < kAbsU; ++u) {
+            const long long i = i0 + u * 256LL;
+            v[u] = i < nb ? fabsf(lb(i).x) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kAbsU; ++u) m[1] = fmaxf(m[1], v[u]);
+    }
+    __shared__ float wmax[2][4];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         float v = m[k];
         for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-        if ((threadIdx.x & 63) == 0) part[k][threadIdx.x >> 6] = v;
+        if ((threadIdx.x & 63) == 0) wmax[k][threadIdx.x >> 6] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
-        const float v = fmaxf(fmaxf(part[threadIdx.x][0], part[threadIdx.x][1]),
-                              fmaxf(part[threadIdx.x][2], part[threadIdx.x][3]));
-        if (v > 0.0f) atomicMax(absmax + threadIdx.x, __float_as_uint(v));
-    }
+    if (threadIdx.x < 2)
+        part[2 * blockIdx.x + threadIdx.x] = fmaxf(fmaxf(wmax[threadIdx.x][0], wmax[threadIdx.x][1]),
+                                                   fmaxf(wmax[threadIdx.x][2], wmax[threadIdx.x][3]));
 }
 
-static __global__ void nd_zero2_kernel(unsigned* p)
+// absmax[k] = float bits of the maximum of part[2 b + k] over `blocks` blocks
+// (one block of 256 threads).
+static __global__ void __launch_bounds__(256)
+nd_absmax2_reduce_kernel(const float* __restrict__ part, int blocks, unsigned* __restrict__ absmax)
 {
-    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
+    float m[2] = {0.0f, 0.0f};
+    for (int b = threadIdx.x; b < blocks; b += 256) {
+        m[0] = fmaxf(m[0], part[2 * b]);
+        m[1] = fmaxf(m[1], part[2 * b + 1]);
+    }
+    __shared__ float wmax[2][4];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        float v = m[k];
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+        if ((threadIdx.x & 63) == 0) wmax[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+        absmax[threadIdx.x] = __float_as_uint(fmaxf(fmaxf(wmax[threadIdx.x][0], wmax[threadIdx.x][1]),
+                                                    fmaxf(wmax[threadIdx.x][2], wmax[threadIdx.x][3])));
 }
 
 template <class LoadA, class LoadB>
@@ -330,12 +368,14 @@ int nd_pack_real_pair(const NdConvPlan& p, char* ws, LoadA la, const int* sa, Lo
     nd_strides(sb, rank, pp.sbstride);
     pp.reverse_b = reverse_b;
     pp.absmax = absmax;
-    hipLaunchKernelGGL(nd_zero2_kernel, dim3(1), dim3(64), 0, st, absmax);
-    ZKX_LAUNCH_CHECK();
+    float* part = reinterpret_cast<float*>(ws + p.off_scal + 256);
     const long long nmax = na > nb ? na : nb;
-    // a few blocks per CU, each striding over the operands
-    const unsigned blocks = static_cast<unsigned>((nmax + 255) / 256 < 512 ? (nmax + 255) / 256 : 512);
-    hipLaunchKernelGGL((nd_absmax2_kernel<LoadA, LoadB>), dim3(blocks), dim3(256), 0, st, la, na, lb, nb, absmax);
+    // two blocks per CU at most, striding over the operands 2048 elements at a time
+    const long long want = (nmax + 2047) / 2048;
+    const int blocks = static_cast<int>(want < kNdAbsBlocks ? want : kNdAbsBlocks);
+    hipLaunchKernelGGL((nd_absmax2_kernel<LoadA, LoadB>), dim3(blocks), dim3(256), 0, st, la, na, lb, nb, part);
+    ZKX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(nd_absmax2_reduce_kernel, dim3(1), dim3(256), 0, st, part, blocks, absmax);
     ZKX_LAUNCH_CHECK();
     hipLaunchKernelGGL((nd_pad2_kernel<LoadA, LoadB>), nd_grid(rows, p.g1[rank - 1]), dim3(256), 0, st, la, lb, pp,
                        reinterpret_cast<float2*>(ws + p.off_x1));
