@@ -19,6 +19,8 @@ import torch
 from ._lib import INVALID_ARGUMENT, INVALID_DTYPE, INVALID_SHAPE, ArgumentError, check, lib
 from .radar import _ptr, _stream
 
+_DTYPE_F32 = 0  # ZKX_DTYPE_F32
+
 
 def _lines(shape, axis):
     rank = len(shape)
@@ -128,8 +130,10 @@ def resample_poly(x, up, down, axis=0, *, coefficients=None, window="kaiser", be
     if x.dim() > 2:
         raise ArgumentError(INVALID_SHAPE, "resample_poly nonidentity rank")
     if coefficients is not None and isinstance(coefficients, torch.Tensor) and coefficients.is_cuda and u < 65536:
-        # device-resident taps are scaled on the device (the same float32 product)
-        taps_d = (coefficients.to(torch.float32) * float(np.float32(u))).contiguous()
+        # device-resident float32 taps are scaled by the library (c * up in float32)
+        c = coefficients.to(torch.float32).contiguous()
+        taps_d = torch.empty(c.numel(), dtype=torch.float32, device=x.device)
+        check(lib().zkx_resample_poly_scale_taps(_ptr(c), _DTYPE_F32, c.numel(), u, _ptr(taps_d), _stream(stream)))
         fp64_out = False
     else:
         if coefficients is not None:
