@@ -57,6 +57,8 @@ struct ColLdsSw {
         re[k] = v.x;
         im[k] = v.y;
     }
+    __device__ __forceinline__ float2 load(int base, int c) const { return load(base + c); }
+    __device__ __forceinline__ void store(int base, int c, float2 v) const { store(base + c, v); }
 };
 
 template <int N, int G0, int G1, int R0, int R1>
@@ -97,39 +99,13 @@ struct StripArgs {
     float* power;      // optional |Y|^2 [batch][N][R]
 };
 
-// Raw buffer access: 32-bit byte offsets from a per-dwell base.  A lane whose
-// offset is kOff (beyond every buffer's size) reads zeros and its stores are
-// dropped by the hardware range check, so edge columns need no branches.
-constexpr unsigned kOff = 0x80000000u;
-constexpr int kBufFlags = 0x00020000;  // raw 32-bit data, gfx9 word 3
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0), static_cast<int>(bytes),
-                                             kBufFlags);
-}
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef unsigned u2v __attribute__((ext_vector_type(2)));
-// (whole-vector bit casts: subscripting the builtin's u2v result element by
-// element has been narrowed to a single dword load by this compiler)
-__device__ __forceinline__ float2 ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-    return make_float2(v.x, v.y);
-}
-__device__ __forceinline__ void st2(__amdgpu_buffer_rsrc_t r, float2 v, unsigned voff, unsigned soff)
-{
-    const f2v w = {v.x, v.y};
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, w), r, voff, soff, 0);
-}
-__device__ __forceinline__ void st1(__amdgpu_buffer_rsrc_t r, float v, unsigned voff, unsigned soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
-}
-__device__ __forceinline__ void stb(__amdgpu_buffer_rsrc_t r, unsigned char v, unsigned voff, unsigned soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, 0);
-}
+// raw buffer access (zkx_hip_common.hpp)
+using buf::kOff;
+using buf::ld2;
+using buf::rsrc;
+using buf::st1;
+using buf::st2;
+using buf::stb;
 
 // Samples of column `col` (rows t + e*T) of the dwell behind `src`; zeros
 // outside [0, R).

@@ -83,30 +83,65 @@ pc_template_kernel(const float2* __restrict__ h, int taps, int keep, int normali
 
 // Fused pulse compression of rows: y = IFFT_N(FFT_N(x_pad) * conj(H)) / N.
 // rows = total pulses (batch * P); each row reads min(S, N) samples.
+// Fused pulse compression of rows: y = IFFT_N(FFT_N(x_pad) * conj(H)) / N.
+// rows = total pulses (batch * P); each row reads min(S, N) samples.  The
+// stride-16 pass takes its twiddles from a small LDS table after the slots
+// (both transforms share it); full rows load, and every row stores, through
+// raw buffer instructions with constant per-element offsets.
+template <int N>
+struct PcCfg {
+    using C = RowCfg<N>;
+    static constexpr size_t LDS_BYTES = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<N>::ENTRIES;
+};
+
 template <int N>
 __global__ void __launch_bounds__(RowCfg<N>::THREADS)
 pc_fused_kernel(const float2* __restrict__ x, long long rows, int S, const float2* __restrict__ H,
                 const float2* __restrict__ tw, float2* __restrict__ y)
 {
     using C = RowCfg<N>;
+    // one row per block from N = 4096 up: the row (and the buffer bases) is
+    // uniform across the block
+    constexpr bool ONE = C::FPB == 1;
     extern __shared__ float lds[];
-    const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
+    const int slot = ONE ? 0 : threadIdx.x / C::T, t = ONE ? threadIdx.x : threadIdx.x % C::T;
     const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
     const bool active = row < rows;
-    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    // (RowLdsC::floats_for(N) == 2 * SLOT_FLOATS: the same slot size)
+    const fft::RowLdsC L{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
+    float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS * C::FPB);
+    fft::fill_pass16_twiddles<N>(p16, tw, threadIdx.x, C::THREADS);
+    const fft::Pass16Twiddles TW{tw, p16};
     const float2* src = x + (active ? row : 0) * static_cast<long long>(S);
     float2 v[C::E];
+    if (ONE && S >= N && active) {
+        const auto rs = buf::rsrc(src, N * 8u);
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-        const int i = t + k * C::T;
-        v[k] = (active && i < S) ? src[i] : make_float2(0.f, 0.f);
+        for (int k = 0; k < C::E; ++k) v[k] = buf::ld2(rs, t * 8u, k * C::T * 8u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+            const int i = t + k * C::T;
+            v[k] = (active && i < S) ? src[i] : make_float2(0.f, 0.f);
+        }
     }
-    fft::transform<N, C::E, false>(v, L, tw, t);
+    fft::transform<N, C::E, false>(v, L, TW, t);
+    if constexpr (ONE) {
+        const auto rh = buf::rsrc(H, N * 8u);
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], H[t + k * C::T]);
-    fft::transform<N, C::E, true>(v, L, tw, t);
-    if (active) {
-        const float s = 1.0f / static_cast<float>(N);
+        for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], buf::ld2(rh, t * 8u, k * C::T * 8u));
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], H[t + k * C::T]);
+    }
+    fft::transform<N, C::E, true>(v, L, TW, t);
+    const float s = 1.0f / static_cast<float>(N);
+    if constexpr (ONE) {
+        // an inactive row's stores fall outside its zero-length buffer
+        const auto ry = buf::rsrc(y + (active ? row : 0) * static_cast<long long>(N), active ? N * 8u : 0u);
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) buf::st2(ry, cscale(v[k], s), t * 8u, k * C::T * 8u);
+    } else if (active) {
         float2* dst = y + row * static_cast<long long>(N);
 #pragma unroll
         for (int k = 0; k < C::E; ++k) dst[t + k * C::T] = cscale(v[k], s);
@@ -136,55 +171,52 @@ pc_segments_kernel(const float2* __restrict__ x, long long rows, int nseg, int L
                              lds + slot * 2 * C::SLOT_FLOATS, tw);
 }
 
-// Persistent form: each block walks rows blockIdx.x*FPB+slot, +gridDim.x*FPB,
-// ... with the next row's samples loaded into registers while the current row
-// is transformed, and the template spectrum held in registers for the whole
-// walk, so HBM latency overlaps the two transforms.
+// Persistent form for one row per block (N >= 4096) and full rows (S >= N):
+// each block walks rows blockIdx.x, + gridDim.x, ... with the next row's
+// samples loaded into registers while the current row is transformed, and
+// the template spectrum held in registers for the whole walk, so HBM latency
+// overlaps the two transforms.
 template <int N, int MINW>
 __global__ void __launch_bounds__(RowCfg<N>::THREADS, MINW)
 pc_persistent_kernel(const float2* __restrict__ x, long long rows, int S, const float2* __restrict__ H,
                      const float2* __restrict__ tw, float2* __restrict__ y)
 {
     using C = RowCfg<N>;
+    static_assert(C::FPB == 1, "one row per block");
     extern __shared__ float lds[];
-    const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
-    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
-    const long long stride = static_cast<long long>(gridDim.x) * C::FPB;
-    long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
+    const int t = threadIdx.x;
+    const fft::RowLdsC L{reinterpret_cast<float2*>(lds)};
+    float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS);
+    fft::fill_pass16_twiddles<N>(p16, tw, t, C::THREADS);
+    const fft::Pass16Twiddles TW{tw, p16};
+    const auto rh = buf::rsrc(H, N * 8u);
     float2 h[C::E], nxt[C::E];
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) h[k] = H[t + k * C::T];
+    for (int k = 0; k < C::E; ++k) h[k] = buf::ld2(rh, t * 8u, k * C::T * 8u);
     auto fetch = [&](long long r) {
-        const bool act = r < rows;
-        const float2* src = x + (act ? r : 0) * static_cast<long long>(S);
+        const auto rs = buf::rsrc(x + r * static_cast<long long>(S), N * 8u);
 #pragma unroll
-        for (int k = 0; k < C::E; ++k) {
-            const int i = t + k * C::T;
-            nxt[k] = (act && i < S) ? src[i] : make_float2(0.f, 0.f);
-        }
+        for (int k = 0; k < C::E; ++k) nxt[k] = buf::ld2(rs, t * 8u, k * C::T * 8u);
     };
-    fetch(row);
-    // every slot of the block runs the same number of iterations (barriers)
-    const long long first = static_cast<long long>(blockIdx.x) * C::FPB;
-    for (long long base = first; base < rows; base += stride, row += stride) {
+    long long row = blockIdx.x;
+    if (row < rows) fetch(row);
+    for (; row < rows; row += gridDim.x) {
         float2 v[C::E];
 #pragma unroll
         for (int k = 0; k < C::E; ++k) v[k] = nxt[k];
-        if (base + stride < rows) fetch(row + stride);
+        if (row + gridDim.x < rows) fetch(row + gridDim.x);
         // opaque copy of t: keeps the compiler from hoisting every pass's
         // twiddles out of the row loop into registers
         int tt = t;
         asm volatile("" : "+v"(tt));
-        fft::transform<N, C::E, false>(v, L, tw, tt);
+        fft::transform<N, C::E, false>(v, L, TW, tt);
 #pragma unroll
         for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], h[k]);
-        fft::transform<N, C::E, true>(v, L, tw, tt);
-        if (row < rows) {
-            const float s = 1.0f / static_cast<float>(N);
-            float2* dst = y + row * static_cast<long long>(N);
+        fft::transform<N, C::E, true>(v, L, TW, tt);
+        const auto ry = buf::rsrc(y + row * static_cast<long long>(N), N * 8u);
+        const float s = 1.0f / static_cast<float>(N);
 #pragma unroll
-            for (int k = 0; k < C::E; ++k) dst[t + k * C::T] = cscale(v[k], s);
-        }
+        for (int k = 0; k < C::E; ++k) buf::st2(ry, cscale(v[k], s), t * 8u, k * C::T * 8u);
     }
 }
 
@@ -1064,18 +1096,20 @@ template <int N, int MINW>
 int launch_pc_persistent(const float2* x, long long rows, int S, const float2* H, const float2* tw, float2* y,
                          int per_cu, hipStream_t st)
 {
-    using C = RowCfg<N>;
-    int rc = allow_lds(pc_persistent_kernel<N, MINW>, C::LDS_BYTES);
-    if (rc) return rc;
-    int dev = 0, cus = 256;
-    ZKX_HIP_CHECK(hipGetDevice(&dev));
-    ZKX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const long long need = (rows + C::FPB - 1) / C::FPB;
-    const long long blocks = std::min<long long>(need, static_cast<long long>(cus) * per_cu);
-    hipLaunchKernelGGL((pc_persistent_kernel<N, MINW>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
-                       C::LDS_BYTES, st, x, rows, S, H, tw, y);
-    ZKX_LAUNCH_CHECK();
-    return ZKX_SUCCESS;
+    if constexpr (RowCfg<N>::FPB != 1) {
+        return ZKX_INTERNAL_ERROR;
+    } else {
+        int rc = allow_lds(pc_persistent_kernel<N, MINW>, PcCfg<N>::LDS_BYTES);
+        if (rc) return rc;
+        int dev = 0, cus = 256;
+        ZKX_HIP_CHECK(hipGetDevice(&dev));
+        ZKX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const long long blocks = std::min<long long>(rows, static_cast<long long>(cus) * per_cu);
+        hipLaunchKernelGGL((pc_persistent_kernel<N, MINW>), dim3(static_cast<unsigned>(blocks)),
+                           dim3(RowCfg<N>::THREADS), PcCfg<N>::LDS_BYTES, st, x, rows, S, H, tw, y);
+        ZKX_LAUNCH_CHECK();
+        return ZKX_SUCCESS;
+    }
 }
 
 template <int N>
@@ -1085,13 +1119,16 @@ int launch_pc_fused(const float2* x, long long rows, int S, const float2* H, flo
     const float2* tw = nullptr;
     int rc = twiddles(N, &tw);
     if (rc) return rc;
-    if (g_pc_variant == 1) return launch_pc_persistent<N, 2>(x, rows, S, H, tw, y, 2, st);
-    if (g_pc_variant == 2) return launch_pc_persistent<N, 2>(x, rows, S, H, tw, y, 3, st);
-    if (g_pc_variant == 3) return launch_pc_persistent<N, 1>(x, rows, S, H, tw, y, 1, st);
+    if constexpr (C::FPB == 1) {
+        // persistent forms (autotuner variants) for full rows
+        if (S >= N && g_pc_variant == 1) return launch_pc_persistent<N, 4>(x, rows, S, H, tw, y, 4, st);
+        if (S >= N && g_pc_variant == 2) return launch_pc_persistent<N, 3>(x, rows, S, H, tw, y, 3, st);
+        if (S >= N && g_pc_variant == 3) return launch_pc_persistent<N, 2>(x, rows, S, H, tw, y, 2, st);
+    }
     const long long blocks = (rows + C::FPB - 1) / C::FPB;
-    if ((rc = allow_lds(pc_fused_kernel<N>, C::LDS_BYTES))) return rc;
+    if ((rc = allow_lds(pc_fused_kernel<N>, PcCfg<N>::LDS_BYTES))) return rc;
     hipLaunchKernelGGL((pc_fused_kernel<N>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
-                       C::LDS_BYTES, st, x, rows, S, H, tw, y);
+                       PcCfg<N>::LDS_BYTES, st, x, rows, S, H, tw, y);
     ZKX_LAUNCH_CHECK();
     return ZKX_SUCCESS;
 }

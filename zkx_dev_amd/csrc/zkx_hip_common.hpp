@@ -29,6 +29,42 @@ __host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return ma
 // its archived Task1 power map equals this form in every cell).
 __device__ __forceinline__ float power_of(float2 a) { return fmaf(a.x, a.x, a.y * a.y); }
 
+namespace buf {
+// Raw buffer access: 32-bit byte offsets from a base address.  A lane whose
+// offset is kOff (beyond every buffer's size) reads zeros and its stores are
+// dropped by the hardware range check, so edge columns need no branches.
+constexpr unsigned kOff = 0x80000000u;
+constexpr int kBufFlags = 0x00020000;  // raw 32-bit data, gfx9 word 3
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0), static_cast<int>(bytes),
+                                             kBufFlags);
+}
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+// (whole-vector bit casts: subscripting the builtin's u2v result element by
+// element has been narrowed to a single dword load by this compiler)
+__device__ __forceinline__ float2 ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(__amdgpu_buffer_rsrc_t r, float2 v, unsigned voff, unsigned soff)
+{
+    const f2v w = {v.x, v.y};
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, w), r, voff, soff, 0);
+}
+__device__ __forceinline__ void st1(__amdgpu_buffer_rsrc_t r, float v, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void stb(__amdgpu_buffer_rsrc_t r, unsigned char v, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, 0);
+}
+}  // namespace buf
+
 // Map a HIP runtime status onto the public error numbers.
 inline int hip_status_to_code(hipError_t e)
 {
