@@ -101,7 +101,7 @@ amb_2d_kernel(const float2* __restrict__ x, int nx, const float2* __restrict__ y
         v[k] = d;
     }
     if constexpr (N > 1) {
-        RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+        RowLds L{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
         zkx::fft::transform<N, C::E, true>(v, L, tw, t);
     }
     if (active) {
@@ -124,7 +124,7 @@ amb_delay_kernel(const float2* __restrict__ x, long long x_count, const float2* 
     using C = RowCfg<N>;
     extern __shared__ float lds[];
     const int t = threadIdx.x;
-    RowLds L{lds, lds + C::SLOT_FLOATS};
+    RowLds L{reinterpret_cast<float2*>(lds)};
     const float xn = norms[0], yn = norms[1];
     float2 v[C::E];
 #pragma unroll

@@ -173,7 +173,7 @@ channelize_fused_kernel(const float* __restrict__ x, const float* __restrict__ h
     __syncthreads();
     const int slot = tid / C::T, t = tid % C::T;
     float* base = lds + slot * 2 * C::SLOT_FLOATS;
-    zkx::fft::RowLds L{base, base + C::SLOT_FLOATS};
+    zkx::fft::RowLds L{reinterpret_cast<float2*>(base)};
     float2 v[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) v[k] = make_float2(base[t + k * C::T], 0.0f);

@@ -198,8 +198,8 @@ nd_packed_rows_kernel(const float2* Z, NdPacked p, const float2* __restrict__ tw
     if (pr < r) return;  // the whole block: the partner's block does both rows
     const long long row = slot == 0 ? r : pr;
     const int other = pr == r ? slot : 1 - slot;
-    const fft::RowLds L{lds + slot * 2 * SL, lds + slot * 2 * SL + SL};
-    const fft::RowLds P{lds + other * 2 * SL, lds + other * 2 * SL + SL};
+    const fft::RowLds L{reinterpret_cast<float2*>(lds + slot * 2 * SL)};
+    const fft::RowLds P{reinterpret_cast<float2*>(lds + other * 2 * SL)};
     const float2* src = Z + row * N;
     float2 v[E];
 #pragma unroll

@@ -81,7 +81,7 @@ bluestein_kernel(const float2* __restrict__ x, long long rows, int n, const floa
     const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
     const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
     const bool active = row < rows;
-    zkx::fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    zkx::fft::RowLds L{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
     const float2* src = x + (active ? row : 0) * static_cast<long long>(n);
     float2 v[C::E];
 #pragma unroll

@@ -119,51 +119,21 @@ struct Dft<1, INV> {
     static __device__ __forceinline__ void run(float2*) {}
 };
 
-// One signal per slot, re/im split, one padding word per 16 to spread banks.
-struct RowLds {
-    float* re;
-    float* im;
-    __device__ __forceinline__ int idx(int i) const { return i + (i >> 4); }
-    __device__ __forceinline__ float2 load(int i) const
-    {
-        const int a = idx(i);
-        return make_float2(re[a], im[a]);
-    }
-    __device__ __forceinline__ void store(int i, float2 v) const
-    {
-        const int a = idx(i);
-        re[a] = v.x;
-        im[a] = v.y;
-    }
-    // element base + c, where c is a multiple of 16, or base is a multiple of
-    // 16 and c < 16 (every access of the passes below): the padding splits as
-    // idx(base) + c + c/16, so c folds into the instruction's offset
-    __device__ __forceinline__ float2 load(int base, int c) const
-    {
-        const int a = idx(base) + c + (c >> 4);
-        return make_float2(re[a], im[a]);
-    }
-    __device__ __forceinline__ void store(int base, int c, float2 v) const
-    {
-        const int a = idx(base) + c + (c >> 4);
-        re[a] = v.x;
-        im[a] = v.y;
-    }
-    static constexpr int floats_for(int n) { return n + n / 16; }
-};
-
 // One signal per slot as interleaved complex values (one 8-byte access per
 // element), one padding element per 16: a half-wave of 16 lanes touching 16
-// consecutive elements, or elements 17 apart, covers every bank once.
-struct RowLdsC {
+// consecutive elements, or elements 17 apart, covers every bank once.  A slot
+// takes 2 * floats_for(n) floats.
+struct RowLds {
     float2* p;
     __device__ __forceinline__ static int idx(int i) { return i + (i >> 4); }
     __device__ __forceinline__ float2 load(int i) const { return p[idx(i)]; }
     __device__ __forceinline__ void store(int i, float2 v) const { p[idx(i)] = v; }
-    // element base + c under the same rule as RowLds
+    // element base + c, where c is a multiple of 16, or base is a multiple of
+    // 16 and c < 16 (every access of the passes below): the padding splits as
+    // idx(base) + c + c/16, so c folds into the instruction's offset
     __device__ __forceinline__ float2 load(int base, int c) const { return p[idx(base) + c + (c >> 4)]; }
     __device__ __forceinline__ void store(int base, int c, float2 v) const { p[idx(base) + c + (c >> 4)] = v; }
-    static constexpr int floats_for(int n) { return 2 * (n + n / 16); }
+    static constexpr int floats_for(int n) { return n + n / 16; }
 };
 
 // TC signals interleaved: element i of slot s lives at i*TC + s.

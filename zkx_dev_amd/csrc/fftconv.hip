@@ -37,7 +37,7 @@ ols_kernel(const float2* __restrict__ x, long long rows, int n1, int n2, long lo
     const bool active = item < rows * segs;
     const long long row = active ? item / segs : 0;
     const long long seg = seg0 + (active ? item % segs : 0);
-    fft::RowLds Ld{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    fft::RowLds Ld{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
     const float2* src = x + row * n1;
     const long long in0 = seg * L - (n2 - 1);
     float2 v[C::E];

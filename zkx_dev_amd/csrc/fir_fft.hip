@@ -67,7 +67,7 @@ fir_fft_kernel(const V* __restrict__ x, int rows, int n, int out_n, int D, int K
     const int rpl = (nblk + BL - 1) / BL;  // rows per line
     const int line = active ? row / rpl : 0;
     const int b0 = (active ? row - line * rpl : 0) * BL;  // first block of this row
-    fft::RowLds Ls{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    fft::RowLds Ls{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
     const V* xl = x + static_cast<long long>(line) * n;
     const int base = (b0 * L - (Kmax - 1)) * D;  // input index of segment sample 0, phase 0
     float2 acc[C::E];

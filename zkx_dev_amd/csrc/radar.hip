@@ -62,7 +62,7 @@ pc_template_kernel(const float2* __restrict__ h, int taps, int keep, int normali
     }
     const double scale = normalize ? 1.0 / sqrt(red[0]) : 1.0;
     const int slot = tid / C::T, t = tid % C::T;
-    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    fft::RowLds L{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
     float2 v[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
@@ -107,8 +107,7 @@ pc_fused_kernel(const float2* __restrict__ x, long long rows, int S, const float
     const int slot = ONE ? 0 : threadIdx.x / C::T, t = ONE ? threadIdx.x : threadIdx.x % C::T;
     const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
     const bool active = row < rows;
-    // (RowLdsC::floats_for(N) == 2 * SLOT_FLOATS: the same slot size)
-    const fft::RowLdsC L{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
+    const fft::RowLds L{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
     float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS * C::FPB);
     fft::fill_pass16_twiddles<N>(p16, tw, threadIdx.x, C::THREADS);
     const fft::Pass16Twiddles TW{tw, p16};
@@ -185,7 +184,7 @@ pc_persistent_kernel(const float2* __restrict__ x, long long rows, int S, const 
     static_assert(C::FPB == 1, "one row per block");
     extern __shared__ float lds[];
     const int t = threadIdx.x;
-    const fft::RowLdsC L{reinterpret_cast<float2*>(lds)};
+    const fft::RowLds L{reinterpret_cast<float2*>(lds)};
     float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS);
     fft::fill_pass16_twiddles<N>(p16, tw, t, C::THREADS);
     const fft::Pass16Twiddles TW{tw, p16};
@@ -231,7 +230,7 @@ fft_rows_kernel(const float2* __restrict__ x, long long rows, const float2* __re
     const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
     const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
     const bool active = row < rows;
-    fft::RowLds L{lds + slot * 2 * C::SLOT_FLOATS, lds + slot * 2 * C::SLOT_FLOATS + C::SLOT_FLOATS};
+    fft::RowLds L{reinterpret_cast<float2*>(lds + slot * 2 * C::SLOT_FLOATS)};
     const float2* src = x + (active ? row : 0) * static_cast<long long>(N);
     float2 v[C::E];
 #pragma unroll

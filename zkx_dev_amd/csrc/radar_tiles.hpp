@@ -247,7 +247,7 @@ __device__ __forceinline__ void pc_segment_row(const float2* pulse, int R, int v
                                                int t, float* row_lds, const float2* tw)
 {
     using C = fft::RowCfg<N>;
-    fft::RowLds L{row_lds, row_lds + C::SLOT_FLOATS};
+    fft::RowLds L{reinterpret_cast<float2*>(row_lds)};
     float2 v[C::E];
     // positions k0 + t + k*T reduced mod R by one conditional subtraction per
     // step: callers guarantee k0 < R and T = N/E <= R
