@@ -57,8 +57,32 @@ struct ColLdsSw {
         re[k] = v.x;
         im[k] = v.y;
     }
-    __device__ __forceinline__ float2 load(int base, int c) const { return load(base + c); }
-    __device__ __forceinline__ void store(int base, int c, float2 v) const { store(base + c, v); }
+    // element base + c, where c is a multiple of 16, or base is a multiple of
+    // 16 and c < 16 (every access of the FFT passes): the swizzle bit of
+    // base + c is then bit 4 of base flipped by bit 4 of c, or bit 4 of base,
+    // so the address is one of two per-thread bases plus the constant c*TC
+    __device__ __forceinline__ int a2(int base, int c) const
+    {
+        const int pb = (base >> 4) & 1;
+        if ((c & 15) == 0) {
+            // (base + c) ^ bit == (base ^ bit) + c, bit = pb ^ bit 4 of c
+            return ((base ^ (pb ^ ((c >> 4) & 1))) + c) * TC + slot;
+        }
+        // base % 16 == 0, c < 16: (base + c) ^ pb == base + (c ^ pb), and
+        // c ^ pb is c + pb for even c, c - pb for odd c
+        return (base + ((c & 1) ? -pb : pb) + c) * TC + slot;
+    }
+    __device__ __forceinline__ float2 load(int base, int c) const
+    {
+        const int k = a2(base, c);
+        return make_float2(re[k], im[k]);
+    }
+    __device__ __forceinline__ void store(int base, int c, float2 v) const
+    {
+        const int k = a2(base, c);
+        re[k] = v.x;
+        im[k] = v.y;
+    }
 };
 
 template <int N, int G0, int G1, int R0, int R1>
