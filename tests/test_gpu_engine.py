@@ -12,9 +12,11 @@ pytestmark = pytest.mark.gpu
 
 GUARD, REF, PFA = (2, 2), (8, 8), 1e-6
 # (P, R, T): power-of-two R (one circular FFT per pulse) and R = 256*k with
-# k not a power of two (overlap-save segments), P from 16 to 256
+# k not a power of two (overlap-save segments), P from 16 to 256; the last two
+# have range lengths that are not a multiple of any strip chunk (partial last
+# chunk and strip in the strip Doppler+CFAR jobs, P = 64 and P = 256)
 MIX = [(32, 1024, 64), (64, 1280, 200), (128, 2048, 512), (16, 3328, 1000), (256, 1024, 100),
-       (128, 4096, 256), (64, 1792, 64), (32, 16384 - 256, 1024)]
+       (128, 4096, 256), (64, 1792, 64), (32, 16384 - 256, 1024), (64, 1000, 100), (256, 2000, 300)]
 
 
 @pytest.fixture(scope="module")

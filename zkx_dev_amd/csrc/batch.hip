@@ -16,14 +16,21 @@
 //     pulse (tiles::pc_segment_row), which gives the length-R circular
 //     correlation exactly.  A job is 8192 samples of FFT work: 8192/N row
 //     segments of one dwell, so jobs cost about the same at every N.
-//   * Doppler + CFAR: tiles of 64 range bins (tiles::doppler_cfar_tile); a
-//     job packs 256/P tiles of one dwell into a 1024-thread block.
+//   * Doppler + CFAR with the engine's default window (guard 2x2, reference
+//     8x8) for P = 32 .. 256: strips of range bins (strip::doppler_cfar_strip,
+//     each column transformed once), 8 chunks per job; P <= 128 in 256-thread
+//     blocks, P = 256 in 512-thread blocks.  Other windows and P = 16: tiles
+//     of 64 range bins (tiles::doppler_cfar_tile, halo columns transformed by
+//     both neighbours); a job packs 256/P tiles of one dwell into a
+//     1024-thread block.
 //   * Both grids are persistent and a multiple of 8; each of the 8 XCDs takes
 //     a contiguous range of jobs, so CFAR tiles that share halo columns run on
 //     one XCD and share its L2.
 //   * Job tables and the run's pointers live in one device blob per lane,
 //     copied from pinned host staging before each run, so the captured graph
-//     (two kernel nodes) never changes.
+//     (four kernel nodes: pulse compression, tiles, strips, P = 256 strips)
+//     never changes; a node without jobs returns at once.
+#include "dc_strip.hpp"
 #include "radar_tiles.hpp"
 #include "runtime.hpp"
 
@@ -48,6 +55,9 @@ constexpr int kDcThreads = 1024;
 constexpr int kMinLog = 4, kMaxLog = 13;  // FFT lengths 16 .. 8192
 constexpr int kMinDopplerLog = 4, kMaxDopplerLog = 8;  // P = 16 .. 256
 constexpr int kSlots = 4;
+// strip Doppler+CFAR jobs: P = 32..128 in 256-thread blocks, P = 256 in 512
+constexpr int kDcsThreads = 256, kDcs256Threads = 512;
+constexpr int kStripChunksPerJob = 8;
 // every pulse-compression job holds 8192 samples: 8192/N rows of 2*(N + N/16) floats
 constexpr size_t kPcLdsBytes = sizeof(float) * 2 * (8192 + 8192 / 16);
 
@@ -72,7 +82,9 @@ struct RunArgs {
     const DwellDev* dwells;
     const Job* pc_jobs;
     const Job* dc_jobs;
-    int n_pc, n_dc;
+    const Job* dcs_jobs;     // strips, P = 32..128 (first = first output bin, count = bins)
+    const Job* dcs256_jobs;  // strips, P = 256
+    int n_pc, n_dc, n_dcs, n_dcs256;
     int window, g0, g1, r0, r1;
     double alpha_over_n;
     const float2* tw[kMaxLog + 1];
@@ -189,6 +201,93 @@ __global__ void __launch_bounds__(kDcThreads) dc_engine_kernel(const RunArgs* __
     }
 }
 
+// Values every lane of the block holds alike (job and dwell fields), moved
+// to scalar registers: the strip body builds buffer descriptors from them,
+// which must be scalar (a per-lane descriptor costs a loop per access).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni(long long v)
+{
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<unsigned long long>(v) >> 32));
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    return reinterpret_cast<T*>(uni(static_cast<long long>(reinterpret_cast<uintptr_t>(p))));
+}
+__device__ __forceinline__ double uni(double v) { return __builtin_bit_cast(double, uni(__builtin_bit_cast(long long, v))); }
+
+// One strip of a dwell: output bins [first, first + count), all P rows.
+template <int N, int NT>
+__device__ __forceinline__ void dcs_job(const RunArgs& ra, const Job& job, float* lds)
+{
+    const DwellDev d = ra.dwells[uni(job.dwell)];
+    const long long w_off = uni(d.w_off), o_off = uni(d.o_off);
+    strip::StripArgs a;
+    a.x = uni(ra.ws) + w_off;
+    a.R = uni(d.R);
+    a.window = uni(ra.window);
+    a.SW = uni(job.count);
+    a.alpha_over_n = uni(ra.alpha_over_n);
+    a.tw = uni(ra.tw[ilog2_c(N)]);
+    a.thr = uni(ra.thr) + o_off;
+    a.det = uni(ra.det) + o_off;
+    float2* rd = uni(ra.rd);
+    a.y = rd ? rd + o_off : nullptr;
+    a.power = nullptr;
+    const int first = uni(job.first);
+    // opaque thread index: keeps the compiler from hoisting each body's
+    // thread-dependent setup out of the job loop (it would stay live in
+    // registers for all three bodies)
+    int tix = threadIdx.x;
+    asm volatile("" : "+v"(tix));
+    strip::doppler_cfar_strip<N, 2, 2, 8, 8, NT>(a, 0, first, first + a.SW, lds, tix);
+}
+
+__global__ void __launch_bounds__(kDcsThreads, 2) dcs_engine_kernel(const RunArgs* __restrict__ rap)
+{
+    extern __shared__ float lds[];
+    const RunArgs& ra = *rap;
+    int j, step, end;
+    xcd_range(ra.n_dcs, j, step, end);
+    for (; j < end; j += step) {
+        const Job job = ra.dcs_jobs[j];
+        switch (job.nlog) {
+        case 5: dcs_job<32, kDcsThreads>(ra, job, lds); break;
+        case 6: dcs_job<64, kDcsThreads>(ra, job, lds); break;
+        case 7: dcs_job<128, kDcsThreads>(ra, job, lds); break;
+        default: break;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kDcs256Threads) dcs256_engine_kernel(const RunArgs* __restrict__ rap)
+{
+    extern __shared__ float lds[];
+    const RunArgs& ra = *rap;
+    int j, step, end;
+    xcd_range(ra.n_dcs256, j, step, end);
+    for (; j < end; j += step) dcs_job<256, kDcs256Threads>(ra, ra.dcs256_jobs[j], lds);
+}
+
+template <int N, int NT>
+using EngineStrip = strip::StripCfg<N, 2, 2, 8, 8, NT>;
+constexpr size_t kDcsLdsBytes = std::max({EngineStrip<32, kDcsThreads>::LDS_BYTES, EngineStrip<64, kDcsThreads>::LDS_BYTES,
+                                          EngineStrip<128, kDcsThreads>::LDS_BYTES});
+constexpr size_t kDcs256LdsBytes = EngineStrip<256, kDcs256Threads>::LDS_BYTES;
+
+// The strip body's buffer offsets are 32-bit: a dwell's P*R*8 bytes stay below 2^31.
+bool strip_dwell(bool fixed, int P, int R)
+{
+    return fixed && P >= 32 && P <= 256 && static_cast<long long>(P) * R * 8 < (1LL << 31);
+}
+int strip_width(int P)
+{
+    const int cw = P == 256 ? EngineStrip<256, kDcs256Threads>::CW : kDcsThreads / (P / 16);
+    return kStripChunksPerJob * cw;
+}
+
 // tiles of 64 range bins per Doppler job for P pulses, and the LDS one job needs
 int dc_tiles_per_job(int nlog) { return kDcThreads / (4 << nlog); }
 
@@ -241,6 +340,7 @@ struct zkx_engine {
     bool fixed_window = false;
     bool use_graph = true;
     int pc_grid = 0, dc_grid = 0;
+    int dcs_grid = 0, dcs256_grid = 0;  // strip kernels: two and one resident blocks per CU
     std::vector<zkx::TemplateRec> templates;
     std::vector<zkx::Lane> lanes;
     std::map<std::pair<int, int>, int> pc_choice;  // (R, T) -> log2 of the FFT length
@@ -331,6 +431,8 @@ int capture_graph(zkx_engine& e, Lane& lane)
             hipLaunchKernelGGL(dc_engine_kernel<true>, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, cs, ra);
         else
             hipLaunchKernelGGL(dc_engine_kernel<false>, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, cs, ra);
+        hipLaunchKernelGGL(dcs_engine_kernel, dim3(e.dcs_grid), dim3(kDcsThreads), kDcsLdsBytes, cs, ra);
+        hipLaunchKernelGGL(dcs256_engine_kernel, dim3(e.dcs256_grid), dim3(kDcs256Threads), kDcs256LdsBytes, cs, ra);
         err = hipStreamEndCapture(cs, &g);
     }
     if (err == hipSuccess) err = hipGraphInstantiate(&lane.exec, g, nullptr, nullptr, 0);
@@ -363,7 +465,7 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
 
     // ---- plan ----
     std::vector<DwellDev> dw(n);
-    std::vector<Job> pcj, dcj;
+    std::vector<Job> pcj, dcj, dcsj, dcs256j;
     long long ws_need = 0;
     const int cm = e.g1 + e.r1;
     const int tco = tiles::FusedCfg<16>::NCM - 2 * cm;
@@ -401,9 +503,15 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
             for (long long f = 0; f < rows; f += fpb)
                 pcj.push_back({i, nl, static_cast<int>(f), static_cast<int>(std::min<long long>(fpb, rows - f))});
             const int plog = ilog2(o.P);
-            const int tiles_n = (o.R + tco - 1) / tco;
-            const int per = dc_tiles_per_job(plog);
-            for (int t0 = 0; t0 < tiles_n; t0 += per) dcj.push_back({i, plog, t0, std::min(per, tiles_n - t0)});
+            if (strip_dwell(e.fixed_window, o.P, o.R)) {
+                const int sw = strip_width(o.P);
+                std::vector<Job>& dst = o.P == 256 ? dcs256j : dcsj;
+                for (int c0 = 0; c0 < o.R; c0 += sw) dst.push_back({i, plog, c0, std::min(sw, o.R - c0)});
+            } else {
+                const int tiles_n = (o.R + tco - 1) / tco;
+                const int per = dc_tiles_per_job(plog);
+                for (int t0 = 0; t0 < tiles_n; t0 += per) dcj.push_back({i, plog, t0, std::min(per, tiles_n - t0)});
+            }
         }
     }
 
@@ -422,7 +530,9 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
     const size_t off_dw = align16(sizeof(RunArgs));
     const size_t off_pc = align16(off_dw + sizeof(DwellDev) * dw.size());
     const size_t off_dc = align16(off_pc + sizeof(Job) * pcj.size());
-    const size_t bytes = align16(off_dc + sizeof(Job) * dcj.size());
+    const size_t off_dcs = align16(off_dc + sizeof(Job) * dcj.size());
+    const size_t off_dcs256 = align16(off_dcs + sizeof(Job) * dcsj.size());
+    const size_t bytes = align16(off_dcs256 + sizeof(Job) * dcs256j.size());
     if (bytes > lane.dev_cap) {
         ZKX_HIP_CHECK(hipStreamSynchronize(st));
         if (lane.dev) (void)hipFree(lane.dev);
@@ -458,8 +568,12 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
     ra.dwells = reinterpret_cast<const DwellDev*>(lane.dev + off_dw);
     ra.pc_jobs = reinterpret_cast<const Job*>(lane.dev + off_pc);
     ra.dc_jobs = reinterpret_cast<const Job*>(lane.dev + off_dc);
+    ra.dcs_jobs = reinterpret_cast<const Job*>(lane.dev + off_dcs);
+    ra.dcs256_jobs = reinterpret_cast<const Job*>(lane.dev + off_dcs256);
     ra.n_pc = static_cast<int>(pcj.size());
     ra.n_dc = static_cast<int>(dcj.size());
+    ra.n_dcs = static_cast<int>(dcsj.size());
+    ra.n_dcs256 = static_cast<int>(dcs256j.size());
     ra.window = e.dop_window;
     ra.g0 = e.g0;
     ra.g1 = e.g1;
@@ -471,6 +585,8 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
     std::memcpy(h + off_dw, dw.data(), sizeof(DwellDev) * dw.size());
     if (!pcj.empty()) std::memcpy(h + off_pc, pcj.data(), sizeof(Job) * pcj.size());
     if (!dcj.empty()) std::memcpy(h + off_dc, dcj.data(), sizeof(Job) * dcj.size());
+    if (!dcsj.empty()) std::memcpy(h + off_dcs, dcsj.data(), sizeof(Job) * dcsj.size());
+    if (!dcs256j.empty()) std::memcpy(h + off_dcs256, dcs256j.data(), sizeof(Job) * dcs256j.size());
     ZKX_HIP_CHECK(hipMemcpyAsync(lane.dev, h, bytes, hipMemcpyHostToDevice, st));
 
     // ---- launch ----
@@ -489,6 +605,15 @@ int engine_run(zkx_engine& e, int lane_idx, const float2* x, const zkx_dwell_des
         else
             hipLaunchKernelGGL(dc_engine_kernel<false>, dim3(e.dc_grid), dim3(kDcThreads), e.dc_lds, st, rap);
         ZKX_LAUNCH_CHECK();
+        if (!dcsj.empty()) {
+            hipLaunchKernelGGL(dcs_engine_kernel, dim3(e.dcs_grid), dim3(kDcsThreads), kDcsLdsBytes, st, rap);
+            ZKX_LAUNCH_CHECK();
+        }
+        if (!dcs256j.empty()) {
+            hipLaunchKernelGGL(dcs256_engine_kernel, dim3(e.dcs256_grid), dim3(kDcs256Threads), kDcs256LdsBytes, st,
+                               rap);
+            ZKX_LAUNCH_CHECK();
+        }
     }
     ZKX_HIP_CHECK(hipEventRecord(lane.ev[s], st));
     lane.ev_live[s] = true;
@@ -570,6 +695,8 @@ int zkx_engine_create(zkx_engine** out, int pc_window, int normalize, int dopple
         e->dc_lds = lds;
         e->pc_grid = zkx::round8(2 * e->cus);
         e->dc_grid = zkx::round8(e->cus);
+        e->dcs_grid = zkx::round8(2 * e->cus);
+        e->dcs256_grid = zkx::round8(e->cus);
         for (int l = zkx::kMinLog; l <= zkx::kMaxLog && !rc; ++l) rc = zkx::twiddles(1 << l, &e->tw[l]);
         if (rc) break;
         e->fixed_window = guard_doppler == 2 && guard_range == 2 && ref_doppler == 8 && ref_range == 8;
@@ -581,6 +708,13 @@ int zkx_engine_create(zkx_engine** out, int pc_window, int normalize, int dopple
         if (err == hipSuccess)
             err = hipFuncSetAttribute(reinterpret_cast<const void*>(zkx::pc_engine_kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(zkx::kPcLdsBytes));
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute(reinterpret_cast<const void*>(zkx::dcs_engine_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(zkx::kDcsLdsBytes));
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute(reinterpret_cast<const void*>(zkx::dcs256_engine_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(zkx::kDcs256LdsBytes));
         if (err != hipSuccess) {
             rc = zkx::hip_status_to_code(err);
             set_last_error(rc, hipGetErrorString(err));

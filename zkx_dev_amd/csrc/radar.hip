@@ -913,7 +913,7 @@ __global__ void __launch_bounds__(256, 2) doppler_cfar_strip_kernel(strip::Strip
     extern __shared__ float lds[];
     const int S0 = blockIdx.x * a.SW;
     const int S1 = min(S0 + a.SW, a.R);
-    strip::doppler_cfar_strip<N, G0, G1, R0, R1>(a, blockIdx.y, S0, S1, lds);
+    strip::doppler_cfar_strip<N, G0, G1, R0, R1>(a, blockIdx.y, S0, S1, lds, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
