@@ -660,6 +660,21 @@ def probe(args):
         filtering.firfilter(h_lp, xr)
         filtering.fftconvolve(xs, h_1025, "same")
     torch.cuda.synchronize()
+    del xs, xr, x, pc
+    # the uneven batch (256 dwells) through the engine, direct launches with
+    # the default FFT lengths and grids
+    from zkx_dev_amd import engine
+    specs = synth.uneven_shapes(256, seed=20261205)
+    ub = synth.make_uneven_batch_device(specs, seed=20261206)
+    eng = engine.DwellEngine(pc_window="hamming", normalize=True, doppler_window="hann", guard=GUARD,
+                             reference=REF, pfa=PFA, lanes=1)
+    tids = [eng.add_template(torch.from_numpy(t)) for t in ub.templates]
+    descs, total = engine.make_descs(ub.shapes, tids)
+    thr = torch.empty(total, dtype=torch.float32, device=dev)
+    det = torch.empty(total, dtype=torch.uint8, device=dev)
+    for _ in range(3):
+        eng.run(ub.x, descs, thr, det)
+    torch.cuda.synchronize()
     print("probe done", flush=True)
 
 
@@ -674,7 +689,25 @@ PROBED = {
     "polyphase_kernel (resample_poly 3/2, 121 taps)": ("polyphase_kernel", 16 * (1 << 20) * 8 + 16 * 1572864 * 8),
     "fir_fft_kernel (upfirdn 1/4 c64 + firfilter f32, mean)": ("fir_fft_kernel", None),
     "ols_kernel (fftconvolve same, 1025 taps)": ("ols_kernel", 16 * (1 << 20) * 16),
+    # the engine's kernels on the 256-dwell uneven batch (bytes: see uneven_alg_bytes)
+    "pc_engine_kernel (256 uneven dwells)": ("pc_engine_kernel", "pc"),
+    "dcs_engine_kernel (256 uneven dwells, P 32..128)": ("dcs_engine_kernel", "dcs"),
+    "dcs256_engine_kernel (256 uneven dwells, P 256)": ("dcs256_engine_kernel", "dcs256"),
 }
+
+
+def uneven_alg_bytes(kind):
+    """Algorithmic bytes of one engine run over the 256-dwell uneven batch:
+    pulse compression 16 B per sample plus the template spectra, the strip
+    Doppler+CFAR 13 B per sample of its dwells."""
+    from zkx_dev_amd import synth
+    specs = synth.uneven_shapes(256, seed=20261205)
+    shapes = [(s[0], s[1]) for s in specs]
+    if kind == "pc":
+        return sum(16 * p * r for p, r in shapes) + sum(8 * s[2] for s in specs)
+    if kind == "dcs":
+        return sum(13 * p * r for p, r in shapes if 32 <= p <= 128)
+    return sum(13 * p * r for p, r in shapes if p == 256)
 
 
 def parse_counters(paths, out_json):
@@ -710,7 +743,9 @@ def parse_counters(paths, out_json):
                            "fetch_factor": kf, "write_factor": kw},
            "kernels": {}}
     for key, (needle, alg) in PROBED.items():
-        rec = {"batch": 64 if "dwell" in key or "cfar" in key or "pc_" in key else None}
+        if isinstance(alg, str):
+            alg = uneven_alg_bytes(alg)
+        rec = {"batch": 64 if ("dwell" in key or "cfar" in key or "pc_" in key) and "engine" not in key else None}
         fk, wk, va = mean_for("FETCH_SIZE", needle), mean_for("WRITE_SIZE", needle), mean_for("SQ_INSTS_VALU", needle)
         if fk is not None and wk is not None and kf and kw:
             rec.update({"fetch_kib": fk, "write_kib": wk, "hbm_bytes_per_launch": fk * 1024.0 * kf + wk * 1024.0 * kw})

@@ -58,8 +58,16 @@ constexpr int kSlots = 4;
 // strip Doppler+CFAR jobs: P = 32..128 in 256-thread blocks, P = 256 in 512
 constexpr int kDcsThreads = 256, kDcs256Threads = 512;
 constexpr int kStripChunksPerJob = 8;
-// every pulse-compression job holds 8192 samples: 8192/N rows of 2*(N + N/16) floats
-constexpr size_t kPcLdsBytes = sizeof(float) * 2 * (8192 + 8192 / 16);
+// every pulse-compression job holds 8192 samples: 8192/N rows of 2*(N + N/16)
+// floats, then the stride-16 pass twiddles of radix 2, 4, 8 and 16 (the
+// tables of every N: entry (r-1)*16 + jm = exp(-2 pi i r jm / (16 R)))
+constexpr size_t kPcRowBytes = sizeof(float) * 2 * (8192 + 8192 / 16);
+constexpr int kP16Entries = 16 * (1 + 3 + 7 + 15);
+constexpr size_t kPcLdsBytes = kPcRowBytes + sizeof(float2) * kP16Entries;
+__host__ __device__ constexpr int p16_offset(int radix)
+{
+    return radix <= 2 ? 0 : radix == 4 ? 16 : radix == 8 ? 64 : 176;
+}
 
 struct DwellDev {
     long long x_off;  // complex64 elements: input dwell [P][R]
@@ -101,6 +109,23 @@ __device__ __forceinline__ void xcd_range(int njobs, int& first, int& step, int&
     step = g >> 3;
 }
 
+// Values every lane of the block holds alike (job and dwell fields), moved
+// to scalar registers: the strip body builds buffer descriptors from them,
+// which must be scalar (a per-lane descriptor costs a loop per access).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni(long long v)
+{
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<unsigned long long>(v) >> 32));
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    return reinterpret_cast<T*>(uni(static_cast<long long>(reinterpret_cast<uintptr_t>(p))));
+}
+__device__ __forceinline__ double uni(double v) { return __builtin_bit_cast(double, uni(__builtin_bit_cast(long long, v))); }
+
 template <int N>
 __device__ __forceinline__ void pc_job(const RunArgs& ra, const Job& job, float* lds)
 {
@@ -111,22 +136,37 @@ __device__ __forceinline__ void pc_job(const RunArgs& ra, const Job& job, float*
     int tix = threadIdx.x;
     asm volatile("" : "+v"(tix));
     const int slot = tix / T, t = tix % T;
-    const DwellDev d = ra.dwells[job.dwell];
-    const bool active = slot < job.count;
-    const int rs = job.first + (active ? slot : 0);
-    const int p = rs / d.S, seg = rs - p * d.S;
-    const int k0 = seg * d.L;
-    const int count = min(d.L, d.R - k0);
-    const float2* pulse = ra.x + d.x_off + static_cast<long long>(p) * d.R;
-    float2* out = ra.ws + d.w_off + static_cast<long long>(p) * d.R;
-    tiles::pc_segment_row<N>(pulse, d.R, d.R, k0, count, active, d.H, out, t, lds + slot * 2 * C::SLOT_FLOATS,
-                             ra.tw[ilog2_c(N)]);
+    const DwellDev& d = ra.dwells[uni(job.dwell)];
+    const int dS = uni(d.S), dL = uni(d.L), dR = uni(d.R);
+    const int jcount = uni(job.count), jfirst = uni(job.first);
+    const bool active = slot < jcount;
+    const int rs = jfirst + (active ? slot : 0);
+    const int p = rs / dS, seg = rs - p * dS;
+    const int k0 = seg * dL;
+    const int count = min(dL, dR - k0);
+    const float2* pulse = uni(ra.x) + uni(d.x_off) + static_cast<long long>(p) * dR;
+    float2* out = uni(ra.ws) + uni(d.w_off) + static_cast<long long>(p) * dR;
+    const float2* p16 = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(lds) + kPcRowBytes);
+    const fft::Pass16Twiddles tw{uni(ra.tw[ilog2_c(N)]), p16 + p16_offset(fft::Pass16Table<N>::R)};
+    tiles::pc_segment_row<N>(pulse, dR, dR, k0, count, active, uni(d.H), out, t, lds + slot * 2 * C::SLOT_FLOATS,
+                             tw);
 }
 
 __global__ void __launch_bounds__(kPcThreads, 4) pc_engine_kernel(const RunArgs* __restrict__ rap)
 {
     extern __shared__ float lds[];
     const RunArgs& ra = *rap;
+    {
+        // the stride-16 pass tables of radix 2, 4, 8, 16 (from the 16R-point tables)
+        float2* p16 = reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + kPcRowBytes);
+        for (int e = threadIdx.x; e < kP16Entries; e += blockDim.x) {
+            const int radix = e < 16 ? 2 : e < 64 ? 4 : e < 176 ? 8 : 16;
+            const int le = e - p16_offset(radix);
+            const int r = le / 16 + 1, jm = le % 16;
+            p16[e] = ra.tw[ilog2_c(16 * radix)][r * jm];
+        }
+        __syncthreads();
+    }
     int j, step, end;
     xcd_range(ra.n_pc, j, step, end);
     for (; j < end; j += step) {
@@ -200,23 +240,6 @@ __global__ void __launch_bounds__(kDcThreads) dc_engine_kernel(const RunArgs* __
         }
     }
 }
-
-// Values every lane of the block holds alike (job and dwell fields), moved
-// to scalar registers: the strip body builds buffer descriptors from them,
-// which must be scalar (a per-lane descriptor costs a loop per access).
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ long long uni(long long v)
-{
-    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
-    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<unsigned long long>(v) >> 32));
-    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
-}
-template <class T>
-__device__ __forceinline__ T* uni(T* p)
-{
-    return reinterpret_cast<T*>(uni(static_cast<long long>(reinterpret_cast<uintptr_t>(p))));
-}
-__device__ __forceinline__ double uni(double v) { return __builtin_bit_cast(double, uni(__builtin_bit_cast(long long, v))); }
 
 // One strip of a dwell: output bins [first, first + count), all P rows.
 template <int N, int NT>

@@ -166,8 +166,10 @@ pc_segments_kernel(const float2* __restrict__ x, long long rows, int nseg, int L
     const int seg = static_cast<int>(r - p * nseg);
     const int k0 = seg * L;
     const int count = min(L, nfft - k0);
+    float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS * C::FPB);
+    fft::fill_pass16_twiddles<N>(p16, tw, threadIdx.x, C::THREADS);
     tiles::pc_segment_row<N>(x + p * S, nfft, valid, k0, count, active, H, y + p * nfft, t,
-                             lds + slot * 2 * C::SLOT_FLOATS, tw);
+                             lds + slot * 2 * C::SLOT_FLOATS, fft::Pass16Twiddles{tw, p16});
 }
 
 // Persistent form for one row per block (N >= 4096) and full rows (S >= N):
@@ -1067,10 +1069,10 @@ int launch_pc_segments(const float2* x, long long rows, int nseg, int L, int nff
     const float2* tw = nullptr;
     int rc = twiddles(N, &tw);
     if (rc) return rc;
-    if ((rc = allow_lds(pc_segments_kernel<N>, C::LDS_BYTES))) return rc;
+    if ((rc = allow_lds(pc_segments_kernel<N>, PcCfg<N>::LDS_BYTES))) return rc;
     const long long blocks = (rows + C::FPB - 1) / C::FPB;
-    hipLaunchKernelGGL((pc_segments_kernel<N>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS), C::LDS_BYTES,
-                       st, x, rows, nseg, L, nfft, valid, S, H, tw, y);
+    hipLaunchKernelGGL((pc_segments_kernel<N>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
+                       PcCfg<N>::LDS_BYTES, st, x, rows, nseg, L, nfft, valid, S, H, tw, y);
     ZKX_LAUNCH_CHECK();
     return ZKX_SUCCESS;
 }

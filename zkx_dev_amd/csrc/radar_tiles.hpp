@@ -238,13 +238,15 @@ __device__ __forceinline__ void doppler_cfar_tile(const DcTile& a, int tile, int
 // conj(H) (H = FFT_N of the windowed, normalised template) and transformed
 // back; outputs i < count are exact (an overlap-save segment).  With N == R
 // and k0 == 0 this is the plain circular correlation of length R.
-// `t` indexes the T = N/E threads of the row; `row_lds` is the row's region.
+// `t` indexes the T = N/E threads of the row; `row_lds` is the row's region;
+// tw is a twiddle source of fft::transform (the N-point table, or it with an
+// LDS table for the stride-16 pass).
 // (No __restrict__ on these pointers: with it the scheduler hoists the
 // spectrum loads above the forward transform and the row needs ~200 VGPRs.)
-template <int N>
+template <int N, class TW>
 __device__ __forceinline__ void pc_segment_row(const float2* pulse, int R, int valid, int k0, int count,
                                                bool active, const float2* H, float2* out,
-                                               int t, float* row_lds, const float2* tw)
+                                               int t, float* row_lds, TW tw)
 {
     using C = fft::RowCfg<N>;
     fft::RowLds L{reinterpret_cast<float2*>(row_lds)};
