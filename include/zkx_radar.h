@@ -67,7 +67,12 @@ int zkx_version(void);
 int zkx_max_lds_fft_length(void);
 /* Kernel variant used for an operation ("pulse_compression", ...), as chosen
  * by the autotuner; variants compute identical results.  No counterpart in
- * the reference (its launches have one fixed shape).  One more name is a
+ * the reference (its launches have one fixed shape).  "dwell_fused" selects
+ * how zkx_dwell_batch_c64 runs cfar_mode 1 for R = 4096, P = 32..128 dwells:
+ * 0 = one persistent launch (pulse compression jobs and strip Doppler+CFAR
+ * jobs from one queue; "dwell_fused_rows", "dwell_fused_chunks" and
+ * "dwell_fused_lead" set its job sizes and lead), 1 = two launches (the
+ * default, faster on this GPU).  One more name is a
  * test switch, not a variant: "inject_fft_failure" = 1 makes the transform
  * stage of pulse_compression / pulse_doppler return ZKX_BACKEND_ERROR
  * before any device work (the reference's synthesized backend status for

@@ -133,19 +133,19 @@ using buf::stb;
 
 // Samples of column `col` (rows t + e*T) of the dwell behind `src`; zeros
 // outside [0, R).
-template <int N, int E, int T>
+template <int N, int E, int T, int AUX>
 __device__ __forceinline__ void strip_load(__amdgpu_buffer_rsrc_t src, int R, int col, int t, float2 (&buf)[E])
 {
     const unsigned voff = (col >= 0 && col < R) ? (static_cast<unsigned>(t) * R + col) * 8u : kOff;
 #pragma unroll
-    for (int e = 0; e < E; ++e) buf[e] = ld2(src, voff, static_cast<unsigned>(e) * T * R * 8u);
+    for (int e = 0; e < E; ++e) buf[e] = ld2<AUX>(src, voff, static_cast<unsigned>(e) * T * R * 8u);
 }
 
 // The whole strip of dwell `dwell`, outputs [S0, S1); tid is the thread's
-// index in the block.  Every thread of the block must call this (it holds
-// barriers).  The launcher guarantees
-// N * R * 8 < 2^31 (all byte offsets fit the buffer range check).
-template <int N, int G0, int G1, int R0, int R1, int NTH = 256>
+// index in the block; LOAD_AUX the cache policy of the sample loads.  Every
+// thread of the block must call this (it holds barriers).  The launcher
+// guarantees N * R * 8 < 2^31 (all byte offsets fit the buffer range check).
+template <int N, int G0, int G1, int R0, int R1, int NTH = 256, int LOAD_AUX = buf::kCached>
 __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell, int S0, int S1, float* lds,
                                                    int tid)
 {
@@ -212,7 +212,7 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
     static_assert(NT % (2 * CW) == 0 && NT % N == 0, "pad/carry split");
 
     float2 buf[E];
-    strip_load<N, E, T>(src, R, S0 + LEAD - CW + s, t, buf);
+    strip_load<N, E, T, LOAD_AUX>(src, R, S0 + LEAD - CW + s, t, buf);
     for (int k = 0; k <= nout; ++k) {
         const int ck = S0 + LEAD - CW + k * CW;  // first column of chunk k
         float2 v[E];
@@ -222,7 +222,7 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
         // registers (keeps the scheduler from hoisting the loads and copying)
         __builtin_amdgcn_sched_barrier(0);
         // (after the last chunk the column is out of range: zeros, no phi)
-        strip_load<N, E, T>(src, R, k < nout ? ck + CW + s : -1, t, buf);
+        strip_load<N, E, T, LOAD_AUX>(src, R, k < nout ? ck + CW + s : -1, t, buf);
         fft::transform<N, E, false>(v, L, TWL, t);
         // |Y|^2 into ring slots CARRY + s
 #pragma unroll

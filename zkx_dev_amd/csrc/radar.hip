@@ -919,6 +919,144 @@ __global__ void __launch_bounds__(256, 2) doppler_cfar_strip_kernel(strip::Strip
 }
 
 // ---------------------------------------------------------------------------
+// Whole dwell pipeline in one persistent launch (pulse compression, then the
+// strip Doppler+CFAR of each dwell as soon as its pulses are compressed).
+// Blocks take jobs from one queue in a fixed order: the PC jobs of dwell d
+// (`rows` pulses each) and, `lead` dwells behind, the strip jobs of dwell
+// d - lead.  A strip job waits until every PC job of its dwell
+// has finished (a per-dwell counter); every job waits only on jobs that were
+// taken from the queue before it by running blocks, so the queue always
+// drains.  The compressed pulses pass from the PC jobs to the strip jobs
+// through memory with system-coherent stores and loads (no cache holds a
+// stale copy, whichever CU or XCD runs either side), and are read back
+// within microseconds of being written, while they are still in the
+// memory-side cache.
+// ---------------------------------------------------------------------------
+int g_fuse_chunks = 8;             // chunks per strip job
+constexpr int kFuseSpinLimit = 1 << 20;  // polls of a dependency before giving up
+
+struct FuseArgs {
+    const float2* x;       // [batch][P][R]
+    const float2* H;       // template spectrum (conjugated in the product)
+    const float2* tw_r;    // R-point twiddles
+    int batch, R, pc_jobs, strips, SW;
+    int rows, lead;        // pulses per PC job; dwells by which PC runs ahead of the strips
+    int job_slot;          // LDS float index of the block's current job number
+    strip::StripArgs dc;   // x = the compressed cube [batch][P][R]
+    int* queue;            // job queues, error flag, per-dwell PC counts (kFuseLine ints apart)
+};
+
+// Queue layout (ints): queue q (one per XCD) at q * kFuseLine; the error flag
+// at 8 * kFuseLine; dwell d's count of finished PC jobs at (9 + d) * kFuseLine
+// (one 256-byte line each: atomics on one line serialize at the memory side).
+constexpr int kFuseLine = 64;
+constexpr int kFuseQueues = 8;
+
+template <int NR, int NP>
+__global__ void __launch_bounds__(256, 2) dwell_fused_kernel(FuseArgs a)
+{
+    using C = RowCfg<NR>;
+    static_assert(C::FPB == 1 && C::THREADS == 256, "one 256-thread row per block");
+    // (no static LDS: it would sit before the dynamic region and leave that
+    // region only 4-byte aligned, and the transforms' 8-byte LDS accesses need
+    // 8-byte alignment)
+    extern __shared__ float lds[];
+    int& s_job = *reinterpret_cast<int*>(lds + a.job_slot);
+    const int tid = threadIdx.x;
+    // blocks are dealt to the XCDs round-robin; the blocks of XCD q take the
+    // dwells d = q, q + 8, ... from queue q, so the PC jobs and strips of a
+    // dwell run on one XCD and hand over through its L2 and the memory-side cache
+    const int q = static_cast<int>(blockIdx.x) % kFuseQueues;
+    const int nd = a.batch > q ? (a.batch - q + kFuseQueues - 1) / kFuseQueues : 0;  // dwells of this queue
+    const int per = a.pc_jobs + a.strips;  // jobs per dwell slot
+    const int total = (nd + a.lead) * per;
+    int* const qc = a.queue + q * kFuseLine;
+    int* const err = a.queue + kFuseQueues * kFuseLine;
+    int* const done = a.queue + (kFuseQueues + 1) * kFuseLine;
+    for (;;) {
+        __syncthreads();  // the previous job's LDS readers are done
+        if (tid == 0) s_job = atomicAdd(qc, 1);
+        __syncthreads();
+        const int j = s_job;
+        if (j >= total) break;
+        const int g = j / per, r = j - g * per;
+        if (r < a.pc_jobs) {
+            // ---- pulse compression of pulses [r*rows, (r+1)*rows) of local dwell g
+            if (g >= nd) continue;
+            const int dw = q + g * kFuseQueues;
+            const fft::RowLds L{reinterpret_cast<float2*>(lds)};
+            float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS);
+            fft::fill_pass16_twiddles<NR>(p16, a.tw_r, tid, C::THREADS);
+            const fft::Pass16Twiddles TW{a.tw_r, p16};
+            const auto rh = buf::rsrc(a.H, NR * 8u);
+            float2 h[C::E];
+#pragma unroll
+            for (int k = 0; k < C::E; ++k) h[k] = buf::ld2(rh, tid * 8u, k * C::T * 8u);
+            const long long row0 = static_cast<long long>(dw) * NP + r * a.rows;
+            // the next row's samples load while the current row is transformed
+            float2 nxt[C::E];
+            {
+                const auto rs = buf::rsrc(a.x + row0 * a.R, NR * 8u);
+#pragma unroll
+                for (int k = 0; k < C::E; ++k) nxt[k] = buf::ld2(rs, tid * 8u, k * C::T * 8u);
+            }
+            for (int qq = 0; qq < a.rows; ++qq) {
+                const long long row = row0 + qq;
+                float2 v[C::E];
+#pragma unroll
+                for (int k = 0; k < C::E; ++k) v[k] = nxt[k];
+                if (qq + 1 < a.rows) {
+                    const auto rs = buf::rsrc(a.x + (row + 1) * a.R, NR * 8u);
+#pragma unroll
+                    for (int k = 0; k < C::E; ++k) nxt[k] = buf::ld2(rs, tid * 8u, k * C::T * 8u);
+                }
+                int tt = tid;
+                asm volatile("" : "+v"(tt));
+                fft::transform<NR, C::E, false>(v, L, TW, tt);
+#pragma unroll
+                for (int k = 0; k < C::E; ++k) v[k] = cmulc(v[k], h[k]);
+                fft::transform<NR, C::E, true>(v, L, TW, tt);
+                const auto ry = buf::rsrc(a.dc.x + row * NR, NR * 8u);
+                const float sc = 1.0f / static_cast<float>(NR);
+#pragma unroll
+                for (int k = 0; k < C::E; ++k) buf::st2<buf::kSystem>(ry, cscale(v[k], sc), tid * 8u, k * C::T * 8u);
+            }
+            // every store of this job has reached memory before the count moves
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (tid == 0)
+                __hip_atomic_fetch_add(done + dw * kFuseLine, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            // ---- strip r - pc_jobs of local dwell g - lead
+            const int gl = g - a.lead;
+            if (gl < 0) continue;
+            const int dw = q + gl * kFuseQueues;
+            if (tid == 0) {
+                int spins = 0;
+                while (__hip_atomic_load(done + dw * kFuseLine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       a.pc_jobs) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > kFuseSpinLimit) {
+                        // bounded wait: flag the failure and go on (the kernel always ends)
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            const int S0 = (r - a.pc_jobs) * a.SW;
+            strip::doppler_cfar_strip<NP, 2, 2, 8, 8, 256, buf::kSystem>(a.dc, dw, S0, min(S0 + a.SW, a.R), lds,
+                                                                          tid);
+        }
+    }
+}
+
+__global__ void zero_ints_kernel(int* p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
 // Doppler transform of any nfft (not a power of two, or beyond the column
 // kernels): corner turn into [R][nfft] rows, any-length row FFT, corner turn
 // back.  FP32 throughout, like the power-of-two path.
@@ -1428,6 +1566,87 @@ int doppler_cfar(int n, const float2* x, int batch, int P, int R, int window, in
     }
 }
 
+// The one-launch pipeline (dwell_fused_kernel) for full-range dwells of
+// R = 4096 samples (one 256-thread row), P = 32, 64 or 128 pulses, no range-Doppler map
+// and the configured window (guard 2x2, reference 8x8); *done stays false
+// otherwise and the caller runs the two-launch form.
+// 0: one launch where it applies; 1 (default): two launches.  The one-launch
+// form gives the same bits but measured slower on 64 dwells of 128 x 4096
+// (243 us against 232 us for the two launches): its PC jobs run at the strip
+// kernel's occupancy (two blocks per CU) and every job pays a queue round trip.
+int g_fuse_variant = 1;
+int g_fuse_rows = 4;     // pulses per PC job
+int g_fuse_lead = 2;     // dwells of one XCD queue by which its PC jobs run ahead of its strip jobs
+
+template <int NR, int NP>
+int launch_dwell_fused(const float2* x, int batch, const float2* H, int window, double alpha_over_n, float* thr,
+                       unsigned char* det, float2* pc, int* queue, hipStream_t st)
+{
+    using SC = strip::StripCfg<NP, 2, 2, 8, 8>;
+    const size_t body = (std::max(PcCfg<NR>::LDS_BYTES, SC::LDS_BYTES) + 15) / 16 * 16;
+    const size_t lds = body + 16;  // + the job number
+    int rc = allow_lds(dwell_fused_kernel<NR, NP>, lds);
+    if (rc) return rc;
+    const float2 *tw_r = nullptr, *tw_p = nullptr;
+    if ((rc = twiddles(NR, &tw_r)) || (rc = twiddles(NP, &tw_p))) return rc;
+    int dev = 0, cus = 256;
+    ZKX_HIP_CHECK(hipGetDevice(&dev));
+    ZKX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    FuseArgs a;
+    a.x = x;
+    a.H = H;
+    a.tw_r = tw_r;
+    a.batch = batch;
+    a.R = NR;
+    a.rows = std::max(1, std::min(NP, g_fuse_rows));
+    while (NP % a.rows) --a.rows;
+    a.lead = std::max(0, g_fuse_lead);
+    a.pc_jobs = NP / a.rows;
+    a.SW = std::max(1, g_fuse_chunks) * SC::CW;
+    a.strips = (NR + a.SW - 1) / a.SW;
+    a.dc.x = pc;
+    a.dc.R = NR;
+    a.dc.window = window;
+    a.dc.SW = a.SW;
+    a.dc.alpha_over_n = alpha_over_n;
+    a.dc.tw = tw_p;
+    a.dc.thr = thr;
+    a.dc.det = det;
+    a.dc.y = nullptr;
+    a.dc.power = nullptr;
+    a.queue = queue;
+    a.job_slot = static_cast<int>(body / sizeof(float));
+    // the counters start at zero (a fill kernel, so the sequence can be captured)
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(16), dim3(256), 0, st, queue, (kFuseQueues + 1 + batch) * kFuseLine);
+    ZKX_LAUNCH_CHECK();
+    hipLaunchKernelGGL((dwell_fused_kernel<NR, NP>), dim3(2 * cus), dim3(256), lds, st, a);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+int dwell_fused(const float2* x, int batch, int P, int R, const float2* H, int window, int g0, int g1, int r0,
+                int r1, double pfa, float* thr, unsigned char* det, float2* rd_map, float2* pc, int* queue,
+                hipStream_t st, bool* done)
+{
+    *done = false;
+    const bool fixed = g0 == 2 && g1 == 2 && r0 == 8 && r1 == 8;
+    if (!fixed || rd_map || R != 4096 || (P != 32 && P != 64 && P != 128))
+        return ZKX_SUCCESS;
+    if (static_cast<long long>(P) * R * 8 >= (1LL << 31)) return ZKX_SUCCESS;
+    const int count = 2 * r0 * (2 * r1 + 2 * g1 + 1) + 2 * (2 * g0 + 1) * r1;
+    const double n = static_cast<double>(count);
+    const double alpha = n * (std::pow(pfa, -1.0 / n) - 1.0);
+    int rc = ZKX_SUCCESS;
+#define ZKX_FUSE_CASE(NR_, NP_) \
+    if (R == NR_ && P == NP_) rc = launch_dwell_fused<NR_, NP_>(x, batch, H, window, alpha / n, thr, det, pc, queue, st);
+    ZKX_FUSE_CASE(4096, 32)
+    ZKX_FUSE_CASE(4096, 64)
+    ZKX_FUSE_CASE(4096, 128)
+#undef ZKX_FUSE_CASE
+    if (rc == ZKX_SUCCESS) *done = true;
+    return rc;
+}
+
 int validate_cfar2d(int rows, int cols, int g0, int g1, int r0, int r1, double pfa)
 {
     if (g0 < 0 || g1 < 0 || r0 < 1 || r1 < 1)
@@ -1551,6 +1770,26 @@ int zkx_set_kernel_variant(const char* op, int variant)
         g_dc_variant = variant;
         return ZKX_SUCCESS;
     }
+    if (name == "dwell_fused") {
+        if (variant < 0 || variant > 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "dwell_fused variant must be 0 or 1");
+        g_fuse_variant = variant;
+        return ZKX_SUCCESS;
+    }
+    if (name == "dwell_fused_rows") {
+        if (variant < 1 || variant > 128) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "dwell_fused_rows must be in [1, 128]");
+        g_fuse_rows = variant;
+        return ZKX_SUCCESS;
+    }
+    if (name == "dwell_fused_chunks") {
+        if (variant < 1 || variant > 256) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "dwell_fused_chunks must be in [1, 256]");
+        g_fuse_chunks = variant;
+        return ZKX_SUCCESS;
+    }
+    if (name == "dwell_fused_lead") {
+        if (variant < 0 || variant > 1024) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "dwell_fused_lead must be in [0, 1024]");
+        g_fuse_lead = variant;
+        return ZKX_SUCCESS;
+    }
     if (name == "doppler_cfar_strip_blocks") {
         if (variant < 1 || variant > (1 << 20)) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "strip block target out of range");
         g_dc_strip_blocks = variant;
@@ -1574,6 +1813,9 @@ int zkx_get_kernel_variant(const char* op)
     if (name == "fir_fft") return g_fir_fft;
     if (name == "doppler_cfar") return g_dc_variant;
     if (name == "doppler_cfar_strip_blocks") return g_dc_strip_blocks;
+    if (name == "dwell_fused") return g_fuse_variant;
+    if (name == "dwell_fused_rows") return g_fuse_rows;
+    if (name == "dwell_fused_lead") return g_fuse_lead;
     if (name == "inject_fft_failure") return g_inject_fft_failure.load();
     return -1;
 }
@@ -1769,9 +2011,12 @@ int zkx_doppler_cfar_c64(const zkx_complex64* x, int batch, int num_pulses, int 
 size_t zkx_dwell_workspace_bytes(int batch, int num_pulses, int range_bins)
 {
     const size_t cells = static_cast<size_t>(batch) * num_pulses * range_bins;
-    // range-compressed cube (c64) + power map (f32) + integral image (f32), 256-byte aligned
+    // range-compressed cube (c64) + power map (f32) + integral image (f32) +
+    // the fused pipeline's job counters (8 queues, an error flag and one
+    // counter per dwell, a 256-byte line each), 256-byte aligned
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-    return al(cells * sizeof(float2)) + al(cells * sizeof(float)) + al(cells * sizeof(float));
+    return al(cells * sizeof(float2)) + al(cells * sizeof(float)) + al(cells * sizeof(float)) +
+           (static_cast<size_t>(batch) + 9) * 256;
 }
 
 int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int range_bins,
@@ -1797,6 +2042,14 @@ int zkx_dwell_batch_c64(const zkx_complex64* x, int batch, int num_pulses, int r
     float2* pc = reinterpret_cast<float2*>(ws);
     float* pw = reinterpret_cast<float*>(ws + al(cells * sizeof(float2)));
     float* I = reinterpret_cast<float*>(ws + al(cells * sizeof(float2)) + al(cells * sizeof(float)));
+    int* queue = reinterpret_cast<int*>(ws + al(cells * sizeof(float2)) + 2 * al(cells * sizeof(float)));
+    if (cfar_mode == 1 && g_fuse_variant == 0) {
+        bool done = false;
+        rc = dwell_fused(reinterpret_cast<const float2*>(x), batch, P, R, reinterpret_cast<const float2*>(spectrum),
+                         doppler_window, guard_doppler, guard_range, ref_doppler, ref_range, pfa, threshold,
+                         detections, reinterpret_cast<float2*>(rd_map), pc, queue, st, &done);
+        if (rc || done) return rc;
+    }
     rc = pc_fused(R, reinterpret_cast<const float2*>(x), static_cast<long long>(batch) * P, R,
                   reinterpret_cast<const float2*>(spectrum), pc, st);
     if (rc) return rc;

@@ -45,15 +45,23 @@ typedef float f2v __attribute__((ext_vector_type(2)));
 typedef unsigned u2v __attribute__((ext_vector_type(2)));
 // (whole-vector bit casts: subscripting the builtin's u2v result element by
 // element has been narrowed to a single dword load by this compiler)
+// Cache policy of an access (the instruction's sc0 / nt / sc1 bits): kCached
+// is the default; kSystem (sc0 | sc1) writes through to memory and reads miss
+// in both cache levels, so a value one CU stores this way is seen by another
+// CU that loads it this way without any cache maintenance.
+constexpr int kCached = 0;
+constexpr int kSystem = 1 | 16;
+template <int AUX = kCached>
 __device__ __forceinline__ float2 ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
-    const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
     return make_float2(v.x, v.y);
 }
+template <int AUX = kCached>
 __device__ __forceinline__ void st2(__amdgpu_buffer_rsrc_t r, float2 v, unsigned voff, unsigned soff)
 {
     const f2v w = {v.x, v.y};
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, w), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, w), r, voff, soff, AUX);
 }
 __device__ __forceinline__ void st1(__amdgpu_buffer_rsrc_t r, float v, unsigned voff, unsigned soff)
 {
