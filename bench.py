@@ -230,7 +230,8 @@ def bench_batched(args):
 
 
 def serve_summary(rates=(1000, 5000, 20000), seconds=1.0,
-                  sweep_rates=(20000, 50000, 80000, 100000, 120000), sweep_seconds=0.5, plain_n=40):
+                  sweep_rates=(20000, 50000, 80000, 100000, 120000, 160000, 200000, 250000), sweep_seconds=0.5,
+                  plain_n=40):
     """The serving configuration: Poisson arrivals at 1k, 5k and 20k
     requests/s of dwells drawn equiprobably from 64x2048, 128x4096 and
     256x8192 complex64 (resident in a device pool), 4 resident chirp templates
@@ -607,7 +608,7 @@ def other_configs(reps):
                                                     "one_dwell_per_call_dwells_per_s", "autotune_s", "workload")}
     # serving: Poisson arrivals at 20k requests/s of mixed shapes, and the
     # 128x4096-only rate sustained at p99 <= 2 ms from a short sweep
-    sv = serve_summary(rates=(20000,), seconds=0.5, sweep_rates=(50000, 80000, 100000), sweep_seconds=0.3,
+    sv = serve_summary(rates=(20000,), seconds=0.5, sweep_rates=(80000, 120000, 160000, 200000), sweep_seconds=0.3,
                        plain_n=20)
     out["serving_poisson"] = {"mixed_20k": sv["mixed_shapes"]["20k"],
                               "sustained_128x4096_at_p99_le_2ms": sv["sustained_128x4096_at_p99_le_2ms"],
