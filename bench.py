@@ -45,8 +45,13 @@ def parse():
                     default="dwell")
     ap.add_argument("--dwells", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--lanes", type=int, default=3,
-                    help="streams that take consecutive batches in turn (own workspace and outputs each)")
+    ap.add_argument("--lanes", type=int, default=1,
+                    help="streams that take consecutive batches in turn (own workspace and outputs each); "
+                         "one lane runs each batch's two launches back to back, so the compressed cube is "
+                         "still in the memory-side cache when the Doppler+CFAR launch reads it")
+    ap.add_argument("--inputs", type=int, default=4,
+                    help="distinct input batches the timed steps rotate through (more bytes than the "
+                         "memory-side cache holds, so no step reads its input from cache)")
     ap.add_argument("--no-others", action="store_true", help="skip the short filter / split-BF16 measurements")
     ap.add_argument("--parse-counters", nargs="+", metavar="CSV_OR_OUT",
                     help="counter CSVs of rocprofv3 runs of --workload probe, then the output JSON")
@@ -800,6 +805,9 @@ def main():
     x = torch.from_numpy(cubes).to(dev)
     t = torch.from_numpy(tmpl).to(dev)
     del cubes
+    # the timed steps rotate through distinct input batches (the first one
+    # rotated in phase by a unit complex factor per batch)
+    xs = [x] + [x * complex(0.6, 0.8) ** (i + 1) for i in range(max(1, args.inputs) - 1)]
     lanes = max(1, args.lanes)
     pipes = [radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
                                  guard=GUARD, reference=REF, pfa=PFA, cfar_mode=args.cfar_mode, max_batch=B)
@@ -869,9 +877,11 @@ def main():
 
     # ---- batched fused pipeline: the timed steps -----------------------------
     # Consecutive batches go to `lanes` streams in turn, each lane with its
-    # own pipeline workspace and output buffers, so one batch's Doppler+CFAR
-    # overlaps the next batch's pulse compression (every batch still runs
-    # the whole chain).
+    # own pipeline workspace and output buffers (every batch still runs the
+    # whole chain).  With one lane (the default) each batch's Doppler+CFAR
+    # reads the compressed cube its pulse compression has just written, much
+    # of it from the memory-side cache; several lanes instead overlap one
+    # batch's Doppler+CFAR with the next batch's pulse compression.
     lane_streams = [torch.cuda.Stream() for _ in range(lanes)] if lanes > 1 else [stream]
 
     def run_steps(nlanes, steps, ev=None):
@@ -884,7 +894,7 @@ def main():
             st = lane_streams[k] if nlanes > 1 else stream
             if ev is not None:
                 ev[s][0].record(st)
-            pipes[k].run(x, threshold=outs[k][0], detections=outs[k][1], stream=st)
+            pipes[k].run(xs[s % len(xs)], threshold=outs[k][0], detections=outs[k][1], stream=st)
             if ev is not None:
                 ev[s][1].record(st)
         for st in lane_streams[:nlanes]:
@@ -908,9 +918,19 @@ def main():
 
     wall, step_ms = timed_steps(lanes)
     dwells_per_s = B * args.steps / wall
-    wall1, _ = timed_steps(1)
-    one_lane_dps = B * args.steps / wall1
-    print(f"# batched: {dwells_per_s:.1f} dwells/s over {lanes} lanes ({one_lane_dps:.1f} on one), "
+    # the other lane arrangement, for comparison (three lanes overlap one
+    # batch's Doppler+CFAR with the next batch's pulse compression)
+    alt_lanes = 3 if lanes == 1 else 1
+    if alt_lanes > len(lane_streams):
+        pipes += [radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
+                                      guard=GUARD, reference=REF, pfa=PFA, cfar_mode=args.cfar_mode, max_batch=B)
+                  for _ in range(alt_lanes - len(pipes))]
+        outs += [(torch.empty((B, P, R), dtype=torch.float32, device=dev),
+                  torch.empty((B, P, R), dtype=torch.uint8, device=dev)) for _ in range(alt_lanes - len(outs))]
+        lane_streams = [torch.cuda.Stream() for _ in range(alt_lanes)]
+    wall_alt, _ = timed_steps(alt_lanes)
+    alt_dps = B * args.steps / wall_alt
+    print(f"# batched: {dwells_per_s:.1f} dwells/s over {lanes} lane(s) ({alt_dps:.1f} over {alt_lanes}), "
           f"step p50 {percentile(step_ms, 50):.3f} ms", file=sys.stderr, flush=True)
 
     # ---- plain per-dwell drop-in ---------------------------------------------
@@ -958,8 +978,8 @@ def main():
                                "CA-CFAR g2x2 r8x8 pfa 1e-6", "batch_dwells": B,
                    "cfar_mode": "reference-identical integral image" if args.cfar_mode == 0
                    else "FP64 windowed, fused with the Doppler FFT",
-                   "lanes": lanes},
-        "one_lane_dwells_per_s": one_lane_dps,
+                   "lanes": lanes, "distinct_input_batches": len(xs)},
+        f"lanes_{alt_lanes}_dwells_per_s": alt_dps,
         "msamples_per_s": dwells_per_s * P * R / 1e6,
         "pipeline_hbm_frac": pipe_bytes / step_s / HBM_PEAK,
         "latency_ms": {"batch_p50": percentile(step_ms, 50), "batch_p99": percentile(step_ms, 99),

@@ -98,8 +98,12 @@ struct StripCfg {
     // (128 B of complex64), so every chunk's loads start on a cache line
     static constexpr int LEAD = (CM + 15) / 16 * 16;
     static constexpr int CARRY = LEAD + CM;       // ring columns carried from one chunk to the next
-    static constexpr int RS = (CW + CARRY) | 1;   // ring row stride (odd: rows of a wave on distinct banks)
-    static constexpr int HS = N + 2 * RM + 1;   // window-sum column stride, RM zero rows either side
+    // Row strides are 2 mod 4 floats: every run starts on 8 bytes, so the
+    // prefix passes read their values as 8-byte pairs (ds_read2_b64), and the
+    // stride is odd in 8-byte units, so a wave's rows (columns) fall on
+    // distinct banks.
+    static constexpr int RS = ((CW + CARRY + 1) / 2 | 1) * 2;      // ring row stride
+    static constexpr int HS = ((N + 2 * RM + 3) / 2 | 1) * 2;      // window-sum column stride, RM zero rows either side
     static constexpr int RING_FLOATS = N * RS;
     static constexpr int H_FLOATS = 2 * CW * HS;
     static constexpr int FFT_FLOATS = 2 * N * CW;
@@ -109,6 +113,7 @@ struct StripCfg {
     static_assert(E == 16 && N % RUN == 0 && CW % RUN == 0, "strip shape");
     static_assert(CW >= CARRY, "the carried columns must come from one chunk");
     static_assert(N * CW == THREADS * RUN, "one horizontal task per thread");
+    static_assert(RS % 4 == 2 && HS % 4 == 2 && HS >= N + 2 * RM + 2 && (RUN + 2 * CM) % 2 == 0, "strides");
 };
 
 struct StripArgs {
@@ -185,12 +190,12 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
     const fft::LdsTwiddles TWL{twl};
 
     // vertical task of this thread: column vo, rows f0 .. f0 + RUN - 1; with
-    // CW = 32 the two half-waves take runs 32 rows apart (disjoint banks)
+    // CW = 32 the two half-waves take adjacent runs, 16 ring rows apart
+    // (16 * RS = 32 mod 64 floats: their ring reads use disjoint banks)
     int vo, vrun;
     if constexpr (CW == 32) {
-        const int w = tid >> 6, h = (tid >> 5) & 1;
         vo = tid & 31;
-        vrun = (w >> 1) * 4 + (w & 1) + 2 * h;
+        vrun = tid >> 5;
     } else {
         vo = tid % CW;
         vrun = tid / CW;
@@ -235,8 +240,8 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const unsigned rowoff = static_cast<unsigned>(e) * T * R;
-                st2(ry, v[e], vo8, rowoff * 8u);
-                st1(rpw, power_of(v[e]), vo4, rowoff * 4u);
+                st2<buf::kStream>(ry, v[e], vo8, rowoff * 8u);
+                st1<buf::kStream>(rpw, power_of(v[e]), vo4, rowoff * 4u);
             }
         }
         __syncthreads();
@@ -250,13 +255,19 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
                 if (z < 2 * RM) zbase[z < RM ? z : N + z] = 0.f;
             }
             {
-                const float* rr = ring + hf * RS + hrun * RUN;  // output i's window: rr[i .. i + 2CM]
+                // output i's window: ring values rr[i .. i + 2CM] (8-byte aligned)
+                constexpr int HN = RUN + 2 * CM;
+                const float2* rr = reinterpret_cast<const float2*>(ring + hf * RS + hrun * RUN);
+                float2 rv[HN / 2];
+#pragma unroll
+                for (int j = 0; j < HN / 2; ++j) rv[j] = rr[j];
                 // FP64 prefix sums of the run's window values: P[q] = sum of
                 // the first q values, so every window sum is one difference
-                double pre[RUN + 2 * CM + 1];
+                double pre[HN + 1];
                 pre[0] = 0.0;
 #pragma unroll
-                for (int q = 0; q < RUN + 2 * CM; ++q) pre[q + 1] = pre[q] + static_cast<double>(rr[q]);
+                for (int q = 0; q < HN; ++q)
+                    pre[q + 1] = pre[q] + static_cast<double>((q & 1) ? rv[q >> 1].y : rv[q >> 1].x);
                 float* hoc = ho + (hrun * RUN) * HS + RM + hf;
                 float* hsc = hs + (hrun * RUN) * HS + RM + hf;
 #pragma unroll
@@ -276,17 +287,29 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
             // horizontal sum that holds a guard cell enters both of the first
             // two terms and cancels.
             {
-                const float* hoc = ho + vo * HS + f0;           // row f0 - RM + q at hoc[q]
-                const float* hsc = hs + vo * HS + f0 + RM - G0;  // row f0 - G0 + q at hsc[q]
+                // row f0 - RM + q of the outer sums at hq[q]; row f0 - G0 + q
+                // of the side bands at hz[ZS + q] (both read as 8-byte pairs
+                // from an 8-byte aligned start, ZS the skipped leading value)
+                constexpr int QN = RUN + 2 * RM, ZN = RUN + 2 * G0;
+                constexpr int ZS = (RM - G0) & 1, ZP = (ZS + ZN + 1) / 2;
+                const float2* hoc = reinterpret_cast<const float2*>(ho + vo * HS + f0);
+                const float2* hsc = reinterpret_cast<const float2*>(hs + vo * HS + f0 + RM - G0 - ZS);
+                float2 hq2[QN / 2], hz2[ZP];
+#pragma unroll
+                for (int j = 0; j < QN / 2; ++j) hq2[j] = hoc[j];
+#pragma unroll
+                for (int j = 0; j < ZP; ++j) hz2[j] = hsc[j];
+                const float* hq = reinterpret_cast<const float*>(hq2);
+                const float* hz = reinterpret_cast<const float*>(hz2) + ZS;
                 // FP64 prefix sums down the column of outer sums (Q) and of
                 // side bands (Z)
-                double qv[RUN + 2 * RM + 1], zv[RUN + 2 * G0 + 1];
+                double qv[QN + 1], zv[ZN + 1];
                 qv[0] = 0.0;
                 zv[0] = 0.0;
 #pragma unroll
-                for (int q = 0; q < RUN + 2 * RM; ++q) qv[q + 1] = qv[q] + static_cast<double>(hoc[q]);
+                for (int q = 0; q < QN; ++q) qv[q + 1] = qv[q] + static_cast<double>(hq[q]);
 #pragma unroll
-                for (int q = 0; q < RUN + 2 * G0; ++q) zv[q + 1] = zv[q] + static_cast<double>(hsc[q]);
+                for (int q = 0; q < ZN; ++q) zv[q + 1] = zv[q] + static_cast<double>(hz[q]);
                 const int c = ck - LEAD + vo;  // output column whose window is ring slots vo .. vo + 2CM
                 const bool colin = c >= CM && c < R - CM;
                 const bool store = c < S1 && c < R;
@@ -304,8 +327,8 @@ __device__ __forceinline__ void doppler_cfar_strip(const StripArgs& a, int dwell
                     const double th = live ? a.alpha_over_n * (bands + side) : 0.0;
                     const float xv = xr[i * RS];
                     const unsigned row = static_cast<unsigned>(i) * R;
-                    st1(rthr, static_cast<float>(th), voff4, row * 4u);
-                    stb(rdet, static_cast<double>(xv) > th ? 1 : 0, voff1, row);
+                    st1<buf::kStream>(rthr, static_cast<float>(th), voff4, row * 4u);
+                    stb<buf::kStream>(rdet, static_cast<double>(xv) > th ? 1 : 0, voff1, row);
                 }
             }
         }

@@ -116,7 +116,9 @@ pc_fused_kernel(const float2* __restrict__ x, long long rows, int S, const float
     if (ONE && S >= N && active) {
         const auto rs = buf::rsrc(src, N * 8u);
 #pragma unroll
-        for (int k = 0; k < C::E; ++k) v[k] = buf::ld2(rs, t * 8u, k * C::T * 8u);
+        // the pulses are read once: streamed, so the caches keep the
+        // compressed output a Doppler stage reads next
+        for (int k = 0; k < C::E; ++k) v[k] = buf::ld2<buf::kStream>(rs, t * 8u, k * C::T * 8u);
     } else {
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
@@ -159,17 +161,19 @@ pc_segments_kernel(const float2* __restrict__ x, long long rows, int nseg, int L
     using C = RowCfg<N>;
     extern __shared__ float lds[];
     const int slot = threadIdx.x / C::T, t = threadIdx.x % C::T;
-    const long long row = static_cast<long long>(blockIdx.x) * C::FPB + slot;
+    // rows < 2^31 (the launcher's limit): 32-bit row arithmetic
+    const int row = static_cast<int>(blockIdx.x) * C::FPB + slot;
     const bool active = row < rows;
-    const long long r = active ? row : 0;
-    const long long p = r / nseg;
-    const int seg = static_cast<int>(r - p * nseg);
+    const int r = active ? row : 0;
+    const int p = r / nseg;
+    const int seg = r - p * nseg;
     const int k0 = seg * L;
     const int count = min(L, nfft - k0);
     float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS * C::FPB);
     fft::fill_pass16_twiddles<N>(p16, tw, threadIdx.x, C::THREADS);
-    tiles::pc_segment_row<N>(x + p * S, nfft, valid, k0, count, active, H, y + p * nfft, t,
-                             lds + slot * 2 * C::SLOT_FLOATS, fft::Pass16Twiddles{tw, p16});
+    tiles::pc_segment_row<N>(x + static_cast<long long>(p) * S, nfft, valid, k0, count, active, H,
+                             y + static_cast<long long>(p) * nfft, t, lds + slot * 2 * C::SLOT_FLOATS,
+                             fft::Pass16Twiddles{tw, p16});
 }
 
 // Persistent form for one row per block (N >= 4096) and full rows (S >= N):
@@ -197,7 +201,7 @@ pc_persistent_kernel(const float2* __restrict__ x, long long rows, int S, const 
     auto fetch = [&](long long r) {
         const auto rs = buf::rsrc(x + r * static_cast<long long>(S), N * 8u);
 #pragma unroll
-        for (int k = 0; k < C::E; ++k) nxt[k] = buf::ld2(rs, t * 8u, k * C::T * 8u);
+        for (int k = 0; k < C::E; ++k) nxt[k] = buf::ld2<buf::kStream>(rs, t * 8u, k * C::T * 8u);
     };
     long long row = blockIdx.x;
     if (row < rows) fetch(row);
@@ -1208,6 +1212,7 @@ int launch_pc_segments(const float2* x, long long rows, int nseg, int L, int nff
     int rc = twiddles(N, &tw);
     if (rc) return rc;
     if ((rc = allow_lds(pc_segments_kernel<N>, PcCfg<N>::LDS_BYTES))) return rc;
+    if (rows >= (1LL << 31) - C::FPB) ZKX_FAIL(ZKX_UNSUPPORTED_OPERATION, "pulse compression: too many segments");
     const long long blocks = (rows + C::FPB - 1) / C::FPB;
     hipLaunchKernelGGL((pc_segments_kernel<N>), dim3(static_cast<unsigned>(blocks)), dim3(C::THREADS),
                        PcCfg<N>::LDS_BYTES, st, x, rows, nseg, L, nfft, valid, S, H, tw, y);

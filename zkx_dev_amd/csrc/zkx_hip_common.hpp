@@ -49,7 +49,11 @@ typedef unsigned u2v __attribute__((ext_vector_type(2)));
 // is the default; kSystem (sc0 | sc1) writes through to memory and reads miss
 // in both cache levels, so a value one CU stores this way is seen by another
 // CU that loads it this way without any cache maintenance.
+// kStream (nt) marks data that is read or written once: its lines are not
+// kept in L2 or the memory-side cache, which then keep what a later kernel
+// reads again (a pipeline stage's output read by the next stage).
 constexpr int kCached = 0;
+constexpr int kStream = 2;
 constexpr int kSystem = 1 | 16;
 template <int AUX = kCached>
 __device__ __forceinline__ float2 ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
@@ -63,13 +67,15 @@ __device__ __forceinline__ void st2(__amdgpu_buffer_rsrc_t r, float2 v, unsigned
     const f2v w = {v.x, v.y};
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, w), r, voff, soff, AUX);
 }
+template <int AUX = kCached>
 __device__ __forceinline__ void st1(__amdgpu_buffer_rsrc_t r, float v, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, AUX);
 }
+template <int AUX = kCached>
 __device__ __forceinline__ void stb(__amdgpu_buffer_rsrc_t r, unsigned char v, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, AUX);
 }
 }  // namespace buf
 
