@@ -49,6 +49,8 @@ def parse():
                     help="streams that take consecutive batches in turn (own workspace and outputs each); "
                          "one lane runs each batch's two launches back to back, so the compressed cube is "
                          "still in the memory-side cache when the Doppler+CFAR launch reads it")
+    ap.add_argument("--compare-lanes", type=int, default=3,
+                    help="also time the dwell pipeline over this many lanes for comparison (0: skip)")
     ap.add_argument("--inputs", type=int, default=4,
                     help="distinct input batches the timed steps rotate through (more bytes than the "
                          "memory-side cache holds, so no step reads its input from cache)")
@@ -920,7 +922,7 @@ def main():
     dwells_per_s = B * args.steps / wall
     # the other lane arrangement, for comparison (three lanes overlap one
     # batch's Doppler+CFAR with the next batch's pulse compression)
-    alt_lanes = 3 if lanes == 1 else 1
+    alt_lanes = args.compare_lanes if args.compare_lanes != lanes else 0
     if alt_lanes > len(lane_streams):
         pipes += [radar.DwellPipeline(t, P, R, normalize=True, pc_window="hamming", doppler_window="hann",
                                       guard=GUARD, reference=REF, pfa=PFA, cfar_mode=args.cfar_mode, max_batch=B)
@@ -928,9 +930,11 @@ def main():
         outs += [(torch.empty((B, P, R), dtype=torch.float32, device=dev),
                   torch.empty((B, P, R), dtype=torch.uint8, device=dev)) for _ in range(alt_lanes - len(outs))]
         lane_streams = [torch.cuda.Stream() for _ in range(alt_lanes)]
-    wall_alt, _ = timed_steps(alt_lanes)
-    alt_dps = B * args.steps / wall_alt
-    print(f"# batched: {dwells_per_s:.1f} dwells/s over {lanes} lane(s) ({alt_dps:.1f} over {alt_lanes}), "
+    alt_dps = None
+    if alt_lanes > 0:
+        wall_alt, _ = timed_steps(alt_lanes)
+        alt_dps = B * args.steps / wall_alt
+    print(f"# batched: {dwells_per_s:.1f} dwells/s over {lanes} lane(s) ({alt_dps} over {alt_lanes}), "
           f"step p50 {percentile(step_ms, 50):.3f} ms", file=sys.stderr, flush=True)
 
     # ---- plain per-dwell drop-in ---------------------------------------------
@@ -979,7 +983,7 @@ def main():
                    "cfar_mode": "reference-identical integral image" if args.cfar_mode == 0
                    else "FP64 windowed, fused with the Doppler FFT",
                    "lanes": lanes, "distinct_input_batches": len(xs)},
-        f"lanes_{alt_lanes}_dwells_per_s": alt_dps,
+        "compare_lanes": {"lanes": alt_lanes, "dwells_per_s": alt_dps} if alt_lanes > 0 else None,
         "msamples_per_s": dwells_per_s * P * R / 1e6,
         "pipeline_hbm_frac": pipe_bytes / step_s / HBM_PEAK,
         "latency_ms": {"batch_p50": percentile(step_ms, 50), "batch_p99": percentile(step_ms, 99),

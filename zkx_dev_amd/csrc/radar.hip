@@ -1480,9 +1480,11 @@ int cfar_window_2d(const float* x, int batch, int rows, int cols, int g0, int g1
 
 // Doppler+CFAR form: 0 strip-streaming (configured cells, 32 <= N <= 128),
 // 1 one tile of 64 transformed bins per block; and the strip kernel's target
-// block count per launch (strips per dwell = ceil(target / batch)).
+// block count per launch (strips per dwell = ceil(target / batch)): 512
+// fills the 256 CUs at two blocks each once, with 8 strips per dwell at
+// batch 64 (6% of each strip's columns transformed twice as halo).
 int g_dc_variant = 0;
-int g_dc_strip_blocks = 1024;
+int g_dc_strip_blocks = 512;
 
 template <int N>
 int launch_doppler_cfar_strip(const float2* x, int batch, int R, int window, double alpha_over_n,
