@@ -72,7 +72,12 @@ int zkx_max_lds_fft_length(void);
  * 0 = one persistent launch (pulse compression jobs and strip Doppler+CFAR
  * jobs from one queue; "dwell_fused_rows", "dwell_fused_chunks" and
  * "dwell_fused_lead" set its job sizes and lead), 1 = two launches (the
- * default, faster on this GPU).  One more name is a
+ * default, faster on this GPU).  "fir_fft" selects the method for long
+ * up = 1 filters on contiguous lines (upfirdn, firfilter): 2 (the default)
+ * 4096-point overlap-save segments with the spectrum folded for decimation
+ * by D in {1, 2, 4, 8}, other D by 1; 1 one transform per polyphase phase;
+ * 0 the direct polyphase kernels; these agree to FP32 rounding, not bit
+ * for bit.  One more name is a
  * test switch, not a variant: "inject_fft_failure" = 1 makes the transform
  * stage of pulse_compression / pulse_doppler return ZKX_BACKEND_ERROR
  * before any device work (the reference's synthesized backend status for

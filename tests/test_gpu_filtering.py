@@ -170,11 +170,19 @@ def test_fftconvolve(F, n1, n2, batch, mode):
 @pytest.mark.parametrize("kind,K,down,shape,cplx", [
     ("upfirdn", 255, 4, (4, 60000), True), ("upfirdn", 200, 3, (2, 30001), True),
     ("upfirdn", 100, 1, (3, 20000), False), ("upfirdn", 255, 4, (2, 50000), False),
-    ("firfilter", 255, 1, (2, 100000), False), ("firfilter", 1000, 1, (1, 70000), False)])
+    ("upfirdn", 129, 2, (2, 40001), True), ("upfirdn", 129, 2, (3, 9000), False),
+    ("upfirdn", 400, 8, (2, 70001), True), ("upfirdn", 400, 8, (1, 33000), False),
+    ("upfirdn", 513, 8, (1, 40000), True), ("upfirdn", 1000, 2, (1, 50000), True),
+    ("firfilter", 255, 1, (2, 100000), False), ("firfilter", 513, 1, (1, 4096), False),
+    ("firfilter", 1000, 1, (1, 70000), False)])
 def test_frequency_domain_fir_matches_direct_and_oracle(F, kind, K, down, shape, cplx):
     """Long up = 1 filters on contiguous lines run as frequency-domain
-    overlap-save; the direct polyphase kernels (variant off) and the oracle
-    agree with it."""
+    overlap-save, in the phase form (variant 1: D phase transforms of N
+    points per segment) and the segment form (variant 2, the default for
+    D in {1, 2, 4, 8} with history up to 1/8 of a 4096-point segment: one
+    4096-point transform and an inverse of 4096/D points on the folded
+    spectrum; other shapes fall back to the phase form); both agree with
+    the direct polyphase kernels (variant 0) and the oracle."""
     from zkx_dev_amd._lib import lib
     rng = np.random.default_rng(K + down)
     h = (rng.standard_normal(K) / np.sqrt(K)).astype(np.float32)
@@ -186,20 +194,24 @@ def test_frequency_domain_fir_matches_direct_and_oracle(F, kind, K, down, shape,
         if kind == "firfilter":
             return F.firfilter(dev(h), dev(x), 1).cpu().numpy()
         return F.upfirdn(dev(h), dev(x), 1, down, 1).cpu().numpy()
-    assert lib().zkx_get_kernel_variant(b"fir_fft") == 1
-    y_fft = run()
-    assert lib().zkx_set_kernel_variant(b"fir_fft", 0) == 0
+    assert lib().zkx_get_kernel_variant(b"fir_fft") == 2
+    y_seg = run()
     try:
+        assert lib().zkx_set_kernel_variant(b"fir_fft", 1) == 0
+        y_ph = run()
+        assert lib().zkx_set_kernel_variant(b"fir_fft", 0) == 0
         y_dir = run()
     finally:
-        lib().zkx_set_kernel_variant(b"fir_fft", 1)
-    assert orc.rel_l2(y_fft, y_dir) < 2e-6
+        lib().zkx_set_kernel_variant(b"fir_fft", 2)
     if kind == "firfilter":
         ref, _ = orc.firfilter(h, x, 1)
     else:
         f = lambda v: orc.upfirdn(h, v, 1, down, 1, fp64_acc=True)  # noqa: E731
         ref = per_component(f, x) if cplx else f(x)
-    assert orc.rel_l2(y_fft, ref) < TOL
+    for y_fft in (y_seg, y_ph):
+        assert y_fft.shape == y_dir.shape
+        assert orc.rel_l2(y_fft, y_dir) < 2e-6
+        assert orc.rel_l2(y_fft, ref) < TOL
 
 
 def test_frequency_domain_firfilter_zi_zf(F):

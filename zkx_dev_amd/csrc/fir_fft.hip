@@ -19,6 +19,7 @@
 #include <algorithm>
 
 namespace zkx {
+extern int g_fir_fft;
 int fft_rows_dispatch(int n, const float2* x, long long rows, bool inverse, float scale, float2* y,
                       hipStream_t st);
 
@@ -121,7 +122,119 @@ fir_fft_kernel(const V* __restrict__ x, int rows, int n, int out_n, int D, int K
     }
 }
 
+// Segment form for D in {1, 2, 4, 8}: each row is an overlap-save segment of
+// NF = 4096 contiguous input samples (real lines: two consecutive segments
+// as the real and imaginary parts of one row).  One forward transform, the
+// product with H = FFT_NF(h), then the spectrum folded to NI = NF/D points:
+// the D aliases k' + q*NI summed give the transform of every D-th output,
+//   y[D j] = (1/NF) sum_{k'<NI} (sum_{q<D} F[k' + q NI]) W_NI^{-k' j},
+// so one NI-point inverse transform yields the decimated outputs.  A thread
+// holds F[t + 256 k] (k < 16), whose aliases are its own elements k' + (16/D) q,
+// so the fold is in registers.  Segment s starts at input s*Lm*D - KD*D
+// (KD = ceil((K-1)/D) output rows of history, Lm = NI - KD outputs).
+template <int D, class V>
+__global__ void __launch_bounds__(256)
+fir_seg_kernel(const V* __restrict__ x, int rpl, int n, int out_n, int KD, int Lm, int nseg,
+               const float2* __restrict__ H, const float2* __restrict__ twF, const float2* __restrict__ twI,
+               V* __restrict__ y)
+{
+    constexpr int NF = 4096, NI = NF / D, E = 16, EI = E / D, T = 256;
+    constexpr int BL = Lane2<V>::BLOCKS;
+    using C = RowCfg<NF>;
+    static_assert(C::T == T && C::FPB == 1 && EI >= 2, "segment shape");
+    extern __shared__ float lds[];
+    const int t = threadIdx.x;
+    const int row = blockIdx.x;
+    const int line = row / rpl;
+    const int s0 = (row - line * rpl) * BL;
+    const fft::RowLds L{reinterpret_cast<float2*>(lds)};
+    float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS);
+    fft::fill_pass16_twiddles<NF>(p16, twF, t, T);
+    const fft::Pass16Twiddles TW{twF, p16};
+    const auto rx = buf::rsrc(x + static_cast<long long>(line) * n, static_cast<unsigned>(n) * sizeof(V));
+    const int b = (s0 * Lm - KD) * D;  // input index of sample 0 of segment s0
+    float2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = b + t + k * T;
+        if constexpr (BL == 1) {
+            v[k] = buf::ld2(rx, static_cast<unsigned>(i) < static_cast<unsigned>(n) ? i * 8u : buf::kOff, 0u);
+        } else {
+            const int i2 = i + Lm * D;  // the next segment
+            const unsigned o1 = static_cast<unsigned>(i) < static_cast<unsigned>(n) ? i * 4u : buf::kOff;
+            const unsigned o2 = static_cast<unsigned>(i2) < static_cast<unsigned>(n) ? i2 * 4u : buf::kOff;
+            v[k] = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, o1, 0u, 0)),
+                               __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, o2, 0u, 0)));
+        }
+    }
+    fft::transform<NF, E, false>(v, L, TW, t);
+    const auto rh = buf::rsrc(H, NF * 8u);
+#pragma unroll
+    for (int k = 0; k < E; ++k) v[k] = cmul(v[k], buf::ld2(rh, t * 8u, k * T * 8u));
+    float2 z[EI];
+#pragma unroll
+    for (int k = 0; k < EI; ++k) {
+        z[k] = v[k];
+#pragma unroll
+        for (int q = 1; q < D; ++q) z[k] = make_float2(z[k].x + v[k + EI * q].x, z[k].y + v[k + EI * q].y);
+    }
+    if constexpr (D == 1)
+        fft::transform<NI, EI, true>(z, L, TW, t);
+    else
+        fft::transform<NI, EI, true>(z, L, twI, t);
+    const float sc = 1.0f / static_cast<float>(NF);
+    // outputs j = t + 256 k' >= KD of the segment; stores past the line's end
+    // fall outside its buffer
+    const auto ry = buf::rsrc(y + static_cast<long long>(line) * out_n, static_cast<unsigned>(out_n) * sizeof(V));
+#pragma unroll
+    for (int k = 0; k < EI; ++k) {
+        const int j = t + k * T;
+        const int m = s0 * Lm + j - KD;
+        if constexpr (BL == 1) {
+            buf::st2(ry, cscale(z[k], sc), j >= KD ? m * 8u : buf::kOff, 0u);
+        } else {
+            buf::st1(ry, z[k].x * sc, j >= KD ? m * 4u : buf::kOff, 0u);
+            buf::st1(ry, z[k].y * sc, (j >= KD && s0 + 1 < nseg) ? (m + Lm) * 4u : buf::kOff, 0u);
+        }
+    }
+}
+
 thread_local ThreadWorkspace t_spec_ws;
+
+template <int D, class V>
+int launch_fir_seg(const float* h, int K, const V* x, int lines, long long n, long long out_n, V* y, hipStream_t st)
+{
+    constexpr int NF = 4096, NI = NF / D;
+    constexpr int BL = Lane2<V>::BLOCKS;
+    using C = RowCfg<NF>;
+    const int KD = (K - 1 + D - 1) / D;
+    const int Lm = NI - KD;
+    void* ws = nullptr;
+    WorkspaceLease lease_(t_spec_ws, st);
+    int rc = lease_.get(sizeof(float2) * NF, &ws);
+    if (rc) return rc;
+    float2* Hs = static_cast<float2*>(ws);
+    hipLaunchKernelGGL(phase_taps_kernel, dim3(NF / 256), dim3(256), 0, st, h, K, 1, NF, Hs);
+    ZKX_LAUNCH_CHECK();
+    if ((rc = fft_rows_dispatch(NF, Hs, 1, false, 1.0f, Hs, st))) return rc;
+    const float2 *twF = nullptr, *twI = nullptr;
+    if ((rc = twiddles(NF, &twF)) || (rc = twiddles(NI, &twI))) return rc;
+    const long long nseg = (out_n + Lm - 1) / Lm;
+    const long long rpl = (nseg + BL - 1) / BL;
+    const long long rows = static_cast<long long>(lines) * rpl;
+    // 32-bit offsets within a line, row index and segment starts
+    if (rows >= (1LL << 31) || n * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
+        out_n * static_cast<long long>(sizeof(V)) >= (1LL << 31) || (nseg + 1) * Lm * D + NF >= (1LL << 31))
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "frequency-domain FIR: line too long for 32-bit indexing");
+    constexpr size_t lds = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<NF>::ENTRIES;
+    ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fir_seg_kernel<D, V>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    hipLaunchKernelGGL((fir_seg_kernel<D, V>), dim3(static_cast<unsigned>(rows)), dim3(256), lds, st, x,
+                       static_cast<int>(rpl), static_cast<int>(n), static_cast<int>(out_n), KD, Lm,
+                       static_cast<int>(nseg), Hs, twF, twI, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
 
 template <int N, class V>
 int launch_fir_fft(const float* h, int K, const V* x, int lines, long long n, int D, long long out_n, V* y,
@@ -171,6 +284,19 @@ int fir_fft_run(const float* h, int K, const V* x, int lines, long long n, int D
     *done = false;
     const int Kmax = (K + D - 1) / D;
     if (Kmax < 48 || lines < 1 || out_n < 1) return ZKX_SUCCESS;
+    // segment form: history at most 1/8 of the 4096-point segment
+    const int KD = (K - 1 + D - 1) / D;
+    if (g_fir_fft == 2 && (D == 1 || D == 2 || D == 4 || D == 8) && KD * D * 8 <= 4096 && out_n >= 4096 / D) {
+        int rc;
+        switch (D) {
+        case 1: rc = launch_fir_seg<1, V>(h, K, x, lines, n, out_n, y, st); break;
+        case 2: rc = launch_fir_seg<2, V>(h, K, x, lines, n, out_n, y, st); break;
+        case 4: rc = launch_fir_seg<4, V>(h, K, x, lines, n, out_n, y, st); break;
+        default: rc = launch_fir_seg<8, V>(h, K, x, lines, n, out_n, y, st); break;
+        }
+        if (!rc) *done = true;
+        return rc;
+    }
     // segment: at most 1/8 of it spent on the history
     int N = 1024;
     while (N < 8192 && (Kmax - 1) * 8 > N) N <<= 1;

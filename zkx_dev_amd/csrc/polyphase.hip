@@ -34,9 +34,11 @@ template <class V>
 int fir_fft_run(const float* h, int K, const V* x, int lines, long long n, int D, long long out_n, V* y,
                 hipStream_t st, bool* done);
 
-// 1: long up = 1 filters on contiguous lines take the frequency-domain path
-// (fir_fft.hip); 0: always the direct polyphase kernels.
-int g_fir_fft = 1;
+// Long up = 1 filters on contiguous lines take a frequency-domain path
+// (fir_fft.hip): 2 the 4096-point segment form where it applies (D in
+// {1, 2, 4, 8}), else the phase form; 1 always the phase form; 0 always the
+// direct polyphase kernels.
+int g_fir_fft = 2;
 
 namespace {
 

@@ -82,8 +82,6 @@ pc_template_kernel(const float2* __restrict__ h, int taps, int keep, int normali
 }
 
 // Fused pulse compression of rows: y = IFFT_N(FFT_N(x_pad) * conj(H)) / N.
-// rows = total pulses (batch * P); each row reads min(S, N) samples.
-// Fused pulse compression of rows: y = IFFT_N(FFT_N(x_pad) * conj(H)) / N.
 // rows = total pulses (batch * P); each row reads min(S, N) samples.  The
 // stride-16 pass takes its twiddles from a small LDS table after the slots
 // (both transforms share it); full rows load, and every row stores, through
@@ -1768,7 +1766,7 @@ int zkx_set_kernel_variant(const char* op, int variant)
         return ZKX_SUCCESS;
     }
     if (name == "fir_fft") {
-        if (variant < 0 || variant > 1) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "fir_fft variant out of range");
+        if (variant < 0 || variant > 2) ZKX_FAIL(ZKX_INVALID_ARGUMENT, "fir_fft variant out of range");
         g_fir_fft = variant;
         return ZKX_SUCCESS;
     }
