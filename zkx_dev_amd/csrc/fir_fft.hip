@@ -199,6 +199,27 @@ fir_seg_kernel(const V* __restrict__ x, int rpl, int n, int out_n, int KD, int L
     }
 }
 
+// H = FFT_NF(h zero-padded to NF), one block.
+template <int NF>
+__global__ void __launch_bounds__(RowCfg<NF>::THREADS)
+fir_spectrum_kernel(const float* __restrict__ h, int K, const float2* __restrict__ tw, float2* __restrict__ H)
+{
+    using C = RowCfg<NF>;
+    static_assert(C::FPB == 1, "one row per block");
+    extern __shared__ float lds[];
+    const int t = threadIdx.x;
+    const fft::RowLds L{reinterpret_cast<float2*>(lds)};
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int i = t + k * C::T;
+        v[k] = make_float2(i < K ? h[i] : 0.f, 0.f);
+    }
+    fft::transform<NF, C::E, false>(v, L, tw, t);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) H[t + k * C::T] = v[k];
+}
+
 thread_local ThreadWorkspace t_spec_ws;
 
 template <int D, class V>
@@ -214,11 +235,10 @@ int launch_fir_seg(const float* h, int K, const V* x, int lines, long long n, lo
     int rc = lease_.get(sizeof(float2) * NF, &ws);
     if (rc) return rc;
     float2* Hs = static_cast<float2*>(ws);
-    hipLaunchKernelGGL(phase_taps_kernel, dim3(NF / 256), dim3(256), 0, st, h, K, 1, NF, Hs);
-    ZKX_LAUNCH_CHECK();
-    if ((rc = fft_rows_dispatch(NF, Hs, 1, false, 1.0f, Hs, st))) return rc;
     const float2 *twF = nullptr, *twI = nullptr;
     if ((rc = twiddles(NF, &twF)) || (rc = twiddles(NI, &twI))) return rc;
+    hipLaunchKernelGGL(fir_spectrum_kernel<NF>, dim3(1), dim3(C::THREADS), C::LDS_BYTES, st, h, K, twF, Hs);
+    ZKX_LAUNCH_CHECK();
     const long long nseg = (out_n + Lm - 1) / Lm;
     const long long rpl = (nseg + BL - 1) / BL;
     const long long rows = static_cast<long long>(lines) * rpl;
