@@ -214,6 +214,47 @@ def test_frequency_domain_fir_matches_direct_and_oracle(F, kind, K, down, shape,
         assert orc.rel_l2(y_fft, ref) < TOL
 
 
+@pytest.mark.parametrize("api,up,down,K,shape", [
+    ("resample_poly", 3, 2, 121, (2, 50001)), ("resample_poly", 4, 2, 200, (1, 40000)),
+    ("resample_poly", 2, 3, 300, (1, 30000)), ("upfirdn", 2, 1, 129, (2, 30000)),
+    ("upfirdn", 4, 1, 255, (1, 20000)), ("upfirdn", 3, 4, 240, (1, 60000)),
+    ("upfirdn", 3, 2, 255, (3, 12000)), ("resample_poly", 3, 2, 301, (1, 30000))])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_segment_form_rate_changing_fir(F, api, up, down, K, shape, cplx):
+    """Rate-changing filters on long contiguous lines (up/g in {2, 3, 4},
+    down/g in {1, 2, 4}, phases of 64 taps and more) run as 4096-point
+    overlap-save segments whose forward transform is shared by the output
+    residues (variant 2, the default); other factors (2/3 here) and shorter
+    phases (3/2 with 121 taps) keep the direct kernels.  Both agree with the direct polyphase kernels (variant 0)
+    and the oracle."""
+    from zkx_dev_amd._lib import lib
+    rng = np.random.default_rng(up * 1000 + down * 10 + K)
+    taps = (rng.standard_normal(K) / np.sqrt(K)).astype(np.float32)
+    x = rng.uniform(-1, 1, shape).astype(np.float32)
+    if cplx:
+        x = (x + 1j * rng.uniform(-1, 1, shape)).astype(np.complex64)
+
+    def run():
+        if api == "resample_poly":
+            return F.resample_poly(dev(x), up, down, 1, coefficients=taps).cpu().numpy()
+        return F.upfirdn(dev(taps), dev(x), up, down, 1).cpu().numpy()
+    assert lib().zkx_get_kernel_variant(b"fir_fft") == 2
+    y_seg = run()
+    try:
+        assert lib().zkx_set_kernel_variant(b"fir_fft", 0) == 0
+        y_dir = run()
+    finally:
+        lib().zkx_set_kernel_variant(b"fir_fft", 2)
+    if api == "resample_poly":
+        f = lambda v: orc.resample_poly_f32(v, up, down, 1, taps=taps)[0]  # noqa: E731
+    else:
+        f = lambda v: orc.upfirdn(taps, v, up, down, 1, fp64_acc=True)  # noqa: E731
+    ref = per_component(f, x) if cplx else f(x)
+    assert y_seg.shape == y_dir.shape == ref.shape
+    assert orc.rel_l2(y_seg, y_dir) < 2e-6
+    assert orc.rel_l2(y_seg, ref) < TOL
+
+
 def test_frequency_domain_firfilter_zi_zf(F):
     rng = np.random.default_rng(5)
     K, n = 255, 40000

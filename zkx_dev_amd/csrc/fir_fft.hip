@@ -220,6 +220,107 @@ fir_spectrum_kernel(const float* __restrict__ h, int K, const float2* __restrict
     for (int k = 0; k < C::E; ++k) H[t + k * C::T] = v[k];
 }
 
+// Rate-changing form (up > 1, Pn = PN residues after gcd reduction): output
+// n = PN*T + r is y_r[T] = sum_i g_r[i] x[D T + c - i], where g_r holds the
+// taps of residue r's phase shifted so every residue reads the same input
+// alignment c.  A segment's forward transform is shared by the PN residues:
+// for each, the product with G_r = FFT(g_r), the fold by D and one
+// (4096/D)-point inverse; residue r's outputs land at stride PN.
+struct PolyTaps {
+    int phase[4];  // phi_r: first tap of residue r's phase
+    int shift[4];  // c - Q_r: leading zeros of g_r
+};
+
+// G[r] = FFT_NF(g_r), block r.
+template <int NF>
+__global__ void __launch_bounds__(RowCfg<NF>::THREADS)
+fir_poly_spectra_kernel(const float* __restrict__ h, int K, int up, PolyTaps pt, const float2* __restrict__ tw,
+                        float2* __restrict__ G)
+{
+    using C = RowCfg<NF>;
+    static_assert(C::FPB == 1, "one row per block");
+    extern __shared__ float lds[];
+    const int t = threadIdx.x, r = blockIdx.x;
+    const fft::RowLds L{reinterpret_cast<float2*>(lds)};
+    float2 v[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+        const int i = t + k * C::T - pt.shift[r];
+        const long long tap = pt.phase[r] + static_cast<long long>(up) * i;
+        v[k] = make_float2(i >= 0 && tap < K ? h[tap] : 0.f, 0.f);
+    }
+    fft::transform<NF, C::E, false>(v, L, tw, t);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) G[static_cast<long long>(r) * NF + t + k * C::T] = v[k];
+}
+
+template <int D, int PN, class V>
+__global__ void __launch_bounds__(256)
+fir_poly_kernel(const V* __restrict__ x, int rpl, int n, int out_n, int KD, int Lm, int nseg, int c,
+                const float2* __restrict__ G, const float2* __restrict__ twF, const float2* __restrict__ twI,
+                V* __restrict__ y)
+{
+    constexpr int NF = 4096, NI = NF / D, E = 16, EI = E / D, T = 256;
+    constexpr int BL = Lane2<V>::BLOCKS;
+    using C = RowCfg<NF>;
+    static_assert(C::T == T && C::FPB == 1 && EI >= 2, "segment shape");
+    extern __shared__ float lds[];
+    const int t = threadIdx.x;
+    const int row = blockIdx.x;
+    const int line = row / rpl;
+    const int s0 = (row - line * rpl) * BL;
+    const fft::RowLds L{reinterpret_cast<float2*>(lds)};
+    float2* p16 = reinterpret_cast<float2*>(lds + 2 * C::SLOT_FLOATS);
+    fft::fill_pass16_twiddles<NF>(p16, twF, t, T);
+    const fft::Pass16Twiddles TW{twF, p16};
+    const auto rx = buf::rsrc(x + static_cast<long long>(line) * n, static_cast<unsigned>(n) * sizeof(V));
+    const int b = (s0 * Lm - KD) * D + c;  // input index of sample 0 of segment s0
+    float2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = b + t + k * T;
+        if constexpr (BL == 1) {
+            v[k] = buf::ld2(rx, static_cast<unsigned>(i) < static_cast<unsigned>(n) ? i * 8u : buf::kOff, 0u);
+        } else {
+            const int i2 = i + Lm * D;  // the next segment
+            const unsigned o1 = static_cast<unsigned>(i) < static_cast<unsigned>(n) ? i * 4u : buf::kOff;
+            const unsigned o2 = static_cast<unsigned>(i2) < static_cast<unsigned>(n) ? i2 * 4u : buf::kOff;
+            v[k] = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, o1, 0u, 0)),
+                               __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, o2, 0u, 0)));
+        }
+    }
+    fft::transform<NF, E, false>(v, L, TW, t);
+    const float sc = 1.0f / static_cast<float>(NF);
+    const auto ry = buf::rsrc(y + static_cast<long long>(line) * out_n, static_cast<unsigned>(out_n) * sizeof(V));
+    const auto rg = buf::rsrc(G, PN * NF * 8u);
+#pragma unroll 1
+    for (int r = 0; r < PN; ++r) {
+        float2 z[EI];
+#pragma unroll
+        for (int k = 0; k < EI; ++k) z[k] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const float2 p = cmul(v[k], buf::ld2(rg, t * 8u, (r * NF + k * T) * 8u));
+            z[k % EI] = make_float2(z[k % EI].x + p.x, z[k % EI].y + p.y);
+        }
+        if constexpr (D == 1)
+            fft::transform<NI, EI, true>(z, L, TW, t);
+        else
+            fft::transform<NI, EI, true>(z, L, twI, t);
+#pragma unroll
+        for (int k = 0; k < EI; ++k) {
+            const int j = t + k * T;
+            const int o = PN * (s0 * Lm + j - KD) + r;
+            if constexpr (BL == 1) {
+                buf::st2(ry, cscale(z[k], sc), j >= KD ? o * 8u : buf::kOff, 0u);
+            } else {
+                buf::st1(ry, z[k].x * sc, j >= KD ? o * 4u : buf::kOff, 0u);
+                buf::st1(ry, z[k].y * sc, (j >= KD && s0 + 1 < nseg) ? (o + PN * Lm) * 4u : buf::kOff, 0u);
+            }
+        }
+    }
+}
+
 thread_local ThreadWorkspace t_spec_ws;
 
 template <int D, class V>
@@ -332,6 +433,119 @@ int fir_fft_run(const float* h, int K, const V* x, int lines, long long n, int D
     if (!rc) *done = true;
     return rc;
 }
+
+namespace {
+
+constexpr int kPolySegMinPhaseTaps = 64;
+thread_local ThreadWorkspace t_poly_spec_ws;
+
+template <int D, int PN, class V>
+int launch_fir_poly(const float* h, int K, int up, const PolyTaps& pt, int KD, int c, const V* x, int lines,
+                    long long n, long long out_n, V* y, hipStream_t st)
+{
+    constexpr int NF = 4096, NI = NF / D;
+    constexpr int BL = Lane2<V>::BLOCKS;
+    using C = RowCfg<NF>;
+    const int Lm = NI - KD;
+    void* ws = nullptr;
+    WorkspaceLease lease_(t_poly_spec_ws, st);
+    int rc = lease_.get(sizeof(float2) * NF * PN, &ws);
+    if (rc) return rc;
+    float2* G = static_cast<float2*>(ws);
+    const float2 *twF = nullptr, *twI = nullptr;
+    if ((rc = twiddles(NF, &twF)) || (rc = twiddles(NI, &twI))) return rc;
+    hipLaunchKernelGGL(fir_poly_spectra_kernel<NF>, dim3(PN), dim3(C::THREADS), C::LDS_BYTES, st, h, K, up, pt, twF,
+                       G);
+    ZKX_LAUNCH_CHECK();
+    const long long T_count = (out_n + PN - 1) / PN;  // outputs per residue (at most)
+    const long long nseg = (T_count + Lm - 1) / Lm;
+    const long long rpl = (nseg + BL - 1) / BL;
+    const long long rows = static_cast<long long>(lines) * rpl;
+    if (rows >= (1LL << 31) || n * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
+        (out_n + PN * static_cast<long long>(Lm) + PN) * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
+        (nseg + 1) * Lm * D + NF + c >= (1LL << 31))
+        ZKX_FAIL(ZKX_INVALID_SHAPE, "frequency-domain FIR: line too long for 32-bit indexing");
+    constexpr size_t lds = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<NF>::ENTRIES;
+    hipLaunchKernelGGL((fir_poly_kernel<D, PN, V>), dim3(static_cast<unsigned>(rows)), dim3(256), lds, st, x,
+                       static_cast<int>(rpl), static_cast<int>(n), static_cast<int>(out_n), KD, Lm,
+                       static_cast<int>(nseg), c, G, twF, twI, y);
+    ZKX_LAUNCH_CHECK();
+    return ZKX_SUCCESS;
+}
+
+template <int D, class V>
+int dispatch_fir_poly(int PN, const float* h, int K, int up, const PolyTaps& pt, int KD, int c, const V* x,
+                      int lines, long long n, long long out_n, V* y, hipStream_t st)
+{
+    switch (PN) {
+    case 2: return launch_fir_poly<D, 2, V>(h, K, up, pt, KD, c, x, lines, n, out_n, y, st);
+    case 3: return launch_fir_poly<D, 3, V>(h, K, up, pt, KD, c, x, lines, n, out_n, y, st);
+    default: return launch_fir_poly<D, 4, V>(h, K, up, pt, KD, c, x, lines, n, out_n, y, st);
+    }
+}
+
+}  // namespace
+
+// The rate-changing segment form for `lines` contiguous lines:
+// y[o] = sum_k h[k] x_up[o*down + m_off - k], x_up[u] = x[u/up] when up | u.
+// Sets *done = false (and does nothing) unless up/g in {2, 3, 4} and
+// down/g in {1, 2, 4} (g = gcd(up, down)), the phases hold at least 64
+// taps, the history fits 1/8 of a segment and the line holds at least one
+// segment of outputs.
+template <class V>
+int fir_poly_run(const float* h, int K, int up, int down, long long m_off, const V* x, int lines, long long n,
+                 long long out_n, V* y, hipStream_t st, bool* done)
+{
+    *done = false;
+    if (g_fir_fft != 2 || lines < 1 || n < 1 || out_n < 1 || m_off < 0) return ZKX_SUCCESS;
+    int g = up, b = down;
+    while (b) {
+        const int tmp = g % b;
+        g = b;
+        b = tmp;
+    }
+    const int PN = up / g, D = down / g;
+    if (PN < 2 || PN > 4 || !(D == 1 || D == 2 || D == 4)) return ZKX_SUCCESS;
+    // the direct kernels' work grows with the taps per phase, this path's
+    // does not: at 41 taps per phase (resample 3/2, 121 taps) the direct
+    // kernel is still faster (142 vs 154 us on 16 x 2^20 complex64)
+    const int Jmax = (K + up - 1) / up;
+    if (Jmax < kPolySegMinPhaseTaps) return ZKX_SUCCESS;
+    PolyTaps pt{};
+    long long Q[4], c = 0;
+    for (int r = 0; r < PN; ++r) {
+        const long long m = static_cast<long long>(r) * down + m_off;
+        pt.phase[r] = static_cast<int>(m % up);
+        Q[r] = m / up;
+        c = std::max(c, Q[r]);
+    }
+    long long Kg = 1;
+    for (int r = 0; r < PN; ++r) {
+        const long long sh = c - Q[r];
+        if (sh > 4096) return ZKX_SUCCESS;
+        pt.shift[r] = static_cast<int>(sh);
+        const long long taps_r = pt.phase[r] < K ? (K - pt.phase[r] + up - 1) / up : 0;
+        Kg = std::max(Kg, sh + taps_r);
+    }
+    const long long KD = (Kg - 1 + D - 1) / D;
+    if (KD * D * 8 > 4096 || c >= (1LL << 30) || out_n < static_cast<long long>(PN) * (4096 / D)) return ZKX_SUCCESS;
+    int rc;
+    switch (D) {
+    case 1: rc = dispatch_fir_poly<1, V>(PN, h, K, up, pt, static_cast<int>(KD), static_cast<int>(c), x, lines, n,
+                                         out_n, y, st); break;
+    case 2: rc = dispatch_fir_poly<2, V>(PN, h, K, up, pt, static_cast<int>(KD), static_cast<int>(c), x, lines, n,
+                                         out_n, y, st); break;
+    default: rc = dispatch_fir_poly<4, V>(PN, h, K, up, pt, static_cast<int>(KD), static_cast<int>(c), x, lines, n,
+                                          out_n, y, st); break;
+    }
+    if (!rc) *done = true;
+    return rc;
+}
+
+template int fir_poly_run<float>(const float*, int, int, int, long long, const float*, int, long long, long long,
+                                 float*, hipStream_t, bool*);
+template int fir_poly_run<float2>(const float*, int, int, int, long long, const float2*, int, long long, long long,
+                                  float2*, hipStream_t, bool*);
 
 template int fir_fft_run<float>(const float*, int, const float*, int, long long, int, long long, float*, hipStream_t,
                                 bool*);

@@ -33,6 +33,9 @@ namespace zkx {
 template <class V>
 int fir_fft_run(const float* h, int K, const V* x, int lines, long long n, int D, long long out_n, V* y,
                 hipStream_t st, bool* done);
+template <class V>
+int fir_poly_run(const float* h, int K, int up, int down, long long m_off, const V* x, int lines, long long n,
+                 long long out_n, V* y, hipStream_t st, bool* done);
 
 // Long up = 1 filters on contiguous lines take a frequency-domain path
 // (fir_fft.hip): 2 the 4096-point segment form where it applies (D in
@@ -354,6 +357,11 @@ template <class V>
 int run_polyphase(const float* h, const V* x, V* y, const PolyPlan& p, int outer, int inner, hipStream_t st)
 {
     if (p.n_out <= 0 || outer <= 0 || inner <= 0) return ZKX_SUCCESS;
+    if (inner == 1 && p.up > 1) {  // long rate-changing filters: the segment form (fir_fft.hip)
+        bool done = false;
+        const int rc = fir_poly_run<V>(h, p.K, p.up, p.down, p.m_off, x, outer, p.n_in, p.n_out, y, st, &done);
+        if (rc || done) return rc;
+    }
     if (inner == 1) {
         bool done = false;
         int rc;
