@@ -347,9 +347,8 @@ int launch_fir_seg(const float* h, int K, const V* x, int lines, long long n, lo
     if (rows >= (1LL << 31) || n * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
         out_n * static_cast<long long>(sizeof(V)) >= (1LL << 31) || (nseg + 1) * Lm * D + NF >= (1LL << 31))
         ZKX_FAIL(ZKX_INVALID_SHAPE, "frequency-domain FIR: line too long for 32-bit indexing");
-    constexpr size_t lds = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<NF>::ENTRIES;
-    ZKX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fir_seg_kernel<D, V>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    constexpr size_t lds = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<NF>::ENTRIES;  // 36.7 KB
+    static_assert(lds <= 64 * 1024, "default dynamic LDS limit");
     hipLaunchKernelGGL((fir_seg_kernel<D, V>), dim3(static_cast<unsigned>(rows)), dim3(256), lds, st, x,
                        static_cast<int>(rpl), static_cast<int>(n), static_cast<int>(out_n), KD, Lm,
                        static_cast<int>(nseg), Hs, twF, twI, y);
@@ -466,6 +465,7 @@ int launch_fir_poly(const float* h, int K, int up, const PolyTaps& pt, int KD, i
         (nseg + 1) * Lm * D + NF + c >= (1LL << 31))
         ZKX_FAIL(ZKX_INVALID_SHAPE, "frequency-domain FIR: line too long for 32-bit indexing");
     constexpr size_t lds = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<NF>::ENTRIES;
+    static_assert(lds <= 64 * 1024, "default dynamic LDS limit");
     hipLaunchKernelGGL((fir_poly_kernel<D, PN, V>), dim3(static_cast<unsigned>(rows)), dim3(256), lds, st, x,
                        static_cast<int>(rpl), static_cast<int>(n), static_cast<int>(out_n), KD, Lm,
                        static_cast<int>(nseg), c, G, twF, twI, y);
