@@ -73,11 +73,13 @@ int zkx_max_lds_fft_length(void);
  * jobs from one queue; "dwell_fused_rows", "dwell_fused_chunks" and
  * "dwell_fused_lead" set its job sizes and lead), 1 = two launches (the
  * default, faster on this GPU).  "fir_fft" selects the method for long
- * up = 1 filters on contiguous lines (upfirdn, firfilter): 2 (the default)
- * 4096-point overlap-save segments with the spectrum folded for decimation
- * by D in {1, 2, 4, 8}, other D by 1; 1 one transform per polyphase phase;
- * 0 the direct polyphase kernels; these agree to FP32 rounding, not bit
- * for bit.  One more name is a
+ * filters on contiguous lines (upfirdn, resample_poly, firfilter): 2 (the
+ * default) 4096-point overlap-save segments with the spectrum folded for
+ * decimation by D in {1, 2, 4, 8} when up = 1, and for rate changes with
+ * up/g in {2, 3, 4}, down/g in {1, 2, 4} and 64 or more taps per phase
+ * (g = gcd(up, down)), otherwise as 1; 1 one transform per polyphase phase
+ * for up = 1, the direct kernels otherwise; 0 the direct polyphase kernels;
+ * these agree to FP32 rounding, not bit for bit.  One more name is a
  * test switch, not a variant: "inject_fft_failure" = 1 makes the transform
  * stage of pulse_compression / pulse_doppler return ZKX_BACKEND_ERROR
  * before any device work (the reference's synthesized backend status for
