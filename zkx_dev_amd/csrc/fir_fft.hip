@@ -323,6 +323,11 @@ fir_poly_kernel(const V* __restrict__ x, int rpl, int n, int out_n, int KD, int 
 
 thread_local ThreadWorkspace t_spec_ws;
 
+// Launchers of the segment forms return kNotTaken, before any device work,
+// when a line is too long for their 32-bit offsets; the caller then takes
+// the direct kernels.
+constexpr int kNotTaken = -1;
+
 template <int D, class V>
 int launch_fir_seg(const float* h, int K, const V* x, int lines, long long n, long long out_n, V* y, hipStream_t st)
 {
@@ -331,6 +336,13 @@ int launch_fir_seg(const float* h, int K, const V* x, int lines, long long n, lo
     using C = RowCfg<NF>;
     const int KD = (K - 1 + D - 1) / D;
     const int Lm = NI - KD;
+    const long long nseg = (out_n + Lm - 1) / Lm;
+    const long long rpl = (nseg + BL - 1) / BL;
+    const long long rows = static_cast<long long>(lines) * rpl;
+    // 32-bit offsets within a line, row index and segment starts
+    if (rows >= (1LL << 31) || n * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
+        (out_n + Lm) * static_cast<long long>(sizeof(V)) >= (1LL << 31) || (nseg + 1) * Lm * D + NF >= (1LL << 31))
+        return kNotTaken;
     void* ws = nullptr;
     WorkspaceLease lease_(t_spec_ws, st);
     int rc = lease_.get(sizeof(float2) * NF, &ws);
@@ -340,13 +352,6 @@ int launch_fir_seg(const float* h, int K, const V* x, int lines, long long n, lo
     if ((rc = twiddles(NF, &twF)) || (rc = twiddles(NI, &twI))) return rc;
     hipLaunchKernelGGL(fir_spectrum_kernel<NF>, dim3(1), dim3(C::THREADS), C::LDS_BYTES, st, h, K, twF, Hs);
     ZKX_LAUNCH_CHECK();
-    const long long nseg = (out_n + Lm - 1) / Lm;
-    const long long rpl = (nseg + BL - 1) / BL;
-    const long long rows = static_cast<long long>(lines) * rpl;
-    // 32-bit offsets within a line, row index and segment starts
-    if (rows >= (1LL << 31) || n * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
-        out_n * static_cast<long long>(sizeof(V)) >= (1LL << 31) || (nseg + 1) * Lm * D + NF >= (1LL << 31))
-        ZKX_FAIL(ZKX_INVALID_SHAPE, "frequency-domain FIR: line too long for 32-bit indexing");
     constexpr size_t lds = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<NF>::ENTRIES;  // 36.7 KB
     static_assert(lds <= 64 * 1024, "default dynamic LDS limit");
     hipLaunchKernelGGL((fir_seg_kernel<D, V>), dim3(static_cast<unsigned>(rows)), dim3(256), lds, st, x,
@@ -414,6 +419,7 @@ int fir_fft_run(const float* h, int K, const V* x, int lines, long long n, int D
         case 4: rc = launch_fir_seg<4, V>(h, K, x, lines, n, out_n, y, st); break;
         default: rc = launch_fir_seg<8, V>(h, K, x, lines, n, out_n, y, st); break;
         }
+        if (rc == kNotTaken) return ZKX_SUCCESS;
         if (!rc) *done = true;
         return rc;
     }
@@ -446,6 +452,14 @@ int launch_fir_poly(const float* h, int K, int up, const PolyTaps& pt, int KD, i
     constexpr int BL = Lane2<V>::BLOCKS;
     using C = RowCfg<NF>;
     const int Lm = NI - KD;
+    const long long T_count = (out_n + PN - 1) / PN;  // outputs per residue (at most)
+    const long long nseg = (T_count + Lm - 1) / Lm;
+    const long long rpl = (nseg + BL - 1) / BL;
+    const long long rows = static_cast<long long>(lines) * rpl;
+    if (rows >= (1LL << 31) || n * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
+        (out_n + PN * static_cast<long long>(Lm) + PN) * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
+        (nseg + 1) * Lm * D + NF + c >= (1LL << 31))
+        return kNotTaken;
     void* ws = nullptr;
     WorkspaceLease lease_(t_poly_spec_ws, st);
     int rc = lease_.get(sizeof(float2) * NF * PN, &ws);
@@ -456,14 +470,6 @@ int launch_fir_poly(const float* h, int K, int up, const PolyTaps& pt, int KD, i
     hipLaunchKernelGGL(fir_poly_spectra_kernel<NF>, dim3(PN), dim3(C::THREADS), C::LDS_BYTES, st, h, K, up, pt, twF,
                        G);
     ZKX_LAUNCH_CHECK();
-    const long long T_count = (out_n + PN - 1) / PN;  // outputs per residue (at most)
-    const long long nseg = (T_count + Lm - 1) / Lm;
-    const long long rpl = (nseg + BL - 1) / BL;
-    const long long rows = static_cast<long long>(lines) * rpl;
-    if (rows >= (1LL << 31) || n * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
-        (out_n + PN * static_cast<long long>(Lm) + PN) * static_cast<long long>(sizeof(V)) >= (1LL << 31) ||
-        (nseg + 1) * Lm * D + NF + c >= (1LL << 31))
-        ZKX_FAIL(ZKX_INVALID_SHAPE, "frequency-domain FIR: line too long for 32-bit indexing");
     constexpr size_t lds = C::LDS_BYTES + sizeof(float2) * fft::Pass16Table<NF>::ENTRIES;
     static_assert(lds <= 64 * 1024, "default dynamic LDS limit");
     hipLaunchKernelGGL((fir_poly_kernel<D, PN, V>), dim3(static_cast<unsigned>(rows)), dim3(256), lds, st, x,
@@ -538,6 +544,7 @@ int fir_poly_run(const float* h, int K, int up, int down, long long m_off, const
     default: rc = dispatch_fir_poly<4, V>(PN, h, K, up, pt, static_cast<int>(KD), static_cast<int>(c), x, lines, n,
                                           out_n, y, st); break;
     }
+    if (rc == kNotTaken) return ZKX_SUCCESS;
     if (!rc) *done = true;
     return rc;
 }
