@@ -695,7 +695,9 @@ PROBED = {
     "cfar_rowscan_tile_kernel": ("cfar_rowscan_tile_kernel", 64 * P * R * 8),
     "cfar_threshold_rows_kernel": ("cfar_threshold_rows_kernel", 64 * P * R * 13),
     "polyphase_kernel (resample_poly 3/2, 121 taps)": ("polyphase_kernel", 16 * (1 << 20) * 8 + 16 * 1572864 * 8),
-    "fir_fft_kernel (upfirdn 1/4 c64 + firfilter f32, mean)": ("fir_fft_kernel", None),
+    "fir_seg_kernel<4, float2> (upfirdn 1/4 c64, 255 taps)": ("fir_seg_kernel<4",
+                                                               16 * (1 << 20) * 8 + 16 * 262208 * 8),
+    "fir_seg_kernel<1, float> (firfilter 2^24 f32, 255 taps)": ("fir_seg_kernel<1", (1 << 24) * 8),
     "ols_kernel (fftconvolve same, 1025 taps)": ("ols_kernel", 16 * (1 << 20) * 16),
     # the engine's kernels on the 256-dwell uneven batch (bytes: see uneven_alg_bytes)
     "pc_engine_kernel (256 uneven dwells)": ("pc_engine_kernel", "pc"),
